@@ -1,0 +1,174 @@
+/*
+ * kmsd.h - C ABI of the MI355X (gfx950) stochastic KPM transport library.
+ *
+ * Drop-in boundary for the reference's compiled CPU kernel, the pybind11
+ * module kubo_msd._native built from cpp/native.cpp (module definition at
+ * cpp/native.cpp:613-720).  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *   - Dense blocks are (N, nb) row-major with interleaved complex entries,
+ *     exactly the reference's Dense layout (cpp/native.cpp:26-41).  Vector
+ *     blocks passed to compute calls are DEVICE pointers owned by the caller
+ *     (e.g. torch tensors); the library never frees caller memory.
+ *   - dtype selects the vector precision: KMSD_C128 (complex128, the
+ *     reference's only type) or KMSD_C64 (complex64 storage, FP64 arithmetic
+ *     in registers and FP64 reductions).
+ *   - Matrices and metric data are HOST arrays at creation time and are
+ *     copied to device memory; nothing is retained from the caller.
+ *   - Calls are ordered on the context's stream.  Calls that must report a
+ *     numerical status (solves, series, chains, moments) synchronize that
+ *     stream before returning.
+ *   - Return codes: KMSD_OK, KMSD_EINVAL (reference: std::invalid_argument ->
+ *     ValueError), KMSD_ENUMERIC (reference: std::runtime_error /
+ *     MetricFailure -> RuntimeError with per-column diagnostics),
+ *     KMSD_ERESOURCE (message starts with "RESOURCE_BLOCKED"), KMSD_EDEVICE.
+ *     kmsd_last_error() gives the message of the last failure in the context.
+ */
+#ifndef KMSD_H
+#define KMSD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { KMSD_OK = 0, KMSD_EINVAL = 1, KMSD_ENUMERIC = 2, KMSD_ERESOURCE = 3, KMSD_EDEVICE = 4 };
+enum { KMSD_C64 = 1, KMSD_C128 = 2 };
+enum { KMSD_CSR = 0, KMSD_BSR4 = 1 };
+enum { KMSD_PRECOND_DIAGONAL = 0, KMSD_PRECOND_BLOCK_JACOBI = 1 };
+enum { KMSD_MU_FP32 = 0, KMSD_MU_BF16X3 = 1, KMSD_MU_FP64 = 2 };
+
+typedef struct kmsd_ctx kmsd_ctx;
+typedef struct kmsd_matrix kmsd_matrix;
+typedef struct kmsd_metric kmsd_metric;
+typedef struct kmsd_engine kmsd_engine;
+
+/* Per-column metric-solve report (reference SolveReport, cpp/native.cpp:123-143).
+ * Arrays are caller-owned host arrays of length nb; any may be NULL. */
+typedef struct {
+    uint8_t *converged;
+    uint8_t *breakdown;
+    int64_t *iterations;
+    double *true_residual;
+    double *relative_residual;
+} kmsd_report;
+
+/* ---- context --------------------------------------------------------- */
+/* No reference counterpart (the reference CPU kernel has no device).  Binds
+ * a HIP device and stream (NULL = null stream) and an HBM budget in bytes
+ * (0 = 85% of free device memory) that every library allocation is charged
+ * against, as the reference GPU edition's require_bytes does
+ * (kubo_msd_gpu/src/kubo_msd/backends/gpu_context.py:128-136). */
+int kmsd_ctx_create(int device, uint64_t hbm_budget_bytes, void *stream, kmsd_ctx **out);
+int kmsd_ctx_destroy(kmsd_ctx *ctx);
+int kmsd_ctx_set_stream(kmsd_ctx *ctx, void *stream);
+int kmsd_ctx_synchronize(kmsd_ctx *ctx);
+/* bytes currently charged to the budget and the budget itself */
+int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
+/* Launch tuning: key "groups" (workgroups per launch, multiple of 8),
+ * "cpl" (complex columns per lane for complex64 blocks: 1 or 2),
+ * "moment_chunk" (Chebyshev orders per partial-reduction launch). */
+int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
+/* Message of the last failure in ctx (or of the last failed create when ctx
+ * is NULL).  Valid until the next call on ctx. */
+const char *kmsd_last_error(const kmsd_ctx *ctx);
+/* Reference: build_info() (cpp/native.cpp:631-639).  Writes a short
+ * NUL-terminated description into buf. */
+int kmsd_build_info(char *buf, int64_t len);
+
+/* ---- matrices -------------------------------------------------------- */
+/* Reference: CSR(indptr:int64[n+1], indices:int64[nnz], data:c128[nnz], n,
+ * threads) (cpp/native.cpp:54-74, 640-646).  Same validation: monotone
+ * pointers starting at 0 and ending at nnz, indices in [0,n), finite values.
+ * values: complex128 interleaved (2*nnz doubles).  Purely real or purely
+ * imaginary matrices are detected and stored as one real array. */
+int kmsd_matrix_create(kmsd_ctx *ctx, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
+                       const int64_t *colidx, const double *values, kmsd_matrix **out);
+int kmsd_matrix_destroy(kmsd_matrix *m);
+/* Reference: CSR.matmul(B) (cpp/native.cpp:75-89, 642-646): out = A B. */
+int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int dtype);
+
+/* ---- metric (overlap S) ---------------------------------------------- */
+/* Reference: Metric(S, diag:f64[n], blocks:[(a,b,L)], threads)
+ * (cpp/native.cpp:212-236, 647-648).  diag: host f64[n] (> 0, finite).
+ * nblocks contiguous atom blocks [block_ptr[q], block_ptr[q+1]) with lower
+ * Cholesky factors concatenated row-major (complex128 interleaved);
+ * nblocks = 0 selects the diagonal preconditioner.  An S that is exactly the
+ * identity is detected; its solves return the right-hand side (the
+ * reference's PCG converges there in one step with alpha = 1). */
+int kmsd_metric_create(kmsd_ctx *ctx, kmsd_matrix *S, const double *diag, int64_t nblocks,
+                       const int64_t *block_ptr, const double *factors, kmsd_metric **out);
+int kmsd_metric_destroy(kmsd_metric *m);
+/* Reference: Metric.set_residual_policy(policy, interval, gap)
+ * (cpp/native.cpp:206-210): adaptive 0 = "fixed", 1 = "adaptive". */
+int kmsd_metric_set_residual_policy(kmsd_metric *m, int adaptive, int interval, double gap);
+/* Reference: Metric.solve(B, rtol, atol, maxiter) -> (Y, report)
+ * (cpp/native.cpp:313-452, 649-654).  Batched PCG, one independent system per
+ * column, per-column RHS scaling, true-residual refresh every `interval`
+ * iterations, final original-equation acceptance.  Returns KMSD_ENUMERIC with
+ * the report filled when a column fails. */
+int kmsd_metric_solve(kmsd_metric *m, const void *B, void *Y, int64_t nb, int dtype, double rtol, double atol,
+                      int maxiter, kmsd_report *report);
+/* Reference: Metric.sample(xi, coeff, center, half) (cpp/native.cpp:453-479):
+ * out = Z p(W) xi with W = Z S Z, Z = diag^-1/2, p by Clenshaw. */
+int kmsd_metric_sample(kmsd_metric *m, const void *xi, void *out, int64_t nb, int dtype, const double *coeff,
+                       int64_t ncoeff, double center, double half);
+/* Reference: Metric.diagnostics() counters (cpp/native.cpp:666-675):
+ * out[0] calls, [1] sum of max iterations, [2] refreshes, [3] residual
+ * checks, [4] restarts; *max_relative. */
+int kmsd_metric_counters(const kmsd_metric *m, int64_t *out5, double *max_relative);
+
+/* ---- engine ----------------------------------------------------------- */
+/* Reference: Engine(H, Metric, [(kind, [CSR...]) x3], rtol, atol, maxiter,
+ * hbar) (cpp/native.cpp:488-503, 676-677).  kinds[3] per Cartesian axis:
+ * -1 absent, 0 explicit velocity V (1 matrix), 1 position X (1 matrix),
+ * 2 composite bond/xbar/delta (3 matrices).  mats[3*axis + k]. */
+int kmsd_engine_create(kmsd_ctx *ctx, kmsd_matrix *H, kmsd_metric *metric, const int *kinds,
+                       kmsd_matrix *const *mats, double rtol, double atol, int maxiter, double hbar,
+                       kmsd_engine **out);
+int kmsd_engine_destroy(kmsd_engine *e);
+/* Solve controls used by the engine's actions (reference
+ * v_cov_with_controls copies the engine with new controls,
+ * cpp/native.cpp:688-697). */
+int kmsd_engine_set_controls(kmsd_engine *e, double rtol, double atol, int maxiter);
+/* Reference: read-only h_calls / v_calls (cpp/native.cpp:718-719). */
+int kmsd_engine_counters(const kmsd_engine *e, int64_t *h_calls, int64_t *v_calls);
+/* Report of the engine's last failed metric solve (nb columns). */
+int kmsd_engine_last_report(const kmsd_engine *e, kmsd_report *report, int64_t *nb);
+/* Reference: Engine.h_action(B) = S^-1 H B (cpp/native.cpp:505, 678-682). */
+int kmsd_h_action(kmsd_engine *e, const void *B, void *out, int64_t nb, int dtype);
+/* Reference: Engine.v_cov(B, dir[3]) (cpp/native.cpp:506-533, 683-687). */
+int kmsd_v_cov(kmsd_engine *e, const double *dir3, const void *B, void *out, int64_t nb, int dtype);
+/* Reference: Engine.series(B, coeff, center, half, dirs)
+ * (cpp/native.cpp:540-573, 698-705): out = sum_k c_k T_k(A) B where A is the
+ * scaled action, augmented with the commutator source (-i hbar/half)
+ * S^-1 V_d B0 in column block d+1 when ndirs > 0.  coeff: complex128
+ * interleaved host array; dirs: host f64[ndirs][3]. */
+int kmsd_series(kmsd_engine *e, const void *B, void *out, int64_t width, int dtype, const double *coeff,
+                int64_t ncoeff, double center, double half, const double *dirs, int64_t ndirs);
+/* Reference: Engine.chain(B, L, center, half) -> (L, N, nb)
+ * (cpp/native.cpp:574-589, 706-711).  out: device (L, N, nb) block. */
+int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, int dtype, double center,
+               double half);
+/* Reference: Engine.moments(B, L, center, half) -> f64 (L, nb)
+ * (cpp/native.cpp:590-609, 712-717): mu[l][j] = Re <S B_j, T_l(A) B_j>.
+ * mu: DEVICE f64 (L, nb). */
+int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
+                 double half);
+
+/* ---- spectral double moments -------------------------------------------- */
+/* Reference: einsum('jnb,knb->bjk', left.conj(), cov) (src/kubo_msd/solvers/
+ * spectral.py:37; GPU edition solvers/spectral.py:65).  left: (m, N, nb),
+ * right: (k, N, nb) device blocks; table: device complex128 (nb, m, k),
+ * table[b][j][q] = sum_i conj(left[j][i][b]) right[q][i][b].
+ * accumulate != 0 adds into table (chunked N).  mode: KMSD_MU_FP64 (FP64
+ * vector arithmetic), KMSD_MU_FP32 (MFMA FP32), KMSD_MU_BF16X3 (MFMA bf16
+ * three-term split). */
+int kmsd_mu_table(kmsd_ctx *ctx, const void *left, const void *right, void *table, int64_t n, int64_t m,
+                  int64_t k, int64_t nb, int dtype, int mode, int accumulate);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

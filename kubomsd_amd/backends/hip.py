@@ -1,0 +1,444 @@
+"""ctypes binding of libkmsd.so (include/kmsd.h) on PyTorch-ROCm device tensors.
+
+The classes mirror the reference's compiled-kernel module ``kubo_msd._native``
+(cpp/native.cpp:613-720): ``HipMatrix`` ~ ``CSR``, ``HipMetric`` ~ ``Metric``,
+``HipEngine`` ~ ``Engine``, with the same method names, argument meaning and
+error behaviour (``NumericalError`` carrying per-column ``diagnostics``,
+message fragments "finite", "representable", "RESOURCE_BLOCKED").  Vector
+blocks are torch tensors on the HIP device, complex128 (the reference's type)
+or complex64 (FP32 storage, FP64 arithmetic and reductions).
+
+There is no CPU fallback: a missing library or device raises
+``BackendUnavailable``.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import scipy.sparse as sp
+
+from ..errors import BackendUnavailable, KuboMSDError, NumericalError, ResourceBlocked
+from ..units import HBAR_EV_FS
+
+LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libkmsd.so"
+KMSD_OK, KMSD_EINVAL, KMSD_ENUMERIC, KMSD_ERESOURCE, KMSD_EDEVICE = range(5)
+KMSD_C64, KMSD_C128 = 1, 2
+KMSD_CSR, KMSD_BSR4 = 0, 1
+KMSD_MU_FP32, KMSD_MU_BF16X3, KMSD_MU_FP64 = 0, 1, 2
+
+_lib = None
+
+# Exported entry points and their ctypes signatures (include/kmsd.h).
+_P, _I64, _D, _I, _U64 = C.c_void_p, C.c_int64, C.c_double, C.c_int, C.c_uint64
+SIGNATURES = {
+    "kmsd_ctx_create": [_I, _U64, _P, _P],
+    "kmsd_ctx_destroy": [_P],
+    "kmsd_ctx_set_stream": [_P, _P],
+    "kmsd_ctx_synchronize": [_P],
+    "kmsd_ctx_memory": [_P, _P, _P],
+    "kmsd_ctx_set_tuning": [_P, C.c_char_p, _I64],
+    "kmsd_last_error": [_P],
+    "kmsd_build_info": [_P, _I64],
+    "kmsd_matrix_create": [_P, _I, _I64, _I64, _P, _P, _P, _P],
+    "kmsd_matrix_destroy": [_P],
+    "kmsd_matrix_matmul": [_P, _P, _P, _I64, _I],
+    "kmsd_metric_create": [_P, _P, _P, _I64, _P, _P, _P],
+    "kmsd_metric_destroy": [_P],
+    "kmsd_metric_set_residual_policy": [_P, _I, _I, _D],
+    "kmsd_metric_solve": [_P, _P, _P, _I64, _I, _D, _D, _I, _P],
+    "kmsd_metric_sample": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D],
+    "kmsd_metric_counters": [_P, _P, _P],
+    "kmsd_engine_create": [_P, _P, _P, _P, _P, _D, _D, _I, _D, _P],
+    "kmsd_engine_destroy": [_P],
+    "kmsd_engine_set_controls": [_P, _D, _D, _I],
+    "kmsd_engine_counters": [_P, _P, _P],
+    "kmsd_engine_last_report": [_P, _P, _P],
+    "kmsd_h_action": [_P, _P, _P, _I64, _I],
+    "kmsd_v_cov": [_P, _P, _P, _P, _I64, _I],
+    "kmsd_series": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D, _P, _I64],
+    "kmsd_chain": [_P, _P, _P, _I64, _I64, _I, _D, _D],
+    "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
+    "kmsd_mu_table": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I],
+}
+
+
+class _Report(C.Structure):
+    _fields_ = [("converged", _P), ("breakdown", _P), ("iterations", _P),
+                ("true_residual", _P), ("relative_residual", _P)]
+
+
+def library():
+    """Loads libkmsd.so; raises BackendUnavailable (never falls back)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise BackendUnavailable(
+                f"HIP backend requires the native library {LIB_PATH}: run "
+                "python -c 'import __graft_entry__; __graft_entry__.build()' (hipcc, gfx950)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        lib.kmsd_last_error.restype = C.c_char_p
+        _lib = lib
+    return _lib
+
+
+def build_info():
+    buf = C.create_string_buffer(512)
+    library().kmsd_build_info(buf, 512)
+    import json
+    return json.loads(buf.value.decode())
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.complex128:
+        return KMSD_C128
+    if t.dtype == torch.complex64:
+        return KMSD_C64
+    raise KuboMSDError("vector blocks must be complex128 or complex64 tensors")
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _ReportBuffers:
+    def __init__(self, b):
+        self.conv = np.zeros(b, np.uint8)
+        self.brk = np.zeros(b, np.uint8)
+        self.its = np.zeros(b, np.int64)
+        self.res = np.zeros(b)
+        self.rel = np.zeros(b)
+        self.struct = _Report(*[_np_ptr(a) for a in (self.conv, self.brk, self.its, self.res, self.rel)])
+
+    def as_dict(self, b=None):
+        b = len(self.conv) if b is None else b
+        return dict(converged=[bool(x) for x in self.conv[:b]], breakdown=[bool(x) for x in self.brk[:b]],
+                    iterations=self.its[:b].tolist(), true_residual=self.res[:b].tolist(),
+                    relative_residual=self.rel[:b].tolist())
+
+
+class HipContext:
+    """A HIP device + stream + HBM budget (include/kmsd.h kmsd_ctx_*)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise BackendUnavailable("HIP backend requires a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = library()
+        self.device = int(device)
+        self.torch_device = torch.device("cuda", self.device)
+        h = C.c_void_p()
+        stream = torch.cuda.current_stream(self.torch_device).cuda_stream
+        rc = self.lib.kmsd_ctx_create(self.device, int(hbm_budget_bytes), C.c_void_p(stream), C.byref(h))
+        if rc:
+            self._raise(rc, self.lib.kmsd_last_error(None).decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.kmsd_ctx_destroy(self.h)
+            self.h = None
+
+    def bind_stream(self):
+        torch = _torch()
+        self.lib.kmsd_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream(self.torch_device).cuda_stream))
+
+    def error(self):
+        return self.lib.kmsd_last_error(self.h).decode()
+
+    @staticmethod
+    def _raise(rc, msg, diagnostics=None):
+        if rc == KMSD_ERESOURCE:
+            err = ResourceBlocked(msg)
+        elif rc == KMSD_EDEVICE:
+            err = NumericalError("HIP device error: " + msg)
+        else:
+            err = NumericalError(msg)
+        if diagnostics is not None:
+            err.diagnostics = diagnostics
+        raise err
+
+    def check(self, rc, diagnostics=None):
+        if rc:
+            self._raise(rc, self.error(), diagnostics)
+
+    def set_tuning(self, key, value):
+        self.check(self.lib.kmsd_ctx_set_tuning(self.h, key.encode(), int(value)))
+
+    def memory(self):
+        used, budget = C.c_uint64(), C.c_uint64()
+        self.lib.kmsd_ctx_memory(self.h, C.byref(used), C.byref(budget))
+        return dict(used_bytes=used.value, budget_bytes=budget.value)
+
+    def synchronize(self):
+        self.check(self.lib.kmsd_ctx_synchronize(self.h))
+
+    def block(self, B, name="block"):
+        """Validates a caller block: 2-D, contiguous, complex, on this device."""
+        torch = _torch()
+        if not isinstance(B, torch.Tensor):
+            raise KuboMSDError(f"{name} must be a device tensor")
+        if B.ndim != 2 or B.shape[0] < 1 or B.shape[1] < 1:
+            raise KuboMSDError("expected nonempty finite (N,batch) array")
+        if B.device != self.torch_device:
+            raise KuboMSDError(f"{name} must live on {self.torch_device}")
+        _dtype_code(B)
+        return B.contiguous()
+
+
+def _csr_host(matrix):
+    m = sp.csr_matrix(matrix, dtype=np.complex128, copy=True)
+    m.sum_duplicates()
+    m.sort_indices()
+    return m
+
+
+class HipMatrix:
+    """Device CSR matrix (reference CSR, cpp/native.cpp:54-90)."""
+
+    def __init__(self, ctx, matrix):
+        self.ctx = ctx
+        m = _csr_host(matrix)
+        if m.shape[0] != m.shape[1]:
+            raise KuboMSDError("invalid CSR dimensions/threads")
+        self.n = m.shape[0]
+        self.nnz = m.nnz
+        ptr = np.ascontiguousarray(m.indptr, np.int64)
+        idx = np.ascontiguousarray(m.indices, np.int64)
+        val = np.ascontiguousarray(m.data, np.complex128)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.kmsd_matrix_create(ctx.h, KMSD_CSR, self.n, self.nnz, _np_ptr(ptr), _np_ptr(idx),
+                                             _np_ptr(val), C.byref(h)))
+        self.h = h
+        self.host = m
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.kmsd_matrix_destroy(self.h)
+            self.h = None
+
+    def matmul(self, B):
+        ctx = self.ctx
+        B = ctx.block(B)
+        if B.shape[0] != self.n:
+            raise KuboMSDError("CSR input dimension mismatch")
+        ctx.bind_stream()
+        out = _torch().empty_like(B)
+        ctx.check(ctx.lib.kmsd_matrix_matmul(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             B.shape[1], _dtype_code(B)))
+        return out
+
+    def __matmul__(self, B):
+        return self.matmul(B)
+
+
+class HipMetric:
+    """Overlap metric with batched device PCG (reference Metric, cpp/native.cpp:150-480)."""
+
+    def __init__(self, ctx, S, diagonal, blocks=()):
+        self.ctx = ctx
+        self.S = S if isinstance(S, HipMatrix) else HipMatrix(ctx, S)
+        self.n = self.S.n
+        diag = np.ascontiguousarray(diagonal, np.float64)
+        if diag.shape != (self.n,):
+            raise KuboMSDError("invalid metric diagonal/threads")
+        blocks = list(blocks)
+        bptr = np.array([0] + [int(b) for (_, b, _) in blocks], np.int64)
+        fac = (np.concatenate([np.ascontiguousarray(L, np.complex128).ravel() for (_, _, L) in blocks])
+               if blocks else np.zeros(1, np.complex128))
+        for q, (a, b, L) in enumerate(blocks):
+            if int(a) != bptr[q]:
+                raise KuboMSDError("invalid block-Jacobi factor partition")
+        h = C.c_void_p()
+        ctx.check(ctx.lib.kmsd_metric_create(ctx.h, self.S.h, _np_ptr(diag), len(blocks), _np_ptr(bptr),
+                                             _np_ptr(fac), C.byref(h)))
+        self.h = h
+        self.diagonal = diag
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.kmsd_metric_destroy(self.h)
+            self.h = None
+
+    def set_residual_policy(self, policy, interval, gap):
+        if policy not in ("fixed", "adaptive"):
+            raise KuboMSDError("invalid residual policy")
+        self.ctx.check(self.ctx.lib.kmsd_metric_set_residual_policy(self.h, int(policy == "adaptive"),
+                                                                   int(interval), float(gap)))
+
+    def solve(self, B, rtol, atol, maxiter):
+        ctx = self.ctx
+        B = ctx.block(B)
+        ctx.bind_stream()
+        out = _torch().empty_like(B)
+        rep = _ReportBuffers(B.shape[1])
+        rc = ctx.lib.kmsd_metric_solve(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()), B.shape[1],
+                                       _dtype_code(B), float(rtol), float(atol), int(maxiter), C.byref(rep.struct))
+        ctx.check(rc, rep.as_dict())
+        return out, rep.as_dict()
+
+    def sample(self, xi, coeff, center, half):
+        ctx = self.ctx
+        xi = ctx.block(xi)
+        ctx.bind_stream()
+        out = _torch().empty_like(xi)
+        c = np.ascontiguousarray(coeff, np.float64)
+        ctx.check(ctx.lib.kmsd_metric_sample(self.h, C.c_void_p(xi.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             xi.shape[1], _dtype_code(xi), _np_ptr(c), len(c), float(center),
+                                             float(half)))
+        return out
+
+    def diagnostics(self):
+        v = np.zeros(5, np.int64)
+        mr = C.c_double()
+        self.ctx.lib.kmsd_metric_counters(self.h, _np_ptr(v), C.byref(mr))
+        return dict(calls=int(v[0]), max_iterations_sum=int(v[1]), true_residual_refreshes=int(v[2]),
+                    true_residual_checks=int(v[3]), residual_restarts=int(v[4]),
+                    max_relative_residual=mr.value)
+
+
+class HipEngine:
+    """H, metric and velocity actions with fused Chebyshev drivers
+    (reference Engine, cpp/native.cpp:488-610)."""
+
+    def __init__(self, ctx, H, metric, definitions, rtol, atol, maxiter, hbar=HBAR_EV_FS):
+        self.ctx = ctx
+        self.H = H if isinstance(H, HipMatrix) else HipMatrix(ctx, H)
+        self.metric = metric
+        self.n = self.H.n
+        kinds = np.array([int(k) for k, _ in definitions], np.int32)
+        if kinds.shape != (3,):
+            raise KuboMSDError("invalid engine operators")
+        ptrs = (C.c_void_p * 9)()
+        self._mats = []
+        for a, (k, mats) in enumerate(definitions):
+            for j, m in enumerate(mats):
+                m = m if isinstance(m, HipMatrix) else HipMatrix(ctx, m)
+                self._mats.append(m)
+                ptrs[3 * a + j] = m.h
+        h = C.c_void_p()
+        ctx.check(ctx.lib.kmsd_engine_create(ctx.h, self.H.h, metric.h, _np_ptr(kinds), ptrs, float(rtol),
+                                             float(atol), int(maxiter), float(hbar), C.byref(h)))
+        self.h = h
+        self.controls = (float(rtol), float(atol), int(maxiter))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.kmsd_engine_destroy(self.h)
+            self.h = None
+
+    def _counters(self):
+        h, v = C.c_int64(), C.c_int64()
+        self.ctx.lib.kmsd_engine_counters(self.h, C.byref(h), C.byref(v))
+        return h.value, v.value
+
+    @property
+    def h_calls(self):
+        return self._counters()[0]
+
+    @property
+    def v_calls(self):
+        return self._counters()[1]
+
+    def _check(self, rc):
+        if rc:
+            nb = C.c_int64()
+            self.ctx.lib.kmsd_engine_last_report(self.h, None, C.byref(nb))
+            diag = None
+            if rc == KMSD_ENUMERIC and nb.value:
+                rep = _ReportBuffers(nb.value)
+                self.ctx.lib.kmsd_engine_last_report(self.h, C.byref(rep.struct), C.byref(nb))
+                if not all(rep.conv):
+                    diag = rep.as_dict()
+            self.ctx.check(rc, diag)
+
+    def _prep(self, B):
+        B = self.ctx.block(B)
+        if B.shape[0] != self.n:
+            raise KuboMSDError("CSR input dimension mismatch")
+        self.ctx.bind_stream()
+        return B
+
+    def h_action(self, B):
+        B = self._prep(B)
+        out = _torch().empty_like(B)
+        self._check(self.ctx.lib.kmsd_h_action(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
+                                               B.shape[1], _dtype_code(B)))
+        return out
+
+    def v_cov(self, B, direction):
+        B = self._prep(B)
+        d = np.ascontiguousarray(direction, np.float64)
+        if d.shape != (3,):
+            raise KuboMSDError("direction must have three components")
+        out = _torch().empty_like(B)
+        self._check(self.ctx.lib.kmsd_v_cov(self.h, _np_ptr(d), C.c_void_p(B.data_ptr()),
+                                            C.c_void_p(out.data_ptr()), B.shape[1], _dtype_code(B)))
+        return out
+
+    def v_cov_with_controls(self, B, direction, rtol, atol, maxiter):
+        """Velocity action with temporary solve controls; the engine keeps its own."""
+        saved = self.controls
+        self.ctx.lib.kmsd_engine_set_controls(self.h, float(rtol), float(atol), int(maxiter))
+        try:
+            return self.v_cov(B, direction)
+        finally:
+            self.ctx.lib.kmsd_engine_set_controls(self.h, *saved)
+
+    def series(self, B, coeff, center, half, directions=()):
+        B = self._prep(B)
+        c = np.ascontiguousarray(coeff, np.complex128)
+        if c.ndim != 1:
+            raise KuboMSDError("Chebyshev coefficients must be 1D")
+        d = np.ascontiguousarray(np.asarray(directions, np.float64).reshape(-1, 3) if len(directions)
+                                 else np.zeros((0, 3)))
+        out = _torch().empty_like(B)
+        self._check(self.ctx.lib.kmsd_series(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             B.shape[1], _dtype_code(B), _np_ptr(c), len(c), float(center),
+                                             float(half), _np_ptr(d), d.shape[0]))
+        return out
+
+    def chain(self, B, length, center, half, out=None):
+        B = self._prep(B)
+        torch = _torch()
+        if out is None:
+            out = torch.empty((int(length),) + tuple(B.shape), dtype=B.dtype, device=B.device)
+        self._check(self.ctx.lib.kmsd_chain(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
+                                            int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
+        return out
+
+    def moments(self, B, length, center, half):
+        """Device float64 (length, nb): mu[l, j] = Re <S B_j, T_l B_j>."""
+        B = self._prep(B)
+        torch = _torch()
+        mu = torch.empty((int(length), B.shape[1]), dtype=torch.float64, device=B.device)
+        self._check(self.ctx.lib.kmsd_moments(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(mu.data_ptr()),
+                                              int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
+        return mu
+
+
+def mu_table(ctx, left, right, mode=KMSD_MU_FP64, table=None, accumulate=False):
+    """table[b, j, q] = sum_i conj(left[j, i, b]) right[q, i, b] (complex128, device)."""
+    torch = _torch()
+    if left.ndim != 3 or right.ndim != 3 or left.shape[1:] != right.shape[1:] or left.dtype != right.dtype:
+        raise KuboMSDError("left/right must be (m,N,nb) and (k,N,nb) blocks of one dtype")
+    left = left.contiguous()
+    right = right.contiguous()
+    m, n, nb = left.shape
+    k = right.shape[0]
+    if table is None:
+        table = torch.zeros((nb, m, k), dtype=torch.complex128, device=left.device)
+    ctx.bind_stream()
+    ctx.check(ctx.lib.kmsd_mu_table(ctx.h, C.c_void_p(left.data_ptr()), C.c_void_p(right.data_ptr()),
+                                    C.c_void_p(table.data_ptr()), n, m, k, nb, _dtype_code(left), int(mode),
+                                    int(bool(accumulate))))
+    return table

@@ -1,0 +1,152 @@
+// Host side of the C ABI declared in include/kmsd.h: device-resident
+// matrices, the overlap metric (batched PCG), the engine's fused Chebyshev
+// drivers and the launch-shape policy.  One HIP stream per context; every
+// library allocation is charged against the context's HBM budget.
+#include "kmsd.h"
+#include "kmsd_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace kmsd;
+
+namespace kmsd {
+thread_local std::string g_create_error;
+}
+
+// ---------------------------------------------------------------------------
+// context
+
+extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_ctx **out) {
+    if (!out) return KMSD_EINVAL;
+    *out = nullptr;
+    kmsd_ctx *c = new kmsd_ctx();
+    const int rc = guard(nullptr, [&] {
+        int count = 0;
+        HIPCHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Error(KMSD_EINVAL, "invalid HIP device index");
+        HIPCHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, device));
+        c->device = device;
+        c->cus = prop.multiProcessorCount;
+        c->stream = static_cast<hipStream_t>(stream);
+        size_t fr = 0, tot = 0;
+        HIPCHECK(hipMemGetInfo(&fr, &tot));
+        c->budget = budget ? budget : uint64_t(0.85 * double(fr));
+        c->groups = int64_t(std::max(8, c->cus * 8));
+        c->flag = static_cast<int32_t *>(dalloc(c, 256));
+    });
+    if (rc) {
+        g_create_error = c->err;
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
+    if (!c) return KMSD_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->flag) (void)hipFree(c->flag);
+    delete c;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_ctx_set_stream(kmsd_ctx *c, void *stream) {
+    if (!c) return KMSD_EINVAL;
+    c->stream = static_cast<hipStream_t>(stream);
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_ctx_synchronize(kmsd_ctx *c) {
+    if (!c) return KMSD_EINVAL;
+    return guard(c, [&] { HIPCHECK(hipStreamSynchronize(c->stream)); });
+}
+
+extern "C" int kmsd_ctx_memory(const kmsd_ctx *c, uint64_t *used, uint64_t *budget) {
+    if (!c) return KMSD_EINVAL;
+    if (used) *used = c->used;
+    if (budget) *budget = c->budget;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) {
+    if (!c || !key) return KMSD_EINVAL;
+    return guard(c, [&] {
+        const std::string k(key);
+        if (k == "groups") {
+            if (value < 8 || value % 8) throw Error(KMSD_EINVAL, "groups must be a positive multiple of 8");
+            c->groups = value;
+        } else if (k == "cpl") {
+            if (value != 1 && value != 2) throw Error(KMSD_EINVAL, "cpl must be 1 or 2");
+            c->cpl = int(value);
+        } else if (k == "moment_chunk") {
+            if (value < 1 || value > 4096) throw Error(KMSD_EINVAL, "moment_chunk must be in [1, 4096]");
+            c->moment_chunk = int(value);
+        } else if (k == "fp32_math") {
+            c->fp32_math = value != 0;
+        } else if (k == "pcg_poll") {
+            if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
+            c->pcg_poll = int(value);
+        } else {
+            throw Error(KMSD_EINVAL, "unknown tuning key");
+        }
+    });
+}
+
+extern "C" const char *kmsd_last_error(const kmsd_ctx *c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int kmsd_build_info(char *buf, int64_t len) {
+    if (!buf || len < 1) return KMSD_EINVAL;
+    std::snprintf(buf, size_t(len),
+                  "{\"language\":\"HIP C++17\",\"target\":\"gfx950\",\"precision\":[\"complex128\",\"complex64+fp64 "
+                  "reductions\"],\"compiler\":\"%s\"}",
+                  __VERSION__);
+    return KMSD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// matrices
+
+extern "C" int kmsd_matrix_create(kmsd_ctx *c, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
+                                  const int64_t *colidx, const double *values, kmsd_matrix **out) {
+    if (!c || !out) return KMSD_EINVAL;
+    *out = nullptr;
+    kmsd_matrix *m = new kmsd_matrix();
+    m->ctx = c;
+    const int rc = guard(c, [&] { matrix_upload(c, m, fmt, n, nnz, rowptr, colidx, values); });
+    if (rc) {
+        matrix_release(m);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_matrix_destroy(kmsd_matrix *m) {
+    if (!m) return KMSD_OK;
+    matrix_release(m);
+    delete m;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int dtype) {
+    if (!m) return KMSD_EINVAL;
+    kmsd_ctx *c = m->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        require_finite(c, dtype, B, nb, m->n, nb);
+        spmm(c, dtype, m, B, nb, out, nb, nb, 1.0, 0.0, false);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}

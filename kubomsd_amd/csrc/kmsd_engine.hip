@@ -1,0 +1,432 @@
+// Engine drivers: scaled action, velocity actions, the augmented Chebyshev
+// series, stored chains and fused moment chains (reference Engine,
+// cpp/native.cpp:488-610).  Everything stays on the device; the host only
+// sequences launches.  With an identity metric every Chebyshev order is ONE
+// fused launch (CSR product + three-term update + store + moment partials).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kmsd_internal.hpp"
+
+using namespace kmsd;
+
+namespace {
+
+int64_t n_of(const kmsd_engine *e) { return e->H->n; }
+
+// OUT = alpha*(S^-1 H X) + beta*X + gamma*P  (+ moment partials / accumulate)
+// `s` carries everything except the operator.
+void cheb(kmsd_engine *e, int dtype, Step s) {
+    kmsd_ctx *c = e->ctx;
+    ++e->h_calls;
+    if (e->metric->identity) {
+        s.A = e->H;
+        launch_step(c, dtype, s);
+        return;
+    }
+    const size_t bytes = size_t(s.n) * size_t(s.w) * elem_bytes(dtype);
+    void *hx = e->tmp.ensure(c, bytes);
+    void *z = e->tmp2.ensure(c, bytes);
+    spmm(c, dtype, e->H, s.X, s.ldx, hx, s.w, s.w, 1.0, 0.0, false);
+    metric_solve(e->metric, dtype, hx, s.w, z, s.w, s.w, e->rtol, e->atol, e->maxiter);
+    s.A = nullptr;
+    s.Z = z;
+    s.ldz = s.w;
+    launch_step(c, dtype, s);
+}
+
+// out (+)= coef * Y where Y = V_axis-action(X) ; helpers for v_cov
+void add_scaled(kmsd_ctx *c, int dtype, const void *Y, int64_t ldy, void *out, int64_t ldo, int64_t n, int32_t w,
+                Cplx coef, bool accumulate) {
+    Step s;
+    s.Z = Y;
+    s.ldz = ldy;
+    s.alpha = coef;
+    s.n = n;
+    s.w = w;
+    if (accumulate) {
+        s.ACC = out;
+        s.lda = ldo;
+    } else {
+        s.OUT = out;
+        s.ldo = ldo;
+    }
+    launch_step(c, dtype, s);
+}
+
+// out (+)= sum_a dir[a] * v_a X   (reference Engine::v_cov / v_cart)
+// Scratch: e->tmp, tmp2 (solves inside), tmp3.
+void vcov(kmsd_engine *e, int dtype, const double *dir, const void *X, int64_t ldx, void *out, int64_t ldo, int32_t w,
+          Cplx outer, bool accumulate) {
+    kmsd_ctx *c = e->ctx;
+    const int64_t n = n_of(e);
+    ++e->v_calls;
+    bool wrote = accumulate;
+    const size_t bytes = size_t(n) * size_t(w) * elem_bytes(dtype);
+    for (int ax = 0; ax < 3; ++ax) {
+        if (dir[ax] == 0.0) continue;
+        const int kind = e->kind[ax];
+        kmsd_matrix *const *m = e->mats[ax];
+        const Cplx coef{outer.re * dir[ax], outer.im * dir[ax]};
+        if (kind == 0 && m[0]) {
+            spmm(c, dtype, m[0], X, ldx, out, ldo, w, coef.re, coef.im, wrote);
+            wrote = true;
+            continue;
+        }
+        if (!((kind == 1 && m[0]) || (kind == 2 && m[0] && m[1] && m[2])))
+            throw Error(KMSD_EINVAL, "missing Cartesian velocity");
+        void *hx = e->tmp.ensure(c, bytes);
+        void *sx = e->tmp2.ensure(c, bytes);
+        void *acc = e->tmp3.ensure(c, bytes);
+        const double ih = 1.0 / e->hbar;  // (i/hbar) folded into the final coefficient
+        if (kind == 1) {
+            // (i/hbar)(H S^-1 X B - X S^-1 H B)
+            spmm(c, dtype, m[0], X, ldx, hx, w, w, 1.0, 0.0, false);
+            metric_solve(e->metric, dtype, hx, w, sx, w, w, e->rtol, e->atol, e->maxiter);
+            spmm(c, dtype, e->H, sx, w, acc, w, w, 1.0, 0.0, false);
+            spmm(c, dtype, e->H, X, ldx, hx, w, w, 1.0, 0.0, false);
+            metric_solve(e->metric, dtype, hx, w, sx, w, w, e->rtol, e->atol, e->maxiter);
+            spmm(c, dtype, m[0], sx, w, acc, w, w, -1.0, 0.0, true);
+        } else {
+            // (i/hbar)(bond B - xbar S^-1 H B + H S^-1 delta B)
+            spmm(c, dtype, m[0], X, ldx, acc, w, w, 1.0, 0.0, false);
+            spmm(c, dtype, e->H, X, ldx, hx, w, w, 1.0, 0.0, false);
+            metric_solve(e->metric, dtype, hx, w, sx, w, w, e->rtol, e->atol, e->maxiter);
+            spmm(c, dtype, m[1], sx, w, acc, w, w, -1.0, 0.0, true);
+            spmm(c, dtype, m[2], X, ldx, hx, w, w, 1.0, 0.0, false);
+            metric_solve(e->metric, dtype, hx, w, sx, w, w, e->rtol, e->atol, e->maxiter);
+            spmm(c, dtype, e->H, sx, w, acc, w, w, 1.0, 0.0, true);
+        }
+        // coef * (i/hbar) * acc
+        add_scaled(c, dtype, acc, w, out, ldo, n, w, Cplx{-coef.im * ih, coef.re * ih}, wrote);
+        wrote = true;
+    }
+    if (!wrote) launch_scale(c, dtype, X, ldx, out, ldo, n, w, Cplx{0, 0});
+}
+
+// Augmented action (reference Engine::augmented, cpp/native.cpp:540-554), in
+// three-term form: OUT = f*(A X) - f*c/a X [+ gamma P] where block d+1 also
+// receives f*(-i hbar/a) S^-1 v_d X0.  f = 1 for the first order, 2 after.
+void augmented_step(kmsd_engine *e, int dtype, const void *X, const void *P, void *OUT, int64_t width, int32_t nb,
+                    double f, double center, double half, const double *dirs, int64_t ndirs) {
+    kmsd_ctx *c = e->ctx;
+    const int64_t n = n_of(e);
+    Step s;
+    s.X = X;
+    s.ldx = width;
+    s.OUT = OUT;
+    s.ldo = width;
+    s.alpha = Cplx{f / half, 0};
+    s.beta = Cplx{-f * center / half, 0};
+    s.use_beta = true;
+    if (P) {
+        s.P = P;
+        s.ldp = width;
+        s.gamma = -1.0;
+        s.use_p = true;
+    }
+    s.n = n;
+    s.w = int32_t(width);
+    cheb(e, dtype, s);
+    const Cplx src{0, -f * e->hbar / half};
+    for (int64_t d = 0; d < ndirs; ++d) {
+        void *blk = static_cast<char *>(OUT) + size_t(d + 1) * nb * elem_bytes(dtype);
+        const double *dir = dirs + 3 * d;
+        bool simple = e->metric->identity;
+        for (int ax = 0; ax < 3; ++ax)
+            if (dir[ax] != 0.0 && e->kind[ax] != 0) simple = false;
+        if (simple) {
+            // S = I and explicit velocities: accumulate V X0 straight into the block.
+            vcov(e, dtype, dir, X, width, blk, width, nb, src, true);
+            continue;
+        }
+        const size_t bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
+        void *v = e->t2.ensure(c, bytes);
+        void *sv = e->t3.ensure(c, bytes);
+        vcov(e, dtype, dir, X, width, v, nb, nb, Cplx{1, 0}, false);
+        metric_solve(e->metric, dtype, v, nb, sv, nb, nb, e->rtol, e->atol, e->maxiter);
+        add_scaled(c, dtype, sv, nb, blk, width, n, nb, src, true);
+    }
+}
+
+void check_half(double half) {
+    if (!(half > 0)) throw Error(KMSD_EINVAL, "spectral half-width must be positive");
+}
+
+}  // namespace
+
+extern "C" int kmsd_engine_create(kmsd_ctx *c, kmsd_matrix *H, kmsd_metric *metric, const int *kinds,
+                                  kmsd_matrix *const *mats, double rtol, double atol, int maxiter, double hbar,
+                                  kmsd_engine **out) {
+    if (!c || !out) return KMSD_EINVAL;
+    *out = nullptr;
+    kmsd_engine *e = new kmsd_engine();
+    const int rc = guard(c, [&] {
+        if (!H || !metric || !kinds || H->n != metric->n) throw Error(KMSD_EINVAL, "invalid engine operators");
+        if (H->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "engine H must be CSR");
+        e->ctx = c;
+        e->H = H;
+        e->metric = metric;
+        for (int a = 0; a < 3; ++a) {
+            e->kind[a] = kinds[a];
+            const int cnt = kinds[a] == 2 ? 3 : (kinds[a] == 0 || kinds[a] == 1 ? 1 : 0);
+            if (kinds[a] < -1 || kinds[a] > 2) throw Error(KMSD_EINVAL, "invalid velocity kind");
+            for (int k = 0; k < cnt; ++k) {
+                kmsd_matrix *m = mats ? mats[3 * a + k] : nullptr;
+                if (!m || m->n != H->n || m->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "invalid engine operators");
+                e->mats[a][k] = m;
+            }
+        }
+        e->rtol = rtol;
+        e->atol = atol;
+        e->maxiter = maxiter;
+        e->hbar = hbar;
+    });
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_engine_destroy(kmsd_engine *e) {
+    delete e;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_engine_set_controls(kmsd_engine *e, double rtol, double atol, int maxiter) {
+    if (!e) return KMSD_EINVAL;
+    e->rtol = rtol;
+    e->atol = atol;
+    e->maxiter = maxiter;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_engine_counters(const kmsd_engine *e, int64_t *h, int64_t *v) {
+    if (!e) return KMSD_EINVAL;
+    if (h) *h = e->h_calls;
+    if (v) *v = e->v_calls;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_engine_last_report(const kmsd_engine *e, kmsd_report *r, int64_t *nb) {
+    if (!e) return KMSD_EINVAL;
+    const kmsd_metric *m = e->metric;
+    const size_t b = m->rep_conv.size();
+    if (nb) *nb = int64_t(b);
+    if (r) {
+        if (r->converged) std::copy(m->rep_conv.begin(), m->rep_conv.end(), r->converged);
+        if (r->breakdown) std::copy(m->rep_brk.begin(), m->rep_brk.end(), r->breakdown);
+        if (r->iterations) std::copy(m->rep_its.begin(), m->rep_its.end(), r->iterations);
+        if (r->true_residual) std::copy(m->rep_res.begin(), m->rep_res.end(), r->true_residual);
+        if (r->relative_residual) std::copy(m->rep_rel.begin(), m->rep_rel.end(), r->relative_residual);
+    }
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_h_action(kmsd_engine *e, const void *B, void *out, int64_t nb, int dtype) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        const int64_t n = n_of(e);
+        require_finite(c, dtype, B, nb, n, int32_t(nb));
+        ++e->h_calls;
+        if (e->metric->identity) {
+            spmm(c, dtype, e->H, B, nb, out, nb, int32_t(nb), 1.0, 0.0, false);
+        } else {
+            void *hx = e->tmp.ensure(c, size_t(n) * nb * elem_bytes(dtype));
+            spmm(c, dtype, e->H, B, nb, hx, nb, int32_t(nb), 1.0, 0.0, false);
+            metric_solve(e->metric, dtype, hx, nb, out, nb, int32_t(nb), e->rtol, e->atol, e->maxiter);
+        }
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int kmsd_v_cov(kmsd_engine *e, const double *dir, const void *B, void *out, int64_t nb, int dtype) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        if (!dir) throw Error(KMSD_EINVAL, "direction must have three components");
+        require_finite(c, dtype, B, nb, n_of(e), int32_t(nb));
+        vcov(e, dtype, dir, B, nb, out, nb, int32_t(nb), Cplx{1, 0}, false);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int kmsd_series(kmsd_engine *e, const void *B, void *out, int64_t width, int dtype, const double *coeff,
+                           int64_t ncoeff, double center, double half, const double *dirs, int64_t ndirs) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, width, dtype);
+        if (ncoeff < 1 || !coeff) throw Error(KMSD_EINVAL, "empty Chebyshev coefficients");
+        const int64_t n = n_of(e);
+        require_finite(c, dtype, B, width, n, int32_t(width));
+        launch_scale(c, dtype, B, width, out, width, n, int32_t(width), Cplx{coeff[0], coeff[1]});
+        if (ncoeff == 1) {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            return;
+        }
+        check_half(half);
+        if (ndirs < 0 || (ndirs && (!dirs || width % (ndirs + 1))))
+            throw Error(KMSD_EINVAL, "invalid augmented block columns");
+        const int32_t nb = int32_t(width / (ndirs + 1));
+        const size_t bytes = size_t(n) * size_t(width) * elem_bytes(dtype);
+        void *u0 = e->t0.ensure(c, bytes);
+        void *u1 = e->t1.ensure(c, bytes);
+        augmented_step(e, dtype, B, nullptr, u0, width, nb, 1.0, center, half, dirs, ndirs);
+        add_scaled(c, dtype, u0, width, out, width, n, int32_t(width), Cplx{coeff[2], coeff[3]}, true);
+        const void *prev = B;
+        void *cur = u0;
+        void *spare = u1;
+        for (int64_t k = 2; k < ncoeff; ++k) {
+            void *nxt = (prev == B) ? spare : const_cast<void *>(prev);
+            augmented_step(e, dtype, cur, prev, nxt, width, nb, 2.0, center, half, dirs, ndirs);
+            add_scaled(c, dtype, nxt, width, out, width, n, int32_t(width), Cplx{coeff[2 * k], coeff[2 * k + 1]}, true);
+            prev = cur;
+            cur = nxt;
+        }
+        if (launch_nonfinite(c, dtype, out, width, n, int32_t(width)))
+            throw Error(KMSD_ENUMERIC, "nonfinite Chebyshev propagation; check bounds and solves");
+    });
+}
+
+extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, int dtype, double center,
+                          double half) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        if (L < 1) throw Error(KMSD_EINVAL, "Chebyshev chain must be nonempty");
+        const int64_t n = n_of(e);
+        require_finite(c, dtype, B, nb, n, int32_t(nb));
+        const size_t stride = size_t(n) * size_t(nb) * elem_bytes(dtype);
+        char *o = static_cast<char *>(out);
+        launch_scale(c, dtype, B, nb, o, nb, n, int32_t(nb), Cplx{1, 0});
+        if (L > 1) {
+            check_half(half);
+            Step s;
+            s.X = B;
+            s.ldx = nb;
+            s.OUT = o + stride;
+            s.ldo = nb;
+            s.alpha = Cplx{1 / half, 0};
+            s.beta = Cplx{-center / half, 0};
+            s.use_beta = true;
+            s.n = n;
+            s.w = int32_t(nb);
+            cheb(e, dtype, s);
+        }
+        for (int64_t k = 2; k < L; ++k) {
+            Step s;
+            s.X = o + size_t(k - 1) * stride;
+            s.ldx = nb;
+            s.P = o + size_t(k - 2) * stride;
+            s.ldp = nb;
+            s.OUT = o + size_t(k) * stride;
+            s.ldo = nb;
+            s.alpha = Cplx{2 / half, 0};
+            s.beta = Cplx{-2 * center / half, 0};
+            s.gamma = -1.0;
+            s.use_beta = s.use_p = true;
+            s.n = n;
+            s.w = int32_t(nb);
+            cheb(e, dtype, s);
+        }
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
+                            double center, double half) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, mu, nb, dtype);
+        if (L < 1) throw Error(KMSD_EINVAL, "moment count must be positive");
+        const int64_t n = n_of(e);
+        const int32_t w = int32_t(nb);
+        require_finite(c, dtype, B, nb, n, w);
+        const size_t bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
+        const void *W = B;
+        if (!e->metric->identity) {
+            void *sb = e->wbuf.ensure(c, bytes);
+            spmm(c, dtype, e->metric->S, B, nb, sb, nb, w, 1.0, 0.0, false);
+            W = sb;
+        }
+        const int64_t G = step_groups(c, n, w, dtype);
+        const int64_t chunk = std::min<int64_t>(c->moment_chunk, L);
+        const size_t slot_doubles = size_t(G) * size_t(w) * 2;
+        double *part = static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
+        int64_t slot = 0, base = 0;
+        auto flush = [&] {
+            launch_reduce(c, part, G, w, slot, mu + base * nb, nb, 0);
+            base += slot;
+            slot = 0;
+        };
+        auto next_part = [&] {
+            if (slot == chunk) flush();
+            return part + size_t(slot++) * slot_doubles;
+        };
+        {
+            Step s;
+            s.Z = B;
+            s.ldz = nb;
+            s.W = W;
+            s.ldw = nb;
+            s.n = n;
+            s.w = w;
+            s.part = next_part();
+            launch_step(c, dtype, s);
+        }
+        if (L > 1) {
+            check_half(half);
+            void *t1 = e->t0.ensure(c, bytes);
+            void *spare = e->t1.ensure(c, bytes);
+            Step s;
+            s.X = B;
+            s.ldx = nb;
+            s.OUT = t1;
+            s.ldo = nb;
+            s.alpha = Cplx{1 / half, 0};
+            s.beta = Cplx{-center / half, 0};
+            s.use_beta = true;
+            s.W = W;
+            s.ldw = nb;
+            s.n = n;
+            s.w = w;
+            s.part = next_part();
+            cheb(e, dtype, s);
+            const void *prev = B;
+            void *cur = t1;
+            for (int64_t k = 2; k < L; ++k) {
+                void *o = (prev == B) ? spare : const_cast<void *>(prev);
+                Step q;
+                q.X = cur;
+                q.ldx = nb;
+                q.P = prev;
+                q.ldp = nb;
+                q.OUT = o;
+                q.ldo = nb;
+                q.alpha = Cplx{2 / half, 0};
+                q.beta = Cplx{-2 * center / half, 0};
+                q.gamma = -1.0;
+                q.use_beta = q.use_p = true;
+                q.W = W;
+                q.ldw = nb;
+                q.n = n;
+                q.w = w;
+                q.part = next_part();
+                cheb(e, dtype, q);
+                prev = cur;
+                cur = o;
+            }
+        }
+        flush();
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}

@@ -1,0 +1,198 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kmsd.h"
+
+struct kmsd_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, used = 0;
+    std::string err;
+    int64_t groups = 2048;  // workgroups per row-walking launch (multiple of 8)
+    int cpl = 0;            // 0 = automatic
+    int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
+    bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
+    int pcg_poll = 4;       // PCG iterations between host status polls
+    int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
+};
+
+// Value kinds of a stored matrix.
+enum { VK_REAL = 0, VK_IMAG = 1, VK_COMPLEX = 2 };
+
+struct kmsd_matrix {
+    kmsd_ctx *ctx = nullptr;
+    int fmt = KMSD_CSR;
+    int64_t n = 0, nnz = 0;
+    int32_t *ptr = nullptr, *idx = nullptr;
+    int vkind = VK_COMPLEX;
+    void *val64 = nullptr;  // double[nnz] or cx<double>[nnz]
+    void *val32 = nullptr;  // float[nnz] or cx<float>[nnz]
+    bool identity = false;  // exactly the identity matrix
+    size_t bytes = 0;
+    // BSR 4x4 (fmt == KMSD_BSR4): block rows nbr = n/4; ptr/idx over blocks;
+    // values are 16 per block, row-major within the block.
+};
+
+namespace kmsd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                           \
+    do {                                                                                                      \
+        hipError_t e_ = (x);                                                                                  \
+        if (e_ != hipSuccess) throw ::kmsd::Error(KMSD_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+extern thread_local std::string g_create_error;
+
+template <class F>
+int guard(kmsd_ctx *c, F &&f) {
+    try {
+        f();
+        if (c) c->err.clear();
+        return KMSD_OK;
+    } catch (const Error &e) {
+        if (c) c->err = e.what();
+        else g_create_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (c) c->err = "host allocation failed";
+        return KMSD_ERESOURCE;
+    } catch (const std::exception &e) {
+        if (c) c->err = e.what();
+        return KMSD_EDEVICE;
+    }
+}
+
+void *dalloc(kmsd_ctx *c, size_t bytes);
+void dfree(kmsd_ctx *c, void *p, size_t bytes);
+
+// Grow-only device workspace charged to the context budget.
+struct DBuf {
+    kmsd_ctx *c = nullptr;
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *ensure(kmsd_ctx *ctx, size_t want);
+    void release();
+    ~DBuf() { release(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+inline size_t elem_bytes(int dtype) { return dtype == KMSD_C64 ? 8 : 16; }
+void check_block_args(const void *B, const void *out, int64_t nb, int dtype);
+
+void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
+                   const int64_t *colidx, const double *values);
+void matrix_release(kmsd_matrix *m);
+
+// ---- launches (kmsd_launch.hip) -------------------------------------------
+struct Cplx {
+    double re = 0, im = 0;
+};
+
+// Generic fused row step:
+//   y   = A X (A != null) or Z
+//   t   = alpha*y + beta*X + gamma*P
+//   OUT = t (store) ; part += conj(W) t (W != null) ; ACC += coef*t (ACC != null)
+// part: complex FP64 partial sums, [groups][w] per step slot.
+struct Step {
+    const kmsd_matrix *A = nullptr;
+    const void *X = nullptr;
+    int64_t ldx = 0;
+    const void *Z = nullptr;
+    int64_t ldz = 0;
+    const void *P = nullptr;
+    int64_t ldp = 0;
+    void *OUT = nullptr;
+    int64_t ldo = 0;
+    const void *W = nullptr;
+    int64_t ldw = 0;
+    void *ACC = nullptr;
+    int64_t lda = 0;
+    double *part = nullptr;
+    Cplx alpha{1, 0}, beta{0, 0}, coef{1, 0};
+    double gamma = 0;
+    bool use_beta = false, use_p = false;
+    int64_t n = 0;
+    int32_t w = 0;
+};
+// Launches one fused step; returns the number of workgroups used (the
+// partial-sum row count) so callers can size/reduce partials.
+int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s);
+// Number of workgroups launch_step will use for (n, w, dtype).
+int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype);
+// out[s*ldo + j] = sum_g part[s][g][j]; mode 0 real part (double), 1 complex.
+void launch_reduce(kmsd_ctx *c, const double *part, int64_t groups, int32_t w, int64_t nsteps, double *out,
+                   int64_t ldo, int mode);
+void launch_scale(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, void *O, int64_t ldo, int64_t n, int32_t w,
+                  Cplx coef);
+bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
+void launch_convert(kmsd_ctx *c, int dtype_in, const void *X, int64_t ldx, int dtype_out, void *O, int64_t ldo,
+                    int64_t n, int32_t w);
+
+// out = alpha * (A X) [+ out if accumulate]
+void spmm(kmsd_ctx *c, int dtype, const kmsd_matrix *A, const void *X, int64_t ldx, void *out, int64_t ldo,
+          int32_t w, double alpha_re, double alpha_im, bool accumulate);
+void require_finite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
+
+}  // namespace kmsd
+
+// ---- metric / engine (kmsd_metric.hip, kmsd_engine.hip) --------------------
+struct kmsd_metric {
+    kmsd_ctx *ctx = nullptr;
+    kmsd_matrix *S = nullptr;
+    bool identity = false;
+    int64_t n = 0;
+    double *diag = nullptr;  // device f64[n]
+    // block-Jacobi: contiguous blocks, device ptr + factors
+    int64_t nblocks = 0;
+    int32_t max_block = 0;
+    int32_t *block_ptr = nullptr;    // device int32[nblocks+1]
+    int64_t *factor_off = nullptr;   // device int64[nblocks] offsets into factors (complex entries)
+    double *factors = nullptr;       // device cx<double>[sum m^2]
+    std::vector<int64_t> host_block_ptr;
+    bool uniform4 = false;           // every block is 4x4 aligned to multiples of 4
+    size_t bytes = 0;
+    bool adaptive = false;
+    int check_interval = 32;
+    double gap_ratio = 0.1;
+    int64_t calls = 0, iterations = 0, refreshes = 0, residual_checks = 0, restarts = 0;
+    double max_relative = 0;
+    // last report (host)
+    std::vector<uint8_t> rep_conv, rep_brk;
+    std::vector<int64_t> rep_its;
+    std::vector<double> rep_res, rep_rel;
+    struct Work;
+    Work *work = nullptr;
+};
+
+struct kmsd_engine {
+    kmsd_ctx *ctx = nullptr;
+    kmsd_matrix *H = nullptr;
+    kmsd_metric *metric = nullptr;
+    int kind[3] = {-1, -1, -1};
+    kmsd_matrix *mats[3][3] = {};
+    double rtol = 1e-10, atol = 0, hbar = 0.6582119569509067;
+    int maxiter = 1000;
+    int64_t h_calls = 0, v_calls = 0;
+    kmsd::DBuf t0, t1, t2, t3, tmp, tmp2, tmp3, part, wbuf;
+};
+
+namespace kmsd {
+// Metric solve on device blocks (any dtype); throws Error(KMSD_ENUMERIC) on
+// failure with the report stored in the metric.
+void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb,
+                  double rtol, double atol, int maxiter);
+}  // namespace kmsd

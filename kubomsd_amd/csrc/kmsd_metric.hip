@@ -1,0 +1,168 @@
+// Overlap metric: device-resident S, diagonal / atom-block Jacobi factors,
+// batched PCG (reference Metric, cpp/native.cpp:145-480).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kmsd_internal.hpp"
+#include "kmsd_pcg.hpp"
+
+using namespace kmsd;
+
+extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *diag, int64_t nblocks,
+                                  const int64_t *block_ptr, const double *factors, kmsd_metric **out) {
+    if (!c || !out) return KMSD_EINVAL;
+    *out = nullptr;
+    kmsd_metric *m = new kmsd_metric();
+    m->ctx = c;
+    const int rc = guard(c, [&] {
+        if (!S || !diag || S->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "invalid metric diagonal/threads");
+        const int64_t n = S->n;
+        m->S = S;
+        m->n = n;
+        for (int64_t i = 0; i < n; ++i)
+            if (!(diag[i] > 0) || !std::isfinite(diag[i])) throw Error(KMSD_EINVAL, "invalid metric diagonal");
+        if (nblocks < 0 || (nblocks && (!block_ptr || !factors)))
+            throw Error(KMSD_EINVAL, "invalid block-Jacobi factor partition");
+        size_t nfac = 0;
+        bool uniform4 = nblocks > 0;
+        int32_t maxb = 0;
+        for (int64_t q = 0; q < nblocks; ++q) {
+            const int64_t a = block_ptr[q], b = block_ptr[q + 1];
+            if ((q == 0 && a != 0) || b <= a || b > n) throw Error(KMSD_EINVAL, "invalid block-Jacobi factor partition");
+            if (b - a > 64) throw Error(KMSD_EINVAL, "block-Jacobi blocks larger than 64 orbitals are not supported");
+            nfac += size_t((b - a) * (b - a));
+            uniform4 = uniform4 && (b - a == 4);
+            maxb = std::max<int32_t>(maxb, int32_t(b - a));
+        }
+        if (nblocks && block_ptr[nblocks] != n) throw Error(KMSD_EINVAL, "incomplete block-Jacobi partition");
+        for (size_t e = 0; e < 2 * nfac; ++e)
+            if (!std::isfinite(factors[e])) throw Error(KMSD_EINVAL, "invalid block-Jacobi factor partition");
+        bool diag_one = true;
+        for (int64_t i = 0; i < n && diag_one; ++i) diag_one = diag[i] == 1.0;
+        m->identity = S->identity && diag_one && nblocks == 0;
+        m->nblocks = nblocks;
+        m->uniform4 = uniform4;
+        m->max_block = maxb;
+        m->diag = static_cast<double *>(dalloc(c, sizeof(double) * n));
+        m->bytes = sizeof(double) * n;
+        HIPCHECK(hipMemcpyAsync(m->diag, diag, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if (nblocks) {
+            std::vector<int32_t> bp(nblocks + 1);
+            std::vector<int64_t> off(nblocks);
+            size_t o = 0;
+            for (int64_t q = 0; q <= nblocks; ++q) bp[q] = int32_t(block_ptr[q]);
+            for (int64_t q = 0; q < nblocks; ++q) {
+                off[q] = int64_t(o);
+                o += size_t((block_ptr[q + 1] - block_ptr[q]) * (block_ptr[q + 1] - block_ptr[q]));
+            }
+            m->host_block_ptr.assign(block_ptr, block_ptr + nblocks + 1);
+            m->block_ptr = static_cast<int32_t *>(dalloc(c, sizeof(int32_t) * bp.size()));
+            m->factor_off = static_cast<int64_t *>(dalloc(c, sizeof(int64_t) * off.size()));
+            m->factors = static_cast<double *>(dalloc(c, sizeof(double) * 2 * nfac));
+            m->bytes += sizeof(int32_t) * bp.size() + sizeof(int64_t) * off.size() + sizeof(double) * 2 * nfac;
+            HIPCHECK(hipMemcpyAsync(m->block_ptr, bp.data(), sizeof(int32_t) * bp.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            HIPCHECK(hipMemcpyAsync(m->factor_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            HIPCHECK(hipMemcpyAsync(m->factors, factors, sizeof(double) * 2 * nfac, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+    if (rc) {
+        kmsd_metric_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_metric_destroy(kmsd_metric *m) {
+    if (!m) return KMSD_OK;
+    kmsd_ctx *c = m->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    pcg_release(m);
+    const size_t nfac_bytes = m->bytes;  // accounted as a whole
+    if (m->diag) (void)hipFree(m->diag);
+    if (m->block_ptr) (void)hipFree(m->block_ptr);
+    if (m->factor_off) (void)hipFree(m->factor_off);
+    if (m->factors) (void)hipFree(m->factors);
+    c->used -= std::min<uint64_t>(c->used, nfac_bytes);
+    delete m;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_metric_set_residual_policy(kmsd_metric *m, int adaptive, int interval, double gap) {
+    if (!m) return KMSD_EINVAL;
+    return guard(m->ctx, [&] {
+        if (interval < 1 || !(gap > 0 && gap < 1)) throw Error(KMSD_EINVAL, "invalid residual policy");
+        m->adaptive = adaptive != 0;
+        m->check_interval = interval;
+        m->gap_ratio = gap;
+    });
+}
+
+static void copy_report(const kmsd_metric *m, kmsd_report *r) {
+    if (!r) return;
+    if (r->converged) std::copy(m->rep_conv.begin(), m->rep_conv.end(), r->converged);
+    if (r->breakdown) std::copy(m->rep_brk.begin(), m->rep_brk.end(), r->breakdown);
+    if (r->iterations) std::copy(m->rep_its.begin(), m->rep_its.end(), r->iterations);
+    if (r->true_residual) std::copy(m->rep_res.begin(), m->rep_res.end(), r->true_residual);
+    if (r->relative_residual) std::copy(m->rep_rel.begin(), m->rep_rel.end(), r->relative_residual);
+}
+
+extern "C" int kmsd_metric_solve(kmsd_metric *m, const void *B, void *Y, int64_t nb, int dtype, double rtol,
+                                 double atol, int maxiter, kmsd_report *report) {
+    if (!m) return KMSD_EINVAL;
+    kmsd_ctx *c = m->ctx;
+    const int rc = guard(c, [&] {
+        check_block_args(B, Y, nb, dtype);
+        require_finite(c, dtype, B, nb, m->n, int32_t(nb));
+        metric_solve(m, dtype, B, nb, Y, nb, int32_t(nb), rtol, atol, maxiter);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+    copy_report(m, report);
+    return rc;
+}
+
+extern "C" int kmsd_metric_counters(const kmsd_metric *m, int64_t *o, double *max_relative) {
+    if (!m) return KMSD_EINVAL;
+    if (o) {
+        o[0] = m->calls;
+        o[1] = m->iterations;
+        o[2] = m->refreshes;
+        o[3] = m->residual_checks;
+        o[4] = m->restarts;
+    }
+    if (max_relative) *max_relative = m->max_relative;
+    return KMSD_OK;
+}
+
+namespace kmsd {
+
+void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb,
+                  double rtol, double atol, int maxiter) {
+    if (!std::isfinite(rtol) || !std::isfinite(atol) || rtol < 0 || atol < 0 || maxiter < 1)
+        throw Error(KMSD_EINVAL, "invalid metric RHS shape/tolerances");
+    if (m->identity) {
+        // S = I: the reference's PCG converges in one iteration with alpha = 1
+        // and returns B up to the /safe*safe rescale; the block is copied.
+        if (B != Y || ldb != ldy) launch_scale(m->ctx, dtype, B, ldb, Y, ldy, m->n, nb, Cplx{1, 0});
+        m->rep_conv.assign(nb, 1);
+        m->rep_brk.assign(nb, 0);
+        m->rep_its.assign(nb, 0);
+        m->rep_res.assign(nb, 0.0);
+        m->rep_rel.assign(nb, 0.0);
+        ++m->calls;
+        return;
+    }
+    pcg_solve(m, dtype, B, ldb, Y, ldy, nb, rtol, atol, maxiter);
+}
+
+}  // namespace kmsd
+
+extern "C" int kmsd_metric_sample(kmsd_metric *m, const void *xi, void *out, int64_t nb, int dtype, const double *coeff,
+                                  int64_t ncoeff, double center, double half) {
+    if (!m) return KMSD_EINVAL;
+    return guard(m->ctx, [&] { throw Error(KMSD_EINVAL, "metric sampling is not available in this build"); });
+}

@@ -1,0 +1,652 @@
+// FP64 CPU oracle for the stochastic KPM transport path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product (kubomsd_amd/) may load
+// this library: it is the checker the HIP path is compared against, and the
+// CPU baseline that bench.py times beside the GPU.
+//
+// It restates, step for step, the reference's compiled CPU kernel
+// (cpp/native.cpp): complex128 arithmetic, row-parallel CSR products with a
+// static OpenMP schedule, column reductions serial in row order, and a full
+// PCG metric solve on every H action, including S = I.  Loop orders and the
+// order of every floating-point accumulation follow the reference so that a
+// single-thread run reproduces it to the last bit (pinned by tests against
+// oracle/_ref, a build of the reference's own source).
+//
+// Exposed through a plain C ABI (ctypes in oracle/native.py).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+using cplx = std::complex<double>;
+using cvec = std::vector<cplx>;
+
+thread_local std::string g_error;
+
+// Dense (N, b) row-major block of complex columns (cpp/native.cpp:26-41).
+struct Block {
+    int64_t n = 0, b = 0;
+    cvec v;
+    Block() = default;
+    Block(int64_t rows, int64_t cols) : n(rows), b(cols), v(size_t(rows) * cols) {}
+};
+
+bool finite(cplx z) { return std::isfinite(z.real()) && std::isfinite(z.imag()); }
+
+// CSR with int64 pointers/indices and complex128 values (cpp/native.cpp:54-90).
+struct Csr {
+    int64_t n = 0;
+    std::vector<int64_t> ptr, idx;
+    cvec val;
+    int threads = 1;
+
+    Block apply(const Block &x) const {
+        if (x.n != n) throw std::invalid_argument("CSR input dimension mismatch");
+        Block y(n, x.b);
+        const int64_t b = x.b;
+        // Serial below 2048 entries, as the reference does (cpp/native.cpp:78-79).
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 2048 && threads > 1)
+        for (int64_t row = 0; row < n; ++row) {
+            cplx *out = y.v.data() + row * b;
+            for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) {
+                const cplx *in = x.v.data() + idx[k] * b;
+                const cplx a = val[k];
+                for (int64_t j = 0; j < b; ++j) out[j] += a * in[j];
+            }
+        }
+        return y;
+    }
+};
+
+struct SolveReport {
+    std::vector<uint8_t> converged, breakdown;
+    std::vector<int64_t> iterations;
+    std::vector<double> residual, relative;
+    explicit SolveReport(int64_t b = 0)
+        : converged(b), breakdown(b), iterations(b), residual(b), relative(b) {}
+};
+
+struct MetricFailure : std::runtime_error {
+    SolveReport report;
+    MetricFailure(const char *what, SolveReport r) : std::runtime_error(what), report(std::move(r)) {}
+};
+
+// Contiguous atom block [first, last) with its lower Cholesky factor
+// (block-Jacobi; cpp/native.cpp:145-149, 212-229).
+struct AtomBlock {
+    int64_t first = 0, last = 0;
+    cvec L;  // (m, m) row-major
+};
+
+struct Metric {
+    std::shared_ptr<Csr> S;
+    std::vector<double> diag;
+    std::vector<AtomBlock> blocks;
+    int threads = 1;
+    bool adaptive = false;
+    int check_interval = 32;
+    double gap_ratio = 0.1;
+    int64_t calls = 0, iterations = 0, refreshes = 0, residual_checks = 0, restarts = 0;
+    double max_relative = 0;
+
+    // z = M^-1 r: diagonal or block-Jacobi L^-H L^-1 (cpp/native.cpp:237-297).
+    Block precondition(const Block &r) const {
+        Block z(r.n, r.b);
+        const int64_t n = r.n, b = r.b;
+        if (blocks.empty()) {
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 4096 && threads > 1)
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < b; ++j) z.v[i * b + j] = r.v[i * b + j] / diag[i];
+            return z;
+        }
+        const int64_t nblk = int64_t(blocks.size());
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 2048 && threads > 1)
+        for (int64_t q = 0; q < nblk; ++q) {
+            const AtomBlock &blk = blocks[q];
+            const int64_t m = blk.last - blk.first, o = blk.first;
+            for (int64_t j = 0; j < b; ++j) {
+                // forward: L w = r
+                for (int64_t i = 0; i < m; ++i) {
+                    cplx s = r.v[(o + i) * b + j];
+                    for (int64_t k = 0; k < i; ++k) s -= blk.L[i * m + k] * z.v[(o + k) * b + j];
+                    z.v[(o + i) * b + j] = s / blk.L[i * m + i];
+                }
+                // backward: L^H z = w
+                for (int64_t i = m; i-- > 0;) {
+                    cplx s = z.v[(o + i) * b + j];
+                    for (int64_t k = i + 1; k < m; ++k) s -= std::conj(blk.L[k * m + i]) * z.v[(o + k) * b + j];
+                    z.v[(o + i) * b + j] = s / std::conj(blk.L[i * m + i]);
+                }
+            }
+        }
+        return z;
+    }
+
+    // Column sums serial in row order (cpp/native.cpp:298-312).
+    static std::vector<cplx> dot(const Block &a, const Block &c) {
+        std::vector<cplx> s(a.b);
+        for (int64_t i = 0; i < a.n; ++i)
+            for (int64_t j = 0; j < a.b; ++j) s[j] += std::conj(a.v[i * a.b + j]) * c.v[i * a.b + j];
+        return s;
+    }
+    static std::vector<double> norms(const Block &a) {
+        std::vector<double> s(a.b);
+        for (int64_t i = 0; i < a.n; ++i)
+            for (int64_t j = 0; j < a.b; ++j) s[j] += std::norm(a.v[i * a.b + j]);
+        for (double &x : s) x = std::sqrt(x);
+        return s;
+    }
+
+    // Batched PCG with per-column RHS scaling, periodic true-residual refresh
+    // and final original-equation acceptance (cpp/native.cpp:313-452).
+    Block solve(const Block &B, double rtol, double atol, int maxiter, SolveReport &report) {
+        if (B.n != S->n || B.b < 1 || !std::isfinite(rtol) || !std::isfinite(atol) || rtol < 0 || atol < 0 ||
+            maxiter < 1)
+            throw std::invalid_argument("invalid metric RHS shape/tolerances");
+        const int64_t n = B.n, b = B.b;
+        std::vector<double> safe(b, 0.0), thr(b), rz(b), alpha(b), beta(b);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j) {
+                const cplx x = B.v[i * b + j];
+                if (!finite(x)) throw std::invalid_argument("metric RHS must be finite");
+                safe[j] = std::max(safe[j], std::max(std::abs(x.real()), std::abs(x.imag())));
+            }
+        for (double &s : safe)
+            if (s == 0.0) s = 1.0;
+        Block q(n, b), y(n, b);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j) q.v[i * b + j] = B.v[i * b + j] / safe[j];
+        const std::vector<double> qn = norms(q);
+        std::vector<uint8_t> active(b);
+        for (int64_t j = 0; j < b; ++j) {
+            thr[j] = std::max(rtol * qn[j], atol / safe[j]);
+            active[j] = qn[j] > thr[j];
+        }
+        Block r = q, z = precondition(r), p = z;
+        {
+            const auto d = dot(r, z);
+            for (int64_t j = 0; j < b; ++j) rz[j] = d[j].real();
+        }
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j)
+                if (!active[j]) p.v[i * b + j] = 0.0;
+        report = SolveReport(b);
+        bool broke = false;
+        for (int k = 0; k < maxiter; ++k) {
+            if (std::none_of(active.begin(), active.end(), [](uint8_t a) { return a != 0; })) break;
+            const Block sp = S->apply(p);
+            const auto curv = dot(p, sp);
+            for (int64_t j = 0; j < b; ++j) {
+                const double cr = curv[j].real(), ci = curv[j].imag();
+                if (active[j] && (!(cr > 0) || !std::isfinite(cr) || !std::isfinite(ci) ||
+                                  std::abs(ci) > 1e-10 * std::max(std::abs(cr), std::numeric_limits<double>::min()))) {
+                    report.breakdown[j] = 1;
+                    broke = true;
+                }
+                alpha[j] = active[j] ? rz[j] / cr : 0.0;
+            }
+            if (broke) break;
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 4096 && threads > 1)
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < b; ++j) {
+                    y.v[i * b + j] += p.v[i * b + j] * alpha[j];
+                    r.v[i * b + j] -= sp.v[i * b + j] * alpha[j];
+                }
+            for (int64_t j = 0; j < b; ++j) report.iterations[j] += active[j];
+            const std::vector<double> rn = norms(r);
+            const bool refresh = (k + 1) % check_interval == 0;
+            std::vector<uint8_t> restart(b, 0);
+            bool candidate = false;
+            for (int64_t j = 0; j < b; ++j) candidate = candidate || (active[j] && rn[j] <= thr[j]);
+            if (refresh || (adaptive && candidate)) {
+                Block t = S->apply(y);
+                ++refreshes;
+                ++residual_checks;
+                for (size_t e = 0; e < t.v.size(); ++e) t.v[e] = q.v[e] - t.v[e];
+                const std::vector<double> tn = norms(t);
+                for (int64_t j = 0; j < b; ++j) {
+                    if (!active[j]) continue;
+                    if (adaptive) {
+                        double gap = 0;
+                        for (int64_t i = 0; i < n; ++i) gap += std::norm(t.v[i * b + j] - r.v[i * b + j]);
+                        restart[j] = tn[j] > thr[j] &&
+                                     (std::sqrt(gap) > gap_ratio * std::max(tn[j], thr[j]) || rn[j] <= thr[j]);
+                        active[j] = tn[j] > thr[j];
+                    } else {
+                        restart[j] = 1;
+                        active[j] = tn[j] > thr[j];
+                    }
+                    if (restart[j]) {
+                        ++restarts;
+                        for (int64_t i = 0; i < n; ++i) r.v[i * b + j] = t.v[i * b + j];
+                    }
+                }
+            } else if (!adaptive) {
+                for (int64_t j = 0; j < b; ++j) active[j] = active[j] && rn[j] > thr[j];
+            }
+            z = precondition(r);
+            const auto nrz = dot(r, z);
+            for (int64_t j = 0; j < b; ++j) {
+                beta[j] = active[j] ? nrz[j].real() / (rz[j] != 0 ? rz[j] : 1.0) : 0.0;
+                rz[j] = nrz[j].real();
+            }
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 4096 && threads > 1)
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < b; ++j)
+                    p.v[i * b + j] =
+                        active[j] ? z.v[i * b + j] + (restart[j] ? cplx(0) : p.v[i * b + j] * beta[j]) : cplx(0);
+        }
+        {
+            const Block sy = S->apply(y);
+            ++residual_checks;
+            for (size_t e = 0; e < r.v.size(); ++e) r.v[e] = q.v[e] - sy.v[e];
+        }
+        const std::vector<double> rn = norms(r);
+        bool ok = !broke;
+        for (int64_t j = 0; j < b; ++j) {
+            report.converged[j] = rn[j] <= thr[j] && !report.breakdown[j];
+            report.relative[j] = qn[j] > 0 ? rn[j] / qn[j] : 0.0;
+            report.residual[j] = rn[j] * safe[j];
+            max_relative = std::max(max_relative, report.relative[j]);
+            ok = ok && report.converged[j];
+        }
+        std::vector<uint8_t> fin(b, 1);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j)
+                if (!finite(y.v[i * b + j])) fin[j] = 0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j)
+                if (fin[j]) {
+                    y.v[i * b + j] *= safe[j];
+                    if (!finite(y.v[i * b + j])) fin[j] = 0;
+                }
+        bool representable = true;
+        for (int64_t j = 0; j < b; ++j) {
+            report.converged[j] = report.converged[j] && fin[j];
+            representable = representable && fin[j];
+        }
+        ok = ok && representable;
+        ++calls;
+        iterations += *std::max_element(report.iterations.begin(), report.iterations.end());
+        if (!ok)
+            throw MetricFailure(!representable ? "S solve solution is nonfinite or not representable in complex128 after rescaling"
+                                : broke       ? "PCG non-positive/complex curvature breakdown"
+                                              : "S solve failed true-residual acceptance",
+                                report);
+        return y;
+    }
+
+    // p(W) xi by Clenshaw with W = Z S Z, Z = diag(S)^-1/2 (cpp/native.cpp:453-479).
+    Block sample(const Block &xi, const std::vector<double> &coeff, double center, double half) const {
+        if (xi.n != S->n || coeff.empty()) throw std::invalid_argument("invalid sampling dimensions");
+        const int64_t n = xi.n, b = xi.b;
+        std::vector<double> zs(diag.size());
+        for (size_t i = 0; i < zs.size(); ++i) zs[i] = 1 / std::sqrt(diag[i]);
+        auto act = [&](const Block &a) {
+            Block s = a;
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < b; ++j) s.v[i * b + j] *= zs[i];
+            Block o = S->apply(s);
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < b; ++j) o.v[i * b + j] = (zs[i] * o.v[i * b + j] - center * a.v[i * b + j]) / half;
+            return o;
+        };
+        Block b1(n, b), b2(n, b), out = xi;
+        if (coeff.size() > 1) {
+            if (!(half > 0)) throw std::invalid_argument("sampling half-width must be positive");
+            for (size_t k = coeff.size() - 1; k > 0; --k) {
+                Block b0 = act(b1);
+                for (size_t e = 0; e < b0.v.size(); ++e) b0.v[e] = 2. * b0.v[e] - b2.v[e] + coeff[k] * xi.v[e];
+                b2 = std::move(b1);
+                b1 = std::move(b0);
+            }
+            out = act(b1);
+            for (size_t e = 0; e < out.v.size(); ++e) out.v[e] -= b2.v[e];
+        } else {
+            std::fill(out.v.begin(), out.v.end(), cplx(0));
+        }
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < b; ++j) out.v[i * b + j] = zs[i] * (out.v[i * b + j] + coeff[0] * xi.v[i * b + j]);
+        return out;
+    }
+};
+
+// Cartesian velocity definition: kind 0 explicit V, 1 position X, 2 composite
+// (bond, xbar, delta) (cpp/native.cpp:482-487).
+struct Cartesian {
+    int kind = -1;
+    std::vector<std::shared_ptr<Csr>> m;
+};
+
+struct Engine {
+    std::shared_ptr<Csr> H;
+    std::shared_ptr<Metric> metric;
+    Cartesian cart[3];
+    double rtol = 1e-10, atol = 0, hbar = 0.6582119569509067;
+    int maxiter = 1000;
+    int64_t h_calls = 0, v_calls = 0;
+    SolveReport last;
+
+    Block solve(const Block &x) { return metric->solve(x, rtol, atol, maxiter, last); }
+    Block h_action(const Block &x) {
+        ++h_calls;
+        return solve(H->apply(x));
+    }
+    // (cpp/native.cpp:506-523)
+    Block v_cart(int axis, const Block &x) {
+        const Cartesian &c = cart[axis];
+        if (c.kind == 0 && c.m.size() == 1) return c.m[0]->apply(x);
+        const cplx iu(0, 1);
+        if (c.kind == 1 && c.m.size() == 1) {
+            Block o = H->apply(solve(c.m[0]->apply(x)));
+            const Block r = c.m[0]->apply(solve(H->apply(x)));
+            for (size_t e = 0; e < o.v.size(); ++e) o.v[e] = iu * (o.v[e] - r.v[e]) / hbar;
+            return o;
+        }
+        if (c.kind == 2 && c.m.size() == 3) {
+            Block o = c.m[0]->apply(x);
+            const Block l = c.m[1]->apply(solve(H->apply(x)));
+            const Block r = H->apply(solve(c.m[2]->apply(x)));
+            for (size_t e = 0; e < o.v.size(); ++e) o.v[e] = iu * (o.v[e] - l.v[e] + r.v[e]) / hbar;
+            return o;
+        }
+        throw std::invalid_argument("missing Cartesian velocity");
+    }
+    Block v_cov(const double *dir, const Block &x) {
+        ++v_calls;
+        Block o(x.n, x.b);
+        for (int a = 0; a < 3; ++a) {
+            if (dir[a] == 0.0) continue;
+            const Block part = v_cart(a, x);
+            for (size_t e = 0; e < o.v.size(); ++e) o.v[e] += dir[a] * part.v[e];
+        }
+        return o;
+    }
+    Block scaled(const Block &x, double c, double a) {
+        if (!(a > 0)) throw std::invalid_argument("spectral half-width must be positive");
+        Block o = h_action(x);
+        for (size_t e = 0; e < o.v.size(); ++e) o.v[e] = (o.v[e] - c * x.v[e]) / a;
+        return o;
+    }
+    // Scaled action on [B0 | B1 | ...] plus the commutator source (-i hbar/a) S^-1 V_a B0
+    // in block a+1 (cpp/native.cpp:540-554).
+    Block augmented(const Block &x, double c, double a, const std::vector<double> &dirs) {
+        Block o = scaled(x, c, a);
+        const int64_t nd = int64_t(dirs.size() / 3);
+        if (nd == 0) return o;
+        if (x.b % (nd + 1)) throw std::invalid_argument("invalid augmented block columns");
+        const int64_t nb = x.b / (nd + 1);
+        Block init(x.n, nb);
+        for (int64_t i = 0; i < x.n; ++i) std::copy_n(x.v.data() + i * x.b, nb, init.v.data() + i * nb);
+        for (int64_t d = 0; d < nd; ++d) {
+            const Block v = solve(v_cov(&dirs[3 * d], init));
+            for (int64_t i = 0; i < x.n; ++i)
+                for (int64_t j = 0; j < nb; ++j) o.v[i * x.b + (d + 1) * nb + j] += cplx(0, -hbar / a) * v.v[i * nb + j];
+        }
+        return o;
+    }
+    // (cpp/native.cpp:555-573)
+    Block series(const Block &x, const cvec &coeff, double c, double a, const std::vector<double> &dirs) {
+        if (coeff.empty()) throw std::invalid_argument("empty Chebyshev coefficients");
+        Block out = x;
+        for (cplx &e : out.v) e *= coeff[0];
+        if (coeff.size() == 1) return out;
+        Block prev = x, curr = augmented(x, c, a, dirs);
+        for (size_t e = 0; e < out.v.size(); ++e) out.v[e] += coeff[1] * curr.v[e];
+        for (size_t k = 2; k < coeff.size(); ++k) {
+            Block next = augmented(curr, c, a, dirs);
+            for (size_t e = 0; e < out.v.size(); ++e) {
+                next.v[e] = 2. * next.v[e] - prev.v[e];
+                out.v[e] += coeff[k] * next.v[e];
+            }
+            prev = std::move(curr);
+            curr = std::move(next);
+        }
+        for (const cplx &e : out.v)
+            if (!finite(e)) throw std::runtime_error("nonfinite Chebyshev propagation; check bounds and solves");
+        return out;
+    }
+    // (cpp/native.cpp:574-589)
+    void chain(const Block &x, int64_t L, double c, double a, cplx *out) {
+        if (L < 1) throw std::invalid_argument("Chebyshev chain must be nonempty");
+        const size_t stride = x.v.size();
+        std::copy(x.v.begin(), x.v.end(), out);
+        if (L == 1) return;
+        Block prev = x, curr = scaled(x, c, a);
+        std::copy(curr.v.begin(), curr.v.end(), out + stride);
+        for (int64_t k = 2; k < L; ++k) {
+            Block next = scaled(curr, c, a);
+            for (size_t e = 0; e < stride; ++e) next.v[e] = 2. * next.v[e] - prev.v[e];
+            std::copy(next.v.begin(), next.v.end(), out + k * stride);
+            prev = std::move(curr);
+            curr = std::move(next);
+        }
+    }
+    // mu[l][j] = Re <S B_j, T_l B_j> (cpp/native.cpp:590-609)
+    void moments(const Block &x, int64_t L, double c, double a, double *mu) {
+        if (L < 1) throw std::invalid_argument("moment count must be positive");
+        const Block sb = metric->S->apply(x);
+        auto store = [&](int64_t l, const Block &t) {
+            const auto d = Metric::dot(sb, t);
+            for (int64_t j = 0; j < x.b; ++j) mu[l * x.b + j] = d[j].real();
+        };
+        store(0, x);
+        if (L == 1) return;
+        Block prev = x, curr = scaled(x, c, a);
+        store(1, curr);
+        for (int64_t k = 2; k < L; ++k) {
+            Block next = scaled(curr, c, a);
+            for (size_t e = 0; e < next.v.size(); ++e) next.v[e] = 2. * next.v[e] - prev.v[e];
+            store(k, next);
+            prev = std::move(curr);
+            curr = std::move(next);
+        }
+    }
+};
+
+Block load(const double *p, int64_t n, int64_t b) {
+    if (n < 1 || b < 1) throw std::invalid_argument("expected nonempty finite (N,batch) array");
+    Block x(n, b);
+    std::memcpy(x.v.data(), p, sizeof(cplx) * size_t(n) * b);
+    for (const cplx &e : x.v)
+        if (!finite(e)) throw std::invalid_argument("expected finite (N,batch) array");
+    return x;
+}
+void store(const Block &x, double *p) { std::memcpy(p, x.v.data(), sizeof(cplx) * x.v.size()); }
+
+void put_report(const SolveReport &r, uint8_t *conv, uint8_t *brk, int64_t *its, double *res, double *rel) {
+    const size_t b = r.converged.size();
+    if (conv) std::copy_n(r.converged.data(), b, conv);
+    if (brk) std::copy_n(r.breakdown.data(), b, brk);
+    if (its) std::copy_n(r.iterations.data(), b, its);
+    if (res) std::copy_n(r.residual.data(), b, res);
+    if (rel) std::copy_n(r.relative.data(), b, rel);
+}
+
+// Status codes: 0 ok, 1 invalid argument, 2 numerical failure (report valid).
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const MetricFailure &e) {
+        g_error = e.what();
+        return 2;
+    } catch (const std::invalid_argument &e) {
+        g_error = e.what();
+        return 1;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return 2;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ko_last_error() { return g_error.c_str(); }
+
+void *ko_csr_create(int64_t n, int64_t nnz, const int64_t *ptr, const int64_t *idx, const double *val, int threads) {
+    auto m = new std::shared_ptr<Csr>(std::make_shared<Csr>());
+    Csr &a = **m;
+    if (guarded([&] {
+            if (threads < 1 || n < 1) throw std::invalid_argument("invalid CSR dimensions/threads");
+            a.n = n;
+            a.threads = threads;
+            a.ptr.assign(ptr, ptr + n + 1);
+            a.idx.assign(idx, idx + nnz);
+            a.val.resize(nnz);
+            std::memcpy(a.val.data(), val, sizeof(cplx) * nnz);
+            if (a.ptr.front() != 0 || a.ptr.back() != nnz) throw std::invalid_argument("invalid CSR row pointers");
+            for (int64_t r = 0; r < n; ++r)
+                if (a.ptr[r] > a.ptr[r + 1]) throw std::invalid_argument("nonmonotone CSR row pointers");
+            for (int64_t k = 0; k < nnz; ++k)
+                if (a.idx[k] < 0 || a.idx[k] >= n || !finite(a.val[k]))
+                    throw std::invalid_argument("invalid CSR index or nonfinite entry");
+        })) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
+void ko_csr_destroy(void *m) { delete static_cast<std::shared_ptr<Csr> *>(m); }
+int ko_csr_matmul(void *m, const double *x, double *y, int64_t n, int64_t b) {
+    return guarded([&] { store((*static_cast<std::shared_ptr<Csr> *>(m))->apply(load(x, n, b)), y); });
+}
+
+// blocks: nblocks contiguous [block_ptr[q], block_ptr[q+1]) with factors
+// concatenated row-major in `factors` (complex interleaved).
+void *ko_metric_create(void *S, const double *diag, int64_t nblocks, const int64_t *block_ptr, const double *factors,
+                       int threads) {
+    auto m = new std::shared_ptr<Metric>(std::make_shared<Metric>());
+    Metric &g = **m;
+    if (guarded([&] {
+            g.S = *static_cast<std::shared_ptr<Csr> *>(S);
+            g.threads = threads;
+            if (threads < 1) throw std::invalid_argument("invalid metric diagonal/threads");
+            g.diag.assign(diag, diag + g.S->n);
+            for (double d : g.diag)
+                if (!(d > 0) || !std::isfinite(d)) throw std::invalid_argument("invalid metric diagonal");
+            size_t off = 0;
+            for (int64_t q = 0; q < nblocks; ++q) {
+                AtomBlock blk;
+                blk.first = block_ptr[q];
+                blk.last = block_ptr[q + 1];
+                const int64_t m2 = (blk.last - blk.first) * (blk.last - blk.first);
+                if (blk.last <= blk.first || blk.last > g.S->n || (q == 0 && blk.first != 0))
+                    throw std::invalid_argument("invalid block-Jacobi factor partition");
+                blk.L.resize(m2);
+                std::memcpy(blk.L.data(), factors + 2 * off, sizeof(cplx) * m2);
+                off += m2;
+                g.blocks.push_back(std::move(blk));
+            }
+            if (nblocks && block_ptr[nblocks] != g.S->n) throw std::invalid_argument("incomplete block-Jacobi partition");
+        })) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
+void ko_metric_destroy(void *m) { delete static_cast<std::shared_ptr<Metric> *>(m); }
+int ko_metric_set_residual_policy(void *m, int adaptive, int interval, double gap) {
+    return guarded([&] {
+        if (interval < 1 || !(gap > 0 && gap < 1)) throw std::invalid_argument("invalid residual policy");
+        Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+        g.adaptive = adaptive != 0;
+        g.check_interval = interval;
+        g.gap_ratio = gap;
+    });
+}
+int ko_metric_solve(void *m, const double *B, double *Y, int64_t n, int64_t b, double rtol, double atol, int maxiter,
+                    uint8_t *conv, uint8_t *brk, int64_t *its, double *res, double *rel) {
+    Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+    SolveReport rep(b);
+    int rc = guarded([&] {
+        try {
+            store(g.solve(load(B, n, b), rtol, atol, maxiter, rep), Y);
+        } catch (const MetricFailure &e) {
+            rep = e.report;
+            throw;
+        }
+    });
+    put_report(rep, conv, brk, its, res, rel);
+    return rc;
+}
+int ko_metric_sample(void *m, const double *xi, double *out, int64_t n, int64_t b, const double *coeff, int64_t nc,
+                     double center, double half) {
+    Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+    return guarded([&] { store(g.sample(load(xi, n, b), std::vector<double>(coeff, coeff + nc), center, half), out); });
+}
+// stats: calls, iterations, refreshes, residual_checks, restarts
+void ko_metric_counters(void *m, int64_t *out5, double *max_relative) {
+    Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+    out5[0] = g.calls;
+    out5[1] = g.iterations;
+    out5[2] = g.refreshes;
+    out5[3] = g.residual_checks;
+    out5[4] = g.restarts;
+    *max_relative = g.max_relative;
+}
+
+// kinds[3]; mats[9] (3 per axis, unused entries null).
+void *ko_engine_create(void *H, void *metric, const int *kinds, void *const *mats, double rtol, double atol,
+                       int maxiter, double hbar) {
+    auto e = new Engine();
+    e->H = *static_cast<std::shared_ptr<Csr> *>(H);
+    e->metric = *static_cast<std::shared_ptr<Metric> *>(metric);
+    e->rtol = rtol;
+    e->atol = atol;
+    e->maxiter = maxiter;
+    e->hbar = hbar;
+    for (int a = 0; a < 3; ++a) {
+        e->cart[a].kind = kinds[a];
+        const int cnt = kinds[a] == 2 ? 3 : (kinds[a] >= 0 ? 1 : 0);
+        for (int k = 0; k < cnt; ++k) e->cart[a].m.push_back(*static_cast<std::shared_ptr<Csr> *>(mats[3 * a + k]));
+    }
+    return e;
+}
+void ko_engine_destroy(void *e) { delete static_cast<Engine *>(e); }
+void ko_engine_set_controls(void *e, double rtol, double atol, int maxiter) {
+    Engine &g = *static_cast<Engine *>(e);
+    g.rtol = rtol;
+    g.atol = atol;
+    g.maxiter = maxiter;
+}
+void ko_engine_counters(void *e, int64_t *h, int64_t *v) {
+    *h = static_cast<Engine *>(e)->h_calls;
+    *v = static_cast<Engine *>(e)->v_calls;
+}
+int ko_engine_last_report(void *e, uint8_t *conv, uint8_t *brk, int64_t *its, double *res, double *rel) {
+    put_report(static_cast<Engine *>(e)->last, conv, brk, its, res, rel);
+    return int(static_cast<Engine *>(e)->last.converged.size());
+}
+int ko_h_action(void *e, const double *x, double *y, int64_t n, int64_t b) {
+    return guarded([&] { store(static_cast<Engine *>(e)->h_action(load(x, n, b)), y); });
+}
+int ko_v_cov(void *e, const double *dir, const double *x, double *y, int64_t n, int64_t b) {
+    return guarded([&] { store(static_cast<Engine *>(e)->v_cov(dir, load(x, n, b)), y); });
+}
+int ko_series(void *e, const double *x, double *y, int64_t n, int64_t b, const double *coeff, int64_t nc, double c,
+              double a, const double *dirs, int64_t ndirs) {
+    return guarded([&] {
+        cvec cf(nc);
+        std::memcpy(cf.data(), coeff, sizeof(cplx) * nc);
+        store(static_cast<Engine *>(e)->series(load(x, n, b), cf, c, a, std::vector<double>(dirs, dirs + 3 * ndirs)), y);
+    });
+}
+int ko_chain(void *e, const double *x, double *out, int64_t n, int64_t b, int64_t L, double c, double a) {
+    return guarded([&] { static_cast<Engine *>(e)->chain(load(x, n, b), L, c, a, reinterpret_cast<cplx *>(out)); });
+}
+int ko_moments(void *e, const double *x, double *mu, int64_t n, int64_t b, int64_t L, double c, double a) {
+    return guarded([&] { static_cast<Engine *>(e)->moments(load(x, n, b), L, c, a, mu); });
+}
+
+}  // extern "C"
