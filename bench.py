@@ -50,7 +50,27 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=5.0, help="target seconds per CPU baseline run")
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a plain 1 GiB device copy (torch copy kernel), to scale HBM counters")
     return p.parse_args()
+
+
+def calibrate_copy(reps=5):
+    """A plain copy of known size: reads and writes 2^30 bytes per launch."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "ms_per_copy": ms,
+                      "copy_GBps_read_plus_write": 2 * (1 << 30) / (ms / 1e3) / 1e9}))
 
 
 def random_phases(n, seed, ids):
@@ -120,6 +140,9 @@ def cpu_baseline(model, B, c, a, target_s):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sample sharding is a design "
                                    "note (independent statistics blocks per GPU, one final gather)",

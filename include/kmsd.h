@@ -114,6 +114,9 @@ int kmsd_metric_solve(kmsd_metric *m, const void *B, void *Y, int64_t nb, int dt
  * out = Z p(W) xi with W = Z S Z, Z = diag^-1/2, p by Clenshaw. */
 int kmsd_metric_sample(kmsd_metric *m, const void *xi, void *out, int64_t nb, int dtype, const double *coeff,
                        int64_t ncoeff, double center, double half);
+/* Reference: OperatorBundle.inner(A, B) = sum(conj(A) * (S @ B), axis=0)
+ * (src/kubo_msd/operators/velocity.py:117).  out: DEVICE complex128[nb]. */
+int kmsd_metric_inner(kmsd_metric *m, const void *A, const void *B, double *out, int64_t nb, int dtype);
 /* Reference: Metric.diagnostics() counters (cpp/native.cpp:666-675):
  * out[0] calls, [1] sum of max iterations, [2] refreshes, [3] residual
  * checks, [4] restarts; *max_relative. */

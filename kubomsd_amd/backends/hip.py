@@ -48,6 +48,7 @@ SIGNATURES = {
     "kmsd_metric_solve": [_P, _P, _P, _I64, _I, _D, _D, _I, _P],
     "kmsd_metric_sample": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D],
     "kmsd_metric_counters": [_P, _P, _P],
+    "kmsd_metric_inner": [_P, _P, _P, _P, _I64, _I],
     "kmsd_engine_create": [_P, _P, _P, _P, _P, _D, _D, _I, _D, _P],
     "kmsd_engine_destroy": [_P],
     "kmsd_engine_set_controls": [_P, _D, _D, _I],
@@ -295,6 +296,20 @@ class HipMetric:
         ctx.check(ctx.lib.kmsd_metric_sample(self.h, C.c_void_p(xi.data_ptr()), C.c_void_p(out.data_ptr()),
                                              xi.shape[1], _dtype_code(xi), _np_ptr(c), len(c), float(center),
                                              float(half)))
+        return out
+
+    def inner(self, A, B):
+        """Device complex128 (nb,): sum_i conj(A_ij) (S B)_ij."""
+        ctx = self.ctx
+        A = ctx.block(A)
+        B = ctx.block(B)
+        if A.shape != B.shape or A.dtype != B.dtype or A.shape[0] != self.n:
+            raise KuboMSDError("inner product blocks must match")
+        ctx.bind_stream()
+        torch = _torch()
+        out = torch.empty(A.shape[1], dtype=torch.complex128, device=A.device)
+        ctx.check(ctx.lib.kmsd_metric_inner(self.h, C.c_void_p(A.data_ptr()), C.c_void_p(B.data_ptr()),
+                                            C.c_void_p(out.data_ptr()), A.shape[1], _dtype_code(A)))
         return out
 
     def diagnostics(self):

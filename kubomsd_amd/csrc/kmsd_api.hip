@@ -55,6 +55,7 @@ extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->flag) (void)hipFree(c->flag);
+    if (c->mu_part) (void)hipFree(c->mu_part);
     delete c;
     return KMSD_OK;
 }
@@ -85,7 +86,7 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             if (value < 8 || value % 8) throw Error(KMSD_EINVAL, "groups must be a positive multiple of 8");
             c->groups = value;
         } else if (k == "cpl") {
-            if (value != 1 && value != 2) throw Error(KMSD_EINVAL, "cpl must be 1 or 2");
+            if (value != 1 && value != 2 && value != 4) throw Error(KMSD_EINVAL, "cpl must be 1, 2 or 4");
             c->cpl = int(value);
         } else if (k == "moment_chunk") {
             if (value < 1 || value > 4096) throw Error(KMSD_EINVAL, "moment_chunk must be in [1, 4096]");

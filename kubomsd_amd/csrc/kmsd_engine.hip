@@ -358,7 +358,10 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             spmm(c, dtype, e->metric->S, B, nb, sb, nb, w, 1.0, 0.0, false);
             W = sb;
         }
-        const int64_t G = step_groups(c, n, w, dtype);
+        void *t1 = e->t0.ensure(c, bytes);
+        void *spare = e->t1.ensure(c, bytes);
+        const int cap = cpl_cap_for(dtype, {{B, nb}, {W, nb}, {t1, nb}, {spare, nb}});
+        const int64_t G = step_groups(c, n, w, dtype, cap);
         const int64_t chunk = std::min<int64_t>(c->moment_chunk, L);
         const size_t slot_doubles = size_t(G) * size_t(w) * 2;
         double *part = static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
@@ -380,13 +383,13 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             s.ldw = nb;
             s.n = n;
             s.w = w;
+            s.cpl_cap = cap;
+            s.expect_groups = G;
             s.part = next_part();
             launch_step(c, dtype, s);
         }
         if (L > 1) {
             check_half(half);
-            void *t1 = e->t0.ensure(c, bytes);
-            void *spare = e->t1.ensure(c, bytes);
             Step s;
             s.X = B;
             s.ldx = nb;
@@ -399,6 +402,8 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             s.ldw = nb;
             s.n = n;
             s.w = w;
+            s.cpl_cap = cap;
+            s.expect_groups = G;
             s.part = next_part();
             cheb(e, dtype, s);
             const void *prev = B;
@@ -420,6 +425,8 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
                 q.ldw = nb;
                 q.n = n;
                 q.w = w;
+                q.cpl_cap = cap;
+                q.expect_groups = G;
                 q.part = next_part();
                 cheb(e, dtype, q);
                 prev = cur;

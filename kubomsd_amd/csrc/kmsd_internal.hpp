@@ -21,6 +21,8 @@ struct kmsd_ctx {
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
     int pcg_poll = 4;       // PCG iterations between host status polls
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
+    void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
+    size_t mu_part_bytes = 0;
 };
 
 // Value kinds of a stored matrix.
@@ -29,15 +31,21 @@ enum { VK_REAL = 0, VK_IMAG = 1, VK_COMPLEX = 2 };
 struct kmsd_matrix {
     kmsd_ctx *ctx = nullptr;
     int fmt = KMSD_CSR;
-    int64_t n = 0, nnz = 0;
-    int32_t *ptr = nullptr, *idx = nullptr;
+    int64_t n = 0, nnz = 0;  // nnz of the full matrix as given
+    // Device layout: the diagonal as a dense array plus the off-diagonal part
+    // as ELL (when rows have near-uniform length) or CSR.
+    int64_t noff = 0;        // stored off-diagonal entries (ELL: n*ell incl. padding)
+    int32_t ell = 0;         // ELL width, 0 = CSR
+    int32_t *ptr = nullptr;  // CSR row pointers of the off-diagonal part (CSR only)
+    int32_t *idx = nullptr;
     int vkind = VK_COMPLEX;
-    void *val64 = nullptr;  // double[nnz] or cx<double>[nnz]
-    void *val32 = nullptr;  // float[nnz] or cx<float>[nnz]
-    bool identity = false;  // exactly the identity matrix
-    size_t bytes = 0;
-    // BSR 4x4 (fmt == KMSD_BSR4): block rows nbr = n/4; ptr/idx over blocks;
-    // values are 16 per block, row-major within the block.
+    void *val64 = nullptr;   // off-diagonal values: double[] or cx<double>[]
+    void *val32 = nullptr;   // float[] or cx<float>[]
+    bool has_diag = false;
+    void *dval64 = nullptr;  // diagonal values (n)
+    void *dval32 = nullptr;
+    bool identity = false;   // exactly the identity matrix
+    size_t bytes = 0;        // device bytes held
 };
 
 namespace kmsd {
@@ -127,12 +135,17 @@ struct Step {
     bool use_beta = false, use_p = false;
     int64_t n = 0;
     int32_t w = 0;
+    int cpl_cap = 4;            // columns-per-lane ceiling (keeps the shape fixed across a chain)
+    const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
+    int64_t expect_groups = 0;  // if nonzero, the launch must use exactly this many workgroups
 };
 // Launches one fused step; returns the number of workgroups used (the
 // partial-sum row count) so callers can size/reduce partials.
 int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s);
 // Number of workgroups launch_step will use for (n, w, dtype).
-int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype);
+int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap = 4);
+// Columns-per-lane ceiling supported by a set of blocks (alignment and leading dimensions).
+int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t>> blocks);
 // out[s*ldo + j] = sum_g part[s][g][j]; mode 0 real part (double), 1 complex.
 void launch_reduce(kmsd_ctx *c, const double *part, int64_t groups, int32_t w, int64_t nsteps, double *out,
                    int64_t ldo, int mode);
@@ -146,6 +159,10 @@ void launch_convert(kmsd_ctx *c, int dtype_in, const void *X, int64_t ldx, int d
 void spmm(kmsd_ctx *c, int dtype, const kmsd_matrix *A, const void *X, int64_t ldx, void *out, int64_t ldo,
           int32_t w, double alpha_re, double alpha_im, bool accumulate);
 void require_finite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
+// Device copy of S scaled by congruence with diag^-1/2 (the sampler's W).
+kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const double *diag);
+void launch_sample_finish(kmsd_ctx *c, int dtype, void *out, const void *xi, int64_t ld, int64_t n, int32_t w,
+                          const double *diag, double c0);
 
 }  // namespace kmsd
 
@@ -176,6 +193,9 @@ struct kmsd_metric {
     std::vector<double> rep_res, rep_rel;
     struct Work;
     Work *work = nullptr;
+    kmsd_matrix *W = nullptr;  // Z S Z for the sampler (built on first use)
+    kmsd::DBuf sb1, sb2;       // sampler Clenshaw buffers
+    kmsd::DBuf pinner;         // partial sums of kmsd_metric_inner
 };
 
 struct kmsd_engine {

@@ -61,12 +61,15 @@ enum : uint32_t {
     F_STORE = 1u << 3,  // OUT_i = t
     F_MOM = 1u << 4,    // partial += conj(W_i) * t
     F_ACC = 1u << 5,    // ACC_i += coef * t
+    F_DIAG = 1u << 6,   // the product includes the stored diagonal d_i X_i
+    F_ELL = 1u << 7,    // off-diagonal part is ELL (ell entries per row, row-major)
 };
 
 struct StepArgs {
-    const int32_t *ptr;   // CSR row pointers (n+1)
-    const int32_t *idx;   // CSR column indices
-    const void *val;      // CSR values (R or cx<R>)
+    const int32_t *ptr;   // CSR row pointers of the off-diagonal part (n+1)
+    const int32_t *idx;   // off-diagonal column indices (CSR or ELL n*ell)
+    const void *val;      // off-diagonal values (R or cx<R>)
+    const void *dval;     // diagonal values (R or cx<R>), n entries
     const void *X;        // gathered input block
     const void *Z;        // row-local input when no SpMM
     const void *P;        // previous iterate
@@ -80,7 +83,9 @@ struct StepArgs {
     double coef_re, coef_im;
     int32_t w;            // columns processed (multiple of CPL)
     int32_t lpr_log2;
+    int32_t ell;          // ELL width (F_ELL)
     uint32_t flags;
+    const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
 
 template <class A>
@@ -88,16 +93,44 @@ __device__ __forceinline__ cx<A> cmul(double ar, double ai, cx<A> x) {
     return {A(ar) * x.re - A(ai) * x.im, A(ar) * x.im + A(ai) * x.re};
 }
 
+// CPL consecutive complex entries of one row as one vector access.
+template <class VT, int CPL>
+struct Pack {
+    typedef VT vec __attribute__((ext_vector_type(2 * CPL)));
+    vec v;
+    __device__ __forceinline__ void load(const cx<VT> *p) { v = *reinterpret_cast<const vec *>(p); }
+    __device__ __forceinline__ void load_nt(const cx<VT> *p) {
+        v = __builtin_nontemporal_load(reinterpret_cast<const vec *>(p));
+    }
+    __device__ __forceinline__ void store(cx<VT> *p) const { *reinterpret_cast<vec *>(p) = v; }
+    __device__ __forceinline__ void store_nt(cx<VT> *p) const {
+        __builtin_nontemporal_store(v, reinterpret_cast<vec *>(p));
+    }
+    __device__ __forceinline__ cx<double> get(int c) const { return {double(v[2 * c]), double(v[2 * c + 1])}; }
+    template <class A>
+    __device__ __forceinline__ cx<A> geta(int c) const { return {A(v[2 * c]), A(v[2 * c + 1])}; }
+    __device__ __forceinline__ void set(int c, cx<double> z) {
+        v[2 * c] = VT(z.re);
+        v[2 * c + 1] = VT(z.im);
+    }
+};
+
 // VT: vector element real type (float -> complex64, double -> complex128)
 // MR: matrix value real type; MC: complex matrix values
 // AT: arithmetic type of the row sums (double for FP64 vectors; float or
 //     double for FP32 vectors)
-// CPL: complex columns per lane (1 or 2)
+// CPL: complex columns per lane (1, 2 or 4)
+//
+// Per row the row-local streams (X_i, P_i, W_i) are issued first, then the
+// off-diagonal gathers, so their latencies overlap; P and W are read once per
+// step and use nontemporal loads, OUT a nontemporal store.
 template <class VT, class MR, bool MC, class AT, int CPL>
 __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     using V = cx<VT>;
     using MV = MatVal<MR, MC>;
     using MT = typename MV::type;
+    using PK = Pack<VT, CPL>;
+    if (a.gate && *a.gate == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lpr = 1 << a.lpr_log2;
     const int rpw = 64 >> a.lpr_log2;          // rows per wavefront
@@ -114,7 +147,9 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     const V *W = static_cast<const V *>(a.W);
     V *ACC = static_cast<V *>(a.ACC);
     const MT *val = static_cast<const MT *>(a.val);
+    const MT *dval = static_cast<const MT *>(a.dval);
     const uint32_t fl = a.flags;
+    const bool need_xi = (fl & F_BETA) || ((fl & F_SPMM) && (fl & F_DIAG));
 
     cx<double> mom[CPL];
 #pragma unroll
@@ -130,90 +165,106 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     for (int64_t blk = b0 + slot; blk < b1; blk += per) {
         const int64_t row = blk * rpb + rloc;
         if (row >= a.n || !colok) continue;
+        PK xi, pv, wv;
+        if (need_xi) xi.load(X + row * a.ldx + col);
+        if (fl & F_P) pv.load_nt(P + row * a.ldp + col);
+        if (fl & F_MOM) wv.load_nt(W + row * a.ldw + col);
         cx<AT> y[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) y[c] = {AT(0), AT(0)};
         if (fl & F_SPMM) {
-            int32_t k = a.ptr[row];
-            const int32_t e = a.ptr[row + 1];
+            int64_t k, e;
+            if (fl & F_ELL) {
+                k = row * a.ell;
+                e = k + a.ell;
+            } else {
+                k = a.ptr[row];
+                e = a.ptr[row + 1];
+            }
             for (; k + 4 <= e; k += 4) {
                 const int32_t j0 = a.idx[k], j1 = a.idx[k + 1], j2 = a.idx[k + 2], j3 = a.idx[k + 3];
                 const MT v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
-                V x0[CPL], x1[CPL], x2[CPL], x3[CPL];
+                PK x0, x1, x2, x3;
+                x0.load(X + int64_t(j0) * a.ldx + col);
+                x1.load(X + int64_t(j1) * a.ldx + col);
+                x2.load(X + int64_t(j2) * a.ldx + col);
+                x3.load(X + int64_t(j3) * a.ldx + col);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    x0[c] = X[int64_t(j0) * a.ldx + col + c];
-                    x1[c] = X[int64_t(j1) * a.ldx + col + c];
-                    x2[c] = X[int64_t(j2) * a.ldx + col + c];
-                    x3[c] = X[int64_t(j3) * a.ldx + col + c];
-                }
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    MV::fma(y[c], v0, to<AT>(x0[c]));
-                    MV::fma(y[c], v1, to<AT>(x1[c]));
-                    MV::fma(y[c], v2, to<AT>(x2[c]));
-                    MV::fma(y[c], v3, to<AT>(x3[c]));
+                    MV::fma(y[c], v0, x0.template geta<AT>(c));
+                    MV::fma(y[c], v1, x1.template geta<AT>(c));
+                    MV::fma(y[c], v2, x2.template geta<AT>(c));
+                    MV::fma(y[c], v3, x3.template geta<AT>(c));
                 }
             }
-            for (; k < e; ++k) {
-                const int32_t j = a.idx[k];
-                const MT v = val[k];
+            if (e - k >= 2) {
+                const int32_t j0 = a.idx[k], j1 = a.idx[k + 1];
+                const MT v0 = val[k], v1 = val[k + 1];
+                PK x0, x1;
+                x0.load(X + int64_t(j0) * a.ldx + col);
+                x1.load(X + int64_t(j1) * a.ldx + col);
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) MV::fma(y[c], v, to<AT>(X[int64_t(j) * a.ldx + col + c]));
+                for (int c = 0; c < CPL; ++c) {
+                    MV::fma(y[c], v0, x0.template geta<AT>(c));
+                    MV::fma(y[c], v1, x1.template geta<AT>(c));
+                }
+                k += 2;
+            }
+            if (k < e) {
+                const int32_t j0 = a.idx[k];
+                const MT v0 = val[k];
+                PK x0;
+                x0.load(X + int64_t(j0) * a.ldx + col);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) MV::fma(y[c], v0, x0.template geta<AT>(c));
+            }
+            if (fl & F_DIAG) {
+                const MT d = dval[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) MV::fma(y[c], d, xi.template geta<AT>(c));
             }
         } else {
+            PK zv;
+            zv.load(Z + row * a.ldz + col);
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) y[c] = to<AT>(Z[row * a.ldz + col + c]);
+            for (int c = 0; c < CPL; ++c) y[c] = zv.template geta<AT>(c);
         }
         // Epilogue in FP64 whatever the storage type: one rounding per stored
         // element, and alpha/beta/gamma are never rounded to FP32.
-        cx<double> t[CPL];
+        PK ts;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) t[c] = cmul<double>(a.alpha_re, a.alpha_im, to<double>(y[c]));
-        if (fl & F_BETA) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                const cx<double> xs = cmul<double>(a.beta_re, a.beta_im, to<double>(X[row * a.ldx + col + c]));
-                t[c].re += xs.re;
-                t[c].im += xs.im;
+        for (int c = 0; c < CPL; ++c) {
+            cx<double> t = cmul<double>(a.alpha_re, a.alpha_im, to<double>(y[c]));
+            if (fl & F_BETA) {
+                const cx<double> xs = cmul<double>(a.beta_re, a.beta_im, xi.get(c));
+                t.re += xs.re;
+                t.im += xs.im;
+            }
+            if (fl & F_P) {
+                const cx<double> p = pv.get(c);
+                t.re = __builtin_fma(a.gamma, p.re, t.re);
+                t.im = __builtin_fma(a.gamma, p.im, t.im);
+            }
+            ts.set(c, t);
+            if (fl & F_ACC) {
+                cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
+                const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t);
+                acc.re += add.re;
+                acc.im += add.im;
+                ACC[row * a.lda + col + c] = to<VT>(acc);
             }
         }
-        if (fl & F_P) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                const cx<double> p = to<double>(P[row * a.ldp + col + c]);
-                t[c].re = __builtin_fma(a.gamma, p.re, t[c].re);
-                t[c].im = __builtin_fma(a.gamma, p.im, t[c].im);
-            }
-        }
-        V ts[CPL];
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) ts[c] = to<VT>(t[c]);
-        if (fl & F_STORE) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) OUT[row * a.ldo + col + c] = ts[c];
-        }
+        if (fl & F_STORE) ts.store_nt(OUT + row * a.ldo + col);
         if (fl & F_MOM) {
             // Products of the stored (rounded) values, accumulated in FP64;
             // FP32 x FP32 products are exact in FP64.
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                const cx<double> wv = to<double>(W[row * a.ldw + col + c]);
-                const cx<double> tv = to<double>(ts[c]);
-                mom[c].re = __builtin_fma(wv.re, tv.re, mom[c].re);
-                mom[c].re = __builtin_fma(wv.im, tv.im, mom[c].re);
-                mom[c].im = __builtin_fma(wv.re, tv.im, mom[c].im);
-                mom[c].im = __builtin_fma(-wv.im, tv.re, mom[c].im);
-            }
-        }
-        if (fl & F_ACC) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
-                const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t[c]);
-                acc.re += add.re;
-                acc.im += add.im;
-                ACC[row * a.lda + col + c] = to<VT>(acc);
+                const cx<double> wd = wv.get(c), tv = ts.get(c);
+                mom[c].re = __builtin_fma(wd.re, tv.re, mom[c].re);
+                mom[c].re = __builtin_fma(wd.im, tv.im, mom[c].re);
+                mom[c].im = __builtin_fma(wd.re, tv.im, mom[c].im);
+                mom[c].im = __builtin_fma(-wd.im, tv.re, mom[c].im);
             }
         }
     }
@@ -250,7 +301,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 // out[s][c] = sum_g part[s][g][c] in a fixed tree order (deterministic).
 // grid (w, nsteps), 256 threads.  mode 0: real part to out_re[s*ldo + c];
 // mode 1: complex to out_cx[s*ldo + c].
-__global__ __launch_bounds__(256) void reduce_partials(const double *part, int64_t groups, int32_t w,
+static __global__ __launch_bounds__(256) void reduce_partials(const double *part, int64_t groups, int32_t w,
                                                        int64_t step_stride, double *out, int64_t ldo,
                                                        int32_t mode) {
     const int c = blockIdx.x, s = blockIdx.y;
@@ -320,6 +371,56 @@ __global__ __launch_bounds__(256) void nonfinite_kernel(const void *Xv, int64_t 
         bad |= !(__builtin_isfinite(v.re) && __builtin_isfinite(v.im));
     }
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+}  // namespace kmsd
+
+namespace kmsd {
+
+// W = Z S Z with Z = diag(d)^-1/2: scales the stored values of a copy of S.
+// One thread per row; ELL rows are [row*ell, row*ell+ell), CSR rows ptr[row]..
+template <class R, bool MC>
+__global__ __launch_bounds__(256) void congruence_scale_kernel(void *offv, void *diagv, const int32_t *ptr,
+                                                               const int32_t *idx, int32_t ell, const double *d,
+                                                               int64_t n) {
+    using MT = typename MatVal<R, MC>::type;
+    MT *off = static_cast<MT *>(offv);
+    MT *dg = static_cast<MT *>(diagv);
+    for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) {
+        const double zr = 1.0 / sqrt(d[r]);
+        const int64_t k0 = ell ? r * ell : ptr[r], k1 = ell ? k0 + ell : ptr[r + 1];
+        for (int64_t k = k0; k < k1; ++k) {
+            const double s = zr / sqrt(d[idx[k]]);
+            if constexpr (MC) {
+                off[k].re = R(double(off[k].re) * s);
+                off[k].im = R(double(off[k].im) * s);
+            } else {
+                off[k] = R(double(off[k]) * s);
+            }
+        }
+        const double s = zr * zr;
+        if constexpr (MC) {
+            dg[r].re = R(double(dg[r].re) * s);
+            dg[r].im = R(double(dg[r].im) * s);
+        } else {
+            dg[r] = R(double(dg[r]) * s);
+        }
+    }
+}
+
+// out_i = z_i (out_i + c0 xi_i), z = diag^-1/2 (final Clenshaw step of the sampler)
+template <class VT>
+__global__ __launch_bounds__(256) void sample_finish_kernel(void *Ov, const void *Xv, int64_t ld, int64_t n, int32_t w,
+                                                            const double *d, double c0) {
+    cx<VT> *O = static_cast<cx<VT> *>(Ov);
+    const cx<VT> *X = static_cast<const cx<VT> *>(Xv);
+    const int64_t total = n * w;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t r = e / w, c = e - r * w;
+        const double z = 1.0 / sqrt(d[r]);
+        const cx<double> o = to<double>(O[r * ld + c]), x = to<double>(X[r * ld + c]);
+        O[r * ld + c] = to<VT>(cx<double>{z * (o.re + c0 * x.re), z * (o.im + c0 * x.im)});
+    }
 }
 
 }  // namespace kmsd

@@ -66,83 +66,107 @@ void matrix_release(kmsd_matrix *m) {
     kmsd_ctx *c = m->ctx;
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
-    const size_t vb = m->fmt == KMSD_BSR4 ? 16 : 1;
-    const size_t vals = size_t(m->nnz) * vb;
-    const size_t mult = m->vkind == VK_COMPLEX ? 2 : 1;
-    const int64_t rows = m->fmt == KMSD_BSR4 ? m->n / 4 : m->n;
-    dfree(c, m->ptr, sizeof(int32_t) * size_t(rows + 1));
-    dfree(c, m->idx, sizeof(int32_t) * size_t(m->nnz));
-    dfree(c, m->val64, sizeof(double) * vals * mult);
-    dfree(c, m->val32, sizeof(float) * vals * mult);
+    for (void *p : {static_cast<void *>(m->ptr), static_cast<void *>(m->idx), m->val64, m->val32, m->dval64,
+                    m->dval32})
+        if (p) (void)hipFree(p);
+    c->used -= std::min<uint64_t>(c->used, m->bytes);
+    m->bytes = 0;
     m->ptr = m->idx = nullptr;
-    m->val64 = m->val32 = nullptr;
+    m->val64 = m->val32 = m->dval64 = m->dval32 = nullptr;
 }
+
+namespace {
+template <class T>
+T *upload(kmsd_ctx *c, kmsd_matrix *m, const std::vector<T> &h) {
+    const size_t bytes = sizeof(T) * std::max<size_t>(1, h.size());
+    T *d = static_cast<T *>(dalloc(c, bytes));
+    m->bytes += bytes;
+    if (!h.empty()) HIPCHECK(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    return d;
+}
+}  // namespace
 
 void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
                    const int64_t *colidx, const double *values) {
-    if (fmt != KMSD_CSR && fmt != KMSD_BSR4) throw Error(KMSD_EINVAL, "unknown matrix format");
+    if (fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "unknown matrix format");
     if (n < 1 || nnz < 0 || !rowptr || (nnz && (!colidx || !values)))
         throw Error(KMSD_EINVAL, "invalid CSR dimensions/threads");
-    if (fmt == KMSD_BSR4 && n % 4) throw Error(KMSD_EINVAL, "BSR4 requires n divisible by 4");
-    const int64_t rows = fmt == KMSD_BSR4 ? n / 4 : n;
     if (n >= (int64_t(1) << 31) || nnz >= (int64_t(1) << 31))
         throw Error(KMSD_EINVAL, "matrix exceeds 32-bit device indexing");
-    if (rowptr[0] != 0 || rowptr[rows] != nnz) throw Error(KMSD_EINVAL, "invalid CSR row pointers");
-    for (int64_t r = 0; r < rows; ++r)
+    if (rowptr[0] != 0 || rowptr[n] != nnz) throw Error(KMSD_EINVAL, "invalid CSR row pointers");
+    for (int64_t r = 0; r < n; ++r)
         if (rowptr[r] > rowptr[r + 1]) throw Error(KMSD_EINVAL, "nonmonotone CSR row pointers");
-    const int64_t vpb = fmt == KMSD_BSR4 ? 16 : 1;
     bool any_re = false, any_im = false;
     for (int64_t k = 0; k < nnz; ++k) {
-        if (colidx[k] < 0 || colidx[k] >= rows) throw Error(KMSD_EINVAL, "invalid CSR index or nonfinite entry");
-        for (int64_t q = 0; q < vpb; ++q) {
-            const double re = values[2 * (k * vpb + q)], im = values[2 * (k * vpb + q) + 1];
-            if (!std::isfinite(re) || !std::isfinite(im))
-                throw Error(KMSD_EINVAL, "invalid CSR index or nonfinite entry");
-            any_re |= re != 0.0;
-            any_im |= im != 0.0;
-        }
+        const double re = values[2 * k], im = values[2 * k + 1];
+        if (colidx[k] < 0 || colidx[k] >= n || !std::isfinite(re) || !std::isfinite(im))
+            throw Error(KMSD_EINVAL, "invalid CSR index or nonfinite entry");
+        any_re |= re != 0.0;
+        any_im |= im != 0.0;
     }
     m->fmt = fmt;
     m->n = n;
     m->nnz = nnz;
     m->vkind = any_im ? (any_re ? VK_COMPLEX : VK_IMAG) : VK_REAL;
-    // identity detection (CSR only)
-    if (fmt == KMSD_CSR && nnz == n) {
+    const int comps = m->vkind == VK_COMPLEX ? 2 : 1;
+    const int off = m->vkind == VK_IMAG ? 1 : 0;
+    // Split: dense diagonal (duplicates summed) + off-diagonal entries.
+    std::vector<double> dg(size_t(n) * comps, 0.0);
+    std::vector<int64_t> cnt(n, 0);
+    int64_t maxlen = 0, noff = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            if (colidx[k] == r) {
+                for (int q = 0; q < comps; ++q) dg[size_t(r) * comps + q] += values[2 * k + off + q];
+            } else {
+                ++cnt[r];
+            }
+        }
+        maxlen = std::max(maxlen, cnt[r]);
+        noff += cnt[r];
+    }
+    m->has_diag = std::any_of(dg.begin(), dg.end(), [](double v) { return v != 0.0; });
+    if (nnz == n) {
         bool id = true;
         for (int64_t r = 0; r < n && id; ++r)
             id = rowptr[r + 1] - rowptr[r] == 1 && colidx[rowptr[r]] == r && values[2 * rowptr[r]] == 1.0 &&
                  values[2 * rowptr[r] + 1] == 0.0;
         m->identity = id;
     }
-    std::vector<int32_t> p32(rows + 1), i32(nnz);
-    for (int64_t r = 0; r <= rows; ++r) p32[r] = int32_t(rowptr[r]);
-    for (int64_t k = 0; k < nnz; ++k) i32[k] = int32_t(colidx[k]);
-    const size_t nv = size_t(nnz * vpb);
-    std::vector<double> v64;
-    std::vector<float> v32;
-    if (m->vkind == VK_COMPLEX) {
-        v64.assign(values, values + 2 * nv);
-        v32.resize(2 * nv);
-        for (size_t e = 0; e < 2 * nv; ++e) v32[e] = float(v64[e]);
-    } else {
-        const int off = m->vkind == VK_IMAG ? 1 : 0;
-        v64.resize(nv);
-        v32.resize(nv);
-        for (size_t e = 0; e < nv; ++e) {
-            v64[e] = values[2 * e + off];
-            v32[e] = float(v64[e]);
+    // ELL when padding adds at most ~25% entries.
+    const bool use_ell = maxlen > 0 && maxlen * n <= noff + noff / 4 + n && maxlen <= 64;
+    m->ell = use_ell ? int32_t(maxlen) : 0;
+    const int64_t stored = use_ell ? maxlen * n : noff;
+    m->noff = stored;
+    std::vector<int32_t> i32(stored), p32;
+    std::vector<double> v64(size_t(stored) * comps, 0.0);
+    if (!use_ell) p32.resize(n + 1);
+    int64_t pos = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        if (!use_ell) p32[r] = int32_t(pos);
+        int64_t q = use_ell ? r * maxlen : pos;
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            if (colidx[k] == r) continue;
+            i32[q] = int32_t(colidx[k]);
+            for (int t = 0; t < comps; ++t) v64[size_t(q) * comps + t] = values[2 * k + off + t];
+            ++q;
+        }
+        if (use_ell) {
+            for (; q < (r + 1) * maxlen; ++q) i32[q] = int32_t(r);  // zero-valued padding
+        } else {
+            pos = q;
         }
     }
-    m->ptr = static_cast<int32_t *>(dalloc(c, sizeof(int32_t) * p32.size()));
-    m->idx = static_cast<int32_t *>(dalloc(c, sizeof(int32_t) * std::max<size_t>(1, i32.size())));
-    m->val64 = dalloc(c, sizeof(double) * std::max<size_t>(1, v64.size()));
-    m->val32 = dalloc(c, sizeof(float) * std::max<size_t>(1, v32.size()));
-    HIPCHECK(hipMemcpyAsync(m->ptr, p32.data(), sizeof(int32_t) * p32.size(), hipMemcpyHostToDevice, c->stream));
-    if (nnz) {
-        HIPCHECK(hipMemcpyAsync(m->idx, i32.data(), sizeof(int32_t) * i32.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHECK(hipMemcpyAsync(m->val64, v64.data(), sizeof(double) * v64.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHECK(hipMemcpyAsync(m->val32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, c->stream));
-    }
+    if (!use_ell) p32[n] = int32_t(pos);
+    std::vector<float> v32(v64.size()), d32(dg.size());
+    for (size_t e = 0; e < v64.size(); ++e) v32[e] = float(v64[e]);
+    for (size_t e = 0; e < dg.size(); ++e) d32[e] = float(dg[e]);
+    if (!use_ell) m->ptr = upload(c, m, p32);
+    m->idx = upload(c, m, i32);
+    m->val64 = upload(c, m, v64);
+    m->val32 = upload(c, m, v32);
+    m->dval64 = upload(c, m, dg);
+    m->dval32 = upload(c, m, d32);
     HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -164,21 +188,38 @@ int ilog2_ceil(int64_t v) {
     return l;
 }
 
-Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype) {
+Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max = 4) {
     Shape s;
-    if (dtype == KMSD_C64) {
-        s.cpl = c->cpl ? c->cpl : 2;
-        if (w % s.cpl) s.cpl = 1;
-    }
+    const int want = c->cpl ? c->cpl : (dtype == KMSD_C64 ? 2 : 1);
+    s.cpl = std::min(want, std::min(cpl_max, dtype == KMSD_C64 ? 4 : 2));
+    while (s.cpl > 1 && w % s.cpl) s.cpl >>= 1;
     const int64_t lanes = (w + s.cpl - 1) / s.cpl;
     s.lpr_log2 = std::min(6, ilog2_ceil(lanes));
     const int64_t lpr = int64_t(1) << s.lpr_log2;
     s.gy = int((lanes + lpr - 1) / lpr);
     const int64_t rpb = 4 * (64 / lpr);
     const int64_t nblk = (n + rpb - 1) / rpb;
-    const int64_t want = ((nblk + 7) / 8) * 8;
-    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, want));
+    const int64_t want_g = ((nblk + 7) / 8) * 8;
+    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, want_g));
     return s;
+}
+
+// Largest columns-per-lane every operand supports (pointer alignment and
+// leading dimensions multiples of cpl).
+int cpl_limit(const Step &s, int dtype) {
+    const size_t eb = elem_bytes(dtype);
+    int lim = 4;
+    auto fit = [&](const void *p, int64_t ld) {
+        if (!p) return;
+        while (lim > 1 && ((reinterpret_cast<uintptr_t>(p) % (eb * lim)) || (ld % lim))) lim >>= 1;
+    };
+    fit(s.X, s.ldx);
+    fit(s.Z, s.ldz);
+    fit(s.P, s.ldp);
+    fit(s.OUT, s.ldo);
+    fit(s.W, s.ldw);
+    fit(s.ACC, s.lda);
+    return lim;
 }
 
 template <class VT, class MR, bool MC, class AT, int CPL>
@@ -199,11 +240,25 @@ int64_t blocks_for(int64_t total) { return std::max<int64_t>(1, std::min<int64_t
 
 }  // namespace
 
-int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype) { return shape_for(c, n, w, dtype).gx; }
+int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap) {
+    return shape_for(c, n, w, dtype, cpl_cap).gx;
+}
+
+int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t>> blocks) {
+    const size_t eb = elem_bytes(dtype);
+    int lim = 4;
+    for (const auto &b : blocks) {
+        if (!b.first) continue;
+        while (lim > 1 && ((reinterpret_cast<uintptr_t>(b.first) % (eb * lim)) || (b.second % lim))) lim >>= 1;
+    }
+    return lim;
+}
 
 int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     if (s.n < 1 || s.w < 1) return 0;
-    const Shape sh = shape_for(c, s.n, s.w, dtype);
+    const Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)));
+    if (s.expect_groups && sh.gx != s.expect_groups)
+        throw Error(KMSD_EDEVICE, "internal error: launch shape changed within a reduction chain");
     StepArgs a{};
     uint32_t fl = 0;
     Cplx alpha = s.alpha;
@@ -213,8 +268,12 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
         fl |= F_SPMM;
         a.ptr = A->ptr;
         a.idx = A->idx;
+        a.ell = A->ell;
+        if (A->ell) fl |= F_ELL;
+        if (A->has_diag) fl |= F_DIAG;
         const bool f32 = dtype == KMSD_C64 && c->fp32_math;
         a.val = f32 ? A->val32 : A->val64;
+        a.dval = f32 ? A->dval32 : A->dval64;
         if (A->vkind == VK_IMAG) alpha = Cplx{-s.alpha.im, s.alpha.re};  // A = i * stored
     }
     if (s.use_beta) fl |= F_BETA;
@@ -246,16 +305,24 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     a.w = s.w;
     a.lpr_log2 = sh.lpr_log2;
     a.flags = fl;
+    a.gate = s.gate;
     const bool cv = A && A->vkind == VK_COMPLEX;
     if (dtype == KMSD_C128) {
-        go_vk<double, double, double, 1>(c, sh, a, cv);
-    } else if (c->fp32_math) {
         if (sh.cpl == 2)
+            go_vk<double, double, double, 2>(c, sh, a, cv);
+        else
+            go_vk<double, double, double, 1>(c, sh, a, cv);
+    } else if (c->fp32_math) {
+        if (sh.cpl == 4)
+            go_vk<float, float, float, 4>(c, sh, a, cv);
+        else if (sh.cpl == 2)
             go_vk<float, float, float, 2>(c, sh, a, cv);
         else
             go_vk<float, float, float, 1>(c, sh, a, cv);
     } else {
-        if (sh.cpl == 2)
+        if (sh.cpl == 4)
+            go_vk<float, double, double, 4>(c, sh, a, cv);
+        else if (sh.cpl == 2)
             go_vk<float, double, double, 2>(c, sh, a, cv);
         else
             go_vk<float, double, double, 1>(c, sh, a, cv);
@@ -339,6 +406,63 @@ void spmm(kmsd_ctx *c, int dtype, const kmsd_matrix *A, const void *X, int64_t l
         s.ldo = ldo;
     }
     launch_step(c, dtype, s);
+}
+
+kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const double *diag) {
+    kmsd_matrix *W = new kmsd_matrix(*S);
+    W->bytes = 0;
+    W->ptr = W->idx = nullptr;
+    W->val64 = W->val32 = W->dval64 = W->dval32 = nullptr;
+    try {
+        const int comps = S->vkind == VK_COMPLEX ? 2 : 1;
+        const int64_t rows = S->n;
+        auto dup = [&](const void *src, size_t bytes) {
+            void *d = dalloc(c, bytes);
+            W->bytes += bytes;
+            HIPCHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+            return d;
+        };
+        const size_t nv = size_t(std::max<int64_t>(1, S->noff)) * comps;
+        if (S->ptr) W->ptr = static_cast<int32_t *>(dup(S->ptr, sizeof(int32_t) * size_t(rows + 1)));
+        W->idx = static_cast<int32_t *>(dup(S->idx, sizeof(int32_t) * size_t(std::max<int64_t>(1, S->noff))));
+        W->val64 = dup(S->val64, sizeof(double) * nv);
+        W->val32 = dup(S->val32, sizeof(float) * nv);
+        W->dval64 = dup(S->dval64, sizeof(double) * size_t(rows) * comps);
+        W->dval32 = dup(S->dval32, sizeof(float) * size_t(rows) * comps);
+        const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((rows + 255) / 256, 4096)));
+        if (S->vkind == VK_COMPLEX) {
+            hipLaunchKernelGGL((congruence_scale_kernel<double, true>), dim3(g), dim3(256), 0, c->stream, W->val64,
+                               W->dval64, W->ptr, W->idx, W->ell, diag, rows);
+            hipLaunchKernelGGL((congruence_scale_kernel<float, true>), dim3(g), dim3(256), 0, c->stream, W->val32,
+                               W->dval32, W->ptr, W->idx, W->ell, diag, rows);
+        } else {
+            hipLaunchKernelGGL((congruence_scale_kernel<double, false>), dim3(g), dim3(256), 0, c->stream, W->val64,
+                               W->dval64, W->ptr, W->idx, W->ell, diag, rows);
+            hipLaunchKernelGGL((congruence_scale_kernel<float, false>), dim3(g), dim3(256), 0, c->stream, W->val32,
+                               W->dval32, W->ptr, W->idx, W->ell, diag, rows);
+        }
+        HIPCHECK(hipGetLastError());
+        W->identity = false;
+        W->has_diag = true;
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        matrix_release(W);
+        delete W;
+        throw;
+    }
+    return W;
+}
+
+void launch_sample_finish(kmsd_ctx *c, int dtype, void *out, const void *xi, int64_t ld, int64_t n, int32_t w,
+                          const double *diag, double c0) {
+    const int64_t g = blocks_for(n * w);
+    if (dtype == KMSD_C128)
+        hipLaunchKernelGGL(sample_finish_kernel<double>, dim3(unsigned(g)), dim3(256), 0, c->stream, out, xi, ld, n,
+                           w, diag, c0);
+    else
+        hipLaunchKernelGGL(sample_finish_kernel<float>, dim3(unsigned(g)), dim3(256), 0, c->stream, out, xi, ld, n, w,
+                           diag, c0);
+    HIPCHECK(hipGetLastError());
 }
 
 }  // namespace kmsd

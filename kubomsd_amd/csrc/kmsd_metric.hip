@@ -82,6 +82,11 @@ extern "C" int kmsd_metric_destroy(kmsd_metric *m) {
     kmsd_ctx *c = m->ctx;
     (void)hipStreamSynchronize(c->stream);
     pcg_release(m);
+    if (m->W) {
+        matrix_release(m->W);
+        delete m->W;
+        m->W = nullptr;
+    }
     const size_t nfac_bytes = m->bytes;  // accounted as a whole
     if (m->diag) (void)hipFree(m->diag);
     if (m->block_ptr) (void)hipFree(m->block_ptr);
@@ -164,5 +169,105 @@ void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y
 extern "C" int kmsd_metric_sample(kmsd_metric *m, const void *xi, void *out, int64_t nb, int dtype, const double *coeff,
                                   int64_t ncoeff, double center, double half) {
     if (!m) return KMSD_EINVAL;
-    return guard(m->ctx, [&] { throw Error(KMSD_EINVAL, "metric sampling is not available in this build"); });
+    kmsd_ctx *c = m->ctx;
+    return guard(c, [&] {
+        check_block_args(xi, out, nb, dtype);
+        if (ncoeff < 1 || !coeff) throw Error(KMSD_EINVAL, "invalid sampling dimensions");
+        const int64_t n = m->n;
+        const int32_t w = int32_t(nb);
+        require_finite(c, dtype, xi, nb, n, w);
+        if (ncoeff == 1) {
+            launch_scale(c, dtype, xi, nb, out, nb, n, w, Cplx{0, 0});
+        } else {
+            if (!(half > 0)) throw Error(KMSD_EINVAL, "sampling half-width must be positive");
+            if (!m->W) m->W = matrix_congruence_copy(c, m->S, m->diag);
+            // Clenshaw: b_k = 2 A b_{k+1} - b_{k+2} + c_k xi, A = (W - center)/half
+            const size_t bytes = size_t(n) * nb * elem_bytes(dtype);
+            void *u = m->sb1.ensure(c, bytes);
+            void *v = m->sb2.ensure(c, bytes);
+            launch_scale(c, dtype, xi, nb, u, nb, n, w, Cplx{coeff[ncoeff - 1], 0});  // b_{K-1}
+            launch_scale(c, dtype, xi, nb, v, nb, n, w, Cplx{0, 0});                 // b_K = 0
+            void *cur = u, *prev = v;  // cur = b_{k+1}, prev = b_{k+2}
+            for (int64_t k = ncoeff - 2; k >= 1; --k) {
+                Step s;  // new = 2 A cur - prev, written over prev
+                s.A = m->W;
+                s.X = cur;
+                s.ldx = nb;
+                s.alpha = Cplx{2 / half, 0};
+                s.beta = Cplx{-2 * center / half, 0};
+                s.use_beta = true;
+                s.P = prev;
+                s.ldp = nb;
+                s.gamma = -1.0;
+                s.use_p = true;
+                s.OUT = prev;
+                s.ldo = nb;
+                s.n = n;
+                s.w = w;
+                launch_step(c, dtype, s);
+                Step t;  // new += c_k xi
+                t.Z = xi;
+                t.ldz = nb;
+                t.alpha = Cplx{coeff[k], 0};
+                t.ACC = prev;
+                t.lda = nb;
+                t.n = n;
+                t.w = w;
+                launch_step(c, dtype, t);
+                std::swap(cur, prev);
+            }
+            // out = A b_1 - b_2 ; then out = z (out + c_0 xi)
+            Step s;
+            s.A = m->W;
+            s.X = cur;
+            s.ldx = nb;
+            s.alpha = Cplx{1 / half, 0};
+            s.beta = Cplx{-center / half, 0};
+            s.use_beta = true;
+            s.P = prev;
+            s.ldp = nb;
+            s.gamma = -1.0;
+            s.use_p = true;
+            s.OUT = out;
+            s.ldo = nb;
+            s.n = n;
+            s.w = w;
+            launch_step(c, dtype, s);
+        }
+        launch_sample_finish(c, dtype, out, xi, nb, n, w, m->diag, coeff[0]);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int kmsd_metric_inner(kmsd_metric *m, const void *A, const void *B, double *out, int64_t nb, int dtype) {
+    if (!m) return KMSD_EINVAL;
+    kmsd_ctx *c = m->ctx;
+    return guard(c, [&] {
+        check_block_args(A, B, nb, dtype);
+        if (!out) throw Error(KMSD_EINVAL, "null output");
+        const int64_t n = m->n;
+        const int32_t w = int32_t(nb);
+        const int cap = cpl_cap_for(dtype, {{A, nb}, {B, nb}});
+        const int64_t G = step_groups(c, n, w, dtype, cap);
+        double *part = static_cast<double *>(m->pinner.ensure(c, sizeof(double) * 2 * size_t(G) * size_t(w)));
+        Step s;
+        if (m->identity) {
+            s.Z = B;
+            s.ldz = nb;
+        } else {
+            s.A = m->S;
+            s.X = B;
+            s.ldx = nb;
+        }
+        s.W = A;
+        s.ldw = nb;
+        s.part = part;
+        s.n = n;
+        s.w = w;
+        s.cpl_cap = cap;
+        s.expect_groups = G;
+        launch_step(c, dtype, s);
+        launch_reduce(c, part, G, w, 1, out, nb, 1);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
 }

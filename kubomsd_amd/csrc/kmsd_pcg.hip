@@ -1,9 +1,459 @@
-// Batched PCG on the device (reference Metric::solve, cpp/native.cpp:313-452).
+// Batched PCG metric solve on the device (reference Metric::solve,
+// cpp/native.cpp:313-452): per-column RHS scaling by the largest component,
+// threshold max(rtol |q|, atol/safe), curvature breakdown test, true-residual
+// refresh every `check_interval` iterations (fixed) or on candidate
+// convergence (adaptive), final original-equation acceptance, rescale with
+// representability checks.  Column state stays on the device; the host polls
+// a flag block every `pcg_poll` iterations.
+//
+// complex64 right-hand sides run FP32 PCG inner solves inside an FP64
+// iterative refinement: the residual b - S y and the solution accumulator y
+// are complex128, and acceptance is decided on that FP64 residual.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "kmsd_pcg.hpp"
+#include "kmsd_pcg_kernels.hpp"
+
+using namespace kmsd;
+
+struct kmsd_metric::Work {
+    DBuf q, y, r, z, p, sp, t, cs, fl, part, part2;
+    DBuf b128, y128, r128, r64, d64, cs2;
+};
 
 namespace kmsd {
-void pcg_solve(kmsd_metric *, int, const void *, int64_t, void *, int64_t, int32_t, double, double, int) {
-    throw Error(KMSD_EINVAL, "nonidentity metric solves are not available in this build");
+
+namespace {
+
+struct RowShape {
+    int cpl, lpr_log2, gy;
+    int64_t gx;
+};
+
+int ilog2c(int64_t v) {
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
 }
-void pcg_release(kmsd_metric *) {}
+
+RowShape row_shape(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cap) {
+    RowShape s;
+    s.cpl = std::min(cap, dtype == KMSD_C64 ? 2 : 1);
+    while (s.cpl > 1 && w % s.cpl) s.cpl >>= 1;
+    const int64_t lanes = (w + s.cpl - 1) / s.cpl;
+    s.lpr_log2 = std::min(6, ilog2c(lanes));
+    const int64_t lpr = int64_t(1) << s.lpr_log2;
+    s.gy = int((lanes + lpr - 1) / lpr);
+    const int64_t rpb = 4 * (64 / lpr);
+    const int64_t nblk = (n + rpb - 1) / rpb;
+    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, ((nblk + 7) / 8) * 8));
+    return s;
+}
+
+struct Solver {
+    kmsd_metric *m;
+    kmsd_ctx *c;
+    int dtype;
+    int64_t n;
+    int32_t w;
+    size_t eb;
+    RowShape rs;
+    void *Q, *Y, *R, *Z, *P, *SP, *T;
+    ColState *cs;
+    PcgFlags *fl;
+    double *part;   // row-kernel partials [rs.gx][w]
+    double *part2;  // step-kernel partials
+    int64_t g2 = 0;
+
+    void row(int stage, const void *B = nullptr, int64_t ldb = 0, void *OUT = nullptr, int64_t ldo = 0,
+             bool require_active = false, const int32_t *gate = nullptr) {
+        PcgArgs a{};
+        a.B = B;
+        a.Q = Q;
+        a.Y = Y;
+        a.R = R;
+        a.Z = Z;
+        a.P = P;
+        a.SP = SP;
+        a.T = T;
+        a.OUT = OUT;
+        a.diag = m->diag;
+        a.cs = cs;
+        a.fl = fl;
+        a.part = part;
+        a.n = n;
+        a.ldb = ldb;
+        a.ldo = ldo;
+        a.w = w;
+        a.lpr_log2 = rs.lpr_log2;
+        a.stage = stage;
+        a.require_active = require_active;
+        a.gate = gate;
+        const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
+        if (dtype == KMSD_C128)
+            hipLaunchKernelGGL((pcg_row_kernel<double, 1>), grid, dim3(256), 0, c->stream, a);
+        else if (rs.cpl == 2)
+            hipLaunchKernelGGL((pcg_row_kernel<float, 2>), grid, dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((pcg_row_kernel<float, 1>), grid, dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+    }
+
+    void col(int stage, const double *p, int64_t groups, double rtol, double atol, bool refresh = false,
+             const int32_t *gate = nullptr) {
+        ColArgs a{};
+        a.part = p;
+        a.groups = groups;
+        a.cs = cs;
+        a.fl = fl;
+        a.rtol = rtol;
+        a.atol = atol;
+        a.gap_ratio = m->gap_ratio;
+        a.w = w;
+        a.stage = stage;
+        a.adaptive = m->adaptive;
+        a.refresh = refresh;
+        a.gate = gate;
+        hipLaunchKernelGGL(pcg_col_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+    }
+
+    // OUT = alpha * S X + gamma * Pt with partial <W, OUT>
+    void spmv(const void *X, void *OUT, const void *Pt, double alpha, double gamma, const void *Wp,
+              const int32_t *gate) {
+        Step s;
+        s.A = m->S;
+        s.X = X;
+        s.ldx = w;
+        s.OUT = OUT;
+        s.ldo = w;
+        s.alpha = Cplx{alpha, 0};
+        if (Pt) {
+            s.P = Pt;
+            s.ldp = w;
+            s.gamma = gamma;
+            s.use_p = true;
+        }
+        if (Wp) {
+            s.W = Wp;
+            s.ldw = w;
+            s.part = part2;
+        }
+        s.n = n;
+        s.w = w;
+        s.gate = gate;
+        g2 = launch_step(c, dtype, s);
+    }
+
+    void precondition(bool require_active, const int32_t *gate = nullptr) {
+        if (m->nblocks == 0) {
+            row(PS_DIAG, nullptr, 0, nullptr, 0, require_active, gate);
+            return;
+        }
+        const int64_t total = m->nblocks * w;
+        const int64_t g = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 16384));
+        if (dtype == KMSD_C128) {
+            if (m->max_block <= 4)
+                hipLaunchKernelGGL((block_precond_kernel<double, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream, R, Z,
+                                   int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
+            else
+                hipLaunchKernelGGL((block_precond_kernel<double, 64>), dim3(unsigned(g)), dim3(256), 0, c->stream, R,
+                                   Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
+        } else {
+            if (m->max_block <= 4)
+                hipLaunchKernelGGL((block_precond_kernel<float, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream, R, Z,
+                                   int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
+            else
+                hipLaunchKernelGGL((block_precond_kernel<float, 64>), dim3(unsigned(g)), dim3(256), 0, c->stream, R,
+                                   Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
+        }
+        HIPCHECK(hipGetLastError());
+        row(PS_DOT, nullptr, 0, nullptr, 0, require_active, gate);
+    }
+
+    void clear_flag(size_t offset, size_t bytes) {
+        HIPCHECK(hipMemsetAsync(reinterpret_cast<char *>(fl) + offset, 0, bytes, c->stream));
+    }
+
+    PcgFlags read_flags() {
+        PcgFlags h{};
+        HIPCHECK(hipMemcpyAsync(&h, fl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return h;
+    }
+};
+
+Solver make_solver(kmsd_metric *m, int dtype, int32_t nb, bool second_state = false) {
+    kmsd_ctx *c = m->ctx;
+    if (!m->work) m->work = new kmsd_metric::Work();
+    kmsd_metric::Work &wk = *m->work;
+    Solver s;
+    s.m = m;
+    s.c = c;
+    s.dtype = dtype;
+    s.n = m->n;
+    s.w = nb;
+    s.eb = elem_bytes(dtype);
+    const size_t bytes = size_t(s.n) * size_t(nb) * s.eb;
+    s.Q = wk.q.ensure(c, bytes);
+    s.Y = wk.y.ensure(c, bytes);
+    s.R = wk.r.ensure(c, bytes);
+    s.Z = wk.z.ensure(c, bytes);
+    s.P = wk.p.ensure(c, bytes);
+    s.SP = wk.sp.ensure(c, bytes);
+    s.T = wk.t.ensure(c, bytes);
+    s.cs = static_cast<ColState *>((second_state ? wk.cs2 : wk.cs).ensure(c, sizeof(ColState) * size_t(nb)));
+    s.fl = static_cast<PcgFlags *>(wk.fl.ensure(c, 256));
+    s.rs = row_shape(c, s.n, nb, dtype, 4);
+    const size_t pg = size_t(std::max<int64_t>(s.rs.gx, c->groups)) * size_t(nb) * 2 * sizeof(double);
+    s.part = static_cast<double *>(wk.part.ensure(c, pg));
+    s.part2 = static_cast<double *>(wk.part2.ensure(c, pg));
+    return s;
+}
+
+struct Outcome {
+    std::vector<ColState> cs;
+    PcgFlags fl;
+};
+
+// PCG in the solver's own precision.  B (ldb) is any block of that dtype;
+// the solution (times the column scale) lands in OUT (ldo).  Never throws on
+// numerical failure: the caller inspects the outcome.
+Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, double rtol, double atol,
+                int maxiter) {
+    kmsd_metric *m = S.m;
+    kmsd_ctx *c = S.c;
+    const int32_t w = S.w;
+    S.clear_flag(0, sizeof(PcgFlags));
+    // 1. per-column scale
+    S.row(PS_ABSMAX, B, ldb);
+    S.col(PC_SAFE, S.part, S.rs.gx, rtol, atol);
+    // 2. q = b/safe, r = q, y = 0 ; |q| -> thr, activity
+    S.row(PS_SCALE, B, ldb);
+    S.col(PC_INIT, S.part, S.rs.gx, rtol, atol);
+    // 3. z = M^-1 r, rz
+    S.precondition(false);
+    S.col(PC_RZ0, S.part, S.rs.gx, rtol, atol);
+    // 4. p = active ? z : 0
+    S.row(PS_PINIT);
+    const int32_t *any = &S.fl->any_active;
+    const int32_t *cand = &S.fl->need_refresh;
+    const int poll = std::max(1, c->pcg_poll);
+    for (int k = 0; k < maxiter; ++k) {
+        if (k % poll == 0) {
+            const PcgFlags h = S.read_flags();
+            if (h.broke || h.any_active == 0) break;
+        }
+        const bool refresh = (k + 1) % m->check_interval == 0;
+        // sp = S p ; curvature <p, sp>
+        S.spmv(S.P, S.SP, nullptr, 1.0, 0.0, S.P, any);
+        S.col(PC_ALPHA, S.part2, S.g2, rtol, atol, false, any);
+        // y += alpha p ; r -= alpha sp ; |r|
+        S.row(PS_UPDATE, nullptr, 0, nullptr, 0, true);
+        S.clear_flag(offsetof(PcgFlags, need_refresh), sizeof(int32_t));
+        S.col(PC_RN, S.part, S.rs.gx, rtol, atol, refresh, any);
+        if (refresh || m->adaptive) {
+            const int32_t *gate = refresh ? any : cand;
+            // t = q - S y ; |t|, |t - r| ; restart / activity
+            S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, gate);
+            S.row(PS_TRUEGAP, nullptr, 0, nullptr, 0, true, gate);
+            S.col(PC_TRUE, S.part, S.rs.gx, rtol, atol, refresh, gate);
+            S.row(PS_RESTART, nullptr, 0, nullptr, 0, true, gate);
+        }
+        // z = M^-1 r ; beta
+        S.precondition(true);
+        S.clear_flag(offsetof(PcgFlags, any_active), sizeof(int32_t));
+        S.col(PC_BETA, S.part, S.rs.gx, rtol, atol);
+        S.row(PS_PUPDATE, nullptr, 0, nullptr, 0, true);
+    }
+    // final acceptance on the original (scaled) equation
+    S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
+    S.row(PS_TRUEGAP);
+    S.col(PC_FINAL, S.part, S.rs.gx, rtol, atol);
+    S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
+    Outcome o;
+    o.cs.resize(w);
+    HIPCHECK(hipMemcpyAsync(o.cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(&o.fl, S.fl, sizeof(PcgFlags), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return o;
+}
+
+void store_report(kmsd_metric *m, const std::vector<ColState> &cs, const std::vector<double> *tn_override = nullptr,
+                  const std::vector<double> *qn_override = nullptr, const std::vector<double> *thr_override = nullptr) {
+    const size_t w = cs.size();
+    m->rep_conv.assign(w, 0);
+    m->rep_brk.assign(w, 0);
+    m->rep_its.assign(w, 0);
+    m->rep_res.assign(w, 0.0);
+    m->rep_rel.assign(w, 0.0);
+    for (size_t j = 0; j < w; ++j) {
+        const ColState &s = cs[j];
+        const double tn = tn_override ? (*tn_override)[j] : s.tn;
+        const double qn = qn_override ? (*qn_override)[j] : s.qn;
+        const double thr = thr_override ? (*thr_override)[j] : s.thr;
+        const double scale = tn_override ? 1.0 : s.safe;
+        m->rep_brk[j] = uint8_t(s.breakdown != 0);
+        m->rep_conv[j] = uint8_t(tn <= thr && !s.breakdown && s.fin);
+        m->rep_its[j] = s.iters;
+        m->rep_rel[j] = qn > 0 ? tn / qn : 0.0;
+        m->rep_res[j] = tn * scale;
+        m->max_relative = std::max(m->max_relative, m->rep_rel[j]);
+    }
+}
+
+void account(kmsd_metric *m, const Outcome &o, int64_t extra_checks) {
+    ++m->calls;
+    int64_t mx = 0;
+    for (const ColState &s : o.cs) mx = std::max<int64_t>(mx, s.iters);
+    m->iterations += mx;
+    m->refreshes += o.fl.refreshes;
+    m->residual_checks += o.fl.checks + extra_checks;
+    m->restarts += o.fl.restarts;
+}
+
+}  // namespace
+
+void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb, double rtol,
+               double atol, int maxiter) {
+    kmsd_ctx *c = m->ctx;
+    if (dtype == KMSD_C128) {
+        Solver S = make_solver(m, dtype, nb);
+        const Outcome o = pcg_run(S, B, ldb, Y, ldy, rtol, atol, maxiter);
+        store_report(m, o.cs);
+        account(m, o, 1);
+        bool ok = true, fin = true, brk = false;
+        for (const ColState &s : o.cs) {
+            fin = fin && s.fin;
+            brk = brk || s.breakdown;
+        }
+        for (size_t j = 0; j < m->rep_conv.size(); ++j) ok = ok && m->rep_conv[j];
+        if (!ok)
+            throw Error(KMSD_ENUMERIC,
+                        !fin ? "S solve solution is nonfinite or not representable in complex128 after rescaling"
+                        : brk ? "PCG non-positive/complex curvature breakdown"
+                              : "S solve failed true-residual acceptance");
+        return;
+    }
+    // complex64: FP32 inner PCG inside FP64 iterative refinement.
+    if (!m->work) m->work = new kmsd_metric::Work();
+    kmsd_metric::Work &wk = *m->work;
+    const int64_t n = m->n;
+    const size_t b128 = size_t(n) * nb * 16, b64 = size_t(n) * nb * 8;
+    void *bb = wk.b128.ensure(c, b128);
+    void *yy = wk.y128.ensure(c, b128);
+    void *rr = wk.r128.ensure(c, b128);
+    void *r64 = wk.r64.ensure(c, b64);
+    void *d64 = wk.d64.ensure(c, b64);
+    ColState *cs2 = static_cast<ColState *>(wk.cs2.ensure(c, sizeof(ColState) * size_t(nb)));
+    double *nrm = static_cast<double *>(wk.part2.ensure(c, size_t(std::max<int64_t>(c->groups, 8)) * nb * 16));
+    launch_convert(c, KMSD_C64, B, ldb, KMSD_C128, bb, nb, n, nb);
+    launch_scale(c, KMSD_C128, bb, nb, yy, nb, n, nb, Cplx{0, 0});
+    launch_scale(c, KMSD_C128, bb, nb, rr, nb, n, nb, Cplx{1, 0});
+    // ||b|| per column (FP64) -> threshold
+    auto colnorm = [&](const void *X) {
+        Step s;
+        s.Z = X;
+        s.ldz = nb;
+        s.W = X;
+        s.ldw = nb;
+        s.part = nrm;
+        s.n = n;
+        s.w = nb;
+        const int64_t g = launch_step(c, KMSD_C128, s);
+        std::vector<double> h(size_t(g) * nb * 2), out(nb, 0.0);
+        HIPCHECK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int64_t q = 0; q < g; ++q)
+            for (int32_t j = 0; j < nb; ++j) out[j] += h[(size_t(q) * nb + j) * 2];
+        for (double &v : out) v = std::sqrt(v);
+        return out;
+    };
+    const std::vector<double> bn = colnorm(bb);
+    std::vector<double> thr(nb);
+    for (int32_t j = 0; j < nb; ++j) thr[j] = std::max(rtol * bn[j], atol);
+    std::vector<double> rn = bn;
+    std::vector<ColState> agg(nb);
+    for (auto &s : agg) {
+        s = ColState{};
+        s.fin = 1;
+        s.safe = 1.0;
+    }
+    const double inner_rtol = std::max(rtol, 1e-6);
+    int64_t checks = 1;
+    for (int outer = 0; outer < 12; ++outer) {
+        bool done = true;
+        for (int32_t j = 0; j < nb; ++j) done = done && rn[j] <= thr[j];
+        if (done) break;
+        // column scale of r: max component (kept in cs2[j].safe)
+        std::vector<ColState> sc(nb);
+        for (int32_t j = 0; j < nb; ++j) {
+            sc[j] = ColState{};
+            sc[j].safe = rn[j] > 0 ? rn[j] / std::sqrt(double(n)) : 1.0;
+        }
+        HIPCHECK(hipMemcpyAsync(cs2, sc.data(), sizeof(ColState) * nb, hipMemcpyHostToDevice, c->stream));
+        const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
+        hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream, rr,
+                           int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
+        HIPCHECK(hipGetLastError());
+        Solver S = make_solver(m, KMSD_C64, nb);
+        const Outcome o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter);
+        for (int32_t j = 0; j < nb; ++j) {
+            agg[j].iters += o.cs[j].iters;
+            agg[j].breakdown |= o.cs[j].breakdown;
+            agg[j].fin &= o.cs[j].fin;
+        }
+        account(m, o, 1);
+        --m->calls;
+        hipLaunchKernelGGL((scaled_convert_kernel<float, double>), dim3(unsigned(g)), dim3(256), 0, c->stream, d64,
+                           int64_t(nb), yy, int64_t(nb), n, nb, cs2, 1);
+        HIPCHECK(hipGetLastError());
+        // r = b - S y in FP64
+        Step s;
+        s.A = m->S;
+        s.X = yy;
+        s.ldx = nb;
+        s.P = bb;
+        s.ldp = nb;
+        s.gamma = 1.0;
+        s.use_p = true;
+        s.alpha = Cplx{-1, 0};
+        s.OUT = rr;
+        s.ldo = nb;
+        s.n = n;
+        s.w = nb;
+        launch_step(c, KMSD_C128, s);
+        rn = colnorm(rr);
+        ++checks;
+        bool brk = false;
+        for (int32_t j = 0; j < nb; ++j) brk = brk || agg[j].breakdown;
+        if (brk) break;
+    }
+    ++m->calls;
+    launch_convert(c, KMSD_C128, yy, nb, KMSD_C64, Y, ldy, n, nb);
+    // finiteness of the stored solution per column
+    bool fin_all = !launch_nonfinite(c, KMSD_C64, Y, ldy, n, nb);
+    for (auto &s : agg) s.fin = s.fin && fin_all;
+    std::vector<double> qn = bn;
+    store_report(m, agg, &rn, &qn, &thr);
+    m->residual_checks += 0;
+    bool ok = true, brk = false;
+    for (size_t j = 0; j < m->rep_conv.size(); ++j) {
+        ok = ok && m->rep_conv[j];
+        brk = brk || m->rep_brk[j];
+    }
+    (void)checks;
+    if (!ok)
+        throw Error(KMSD_ENUMERIC, !fin_all ? "S solve solution is nonfinite or not representable after rescaling"
+                                   : brk    ? "PCG non-positive/complex curvature breakdown"
+                                            : "S solve failed true-residual acceptance");
+}
+
+void pcg_release(kmsd_metric *m) {
+    delete m->work;
+    m->work = nullptr;
+}
+
 }  // namespace kmsd

@@ -1,0 +1,492 @@
+// Device kernels of the batched PCG metric solve.  One independent system per
+// column; every per-column scalar (scale, threshold, alpha, beta, <r,z>,
+// activity, breakdown, iteration count) lives in device memory, so the
+// iteration needs no host round trip; the host only polls a small flag block
+// every few iterations.  Vector passes use the same row/lane mapping as the
+// fused step kernel; column reductions go through [groups][w] partials that
+// the per-column kernels reduce in a fixed order (deterministic).
+#pragma once
+#include "kmsd_kernels.hpp"
+
+namespace kmsd {
+
+struct ColState {
+    double safe, thr, qn, rz, alpha, beta, rn, tn, gap;
+    int32_t active, restart, breakdown, fin;
+    int64_t iters;
+};
+
+struct PcgFlags {
+    int32_t any_active, broke, need_refresh, nonfinite;
+    int64_t refreshes, restarts, checks;
+};
+
+enum : int32_t {
+    PS_ABSMAX = 0,   // partial: max(|re|,|im|) of B          (part re)
+    PS_SCALE = 1,    // Q = B/safe, R = Q, Y = 0 ; partial |Q|^2
+    PS_PINIT = 2,    // P = active ? Z : 0
+    PS_UPDATE = 3,   // Y += alpha P, R -= alpha SP ; partial |R|^2
+    PS_RESTART = 4,  // R = restart ? T : R
+    PS_DIAG = 5,     // Z = R / d ; partial Re<R,Z>
+    PS_DOT = 6,      // partial Re<R,Z> (block preconditioner applied separately)
+    PS_PUPDATE = 7,  // P = active ? Z + (restart ? 0 : beta P) : 0
+    PS_TRUEGAP = 8,  // partial |T|^2 (re) and |T-R|^2 (im)
+    PS_RESCALE = 9,  // OUT = Y * safe ; flags nonfinite columns
+};
+
+struct PcgArgs {
+    const void *B;  // input RHS (ldb)
+    void *Q, *Y, *R, *Z, *P;
+    const void *SP, *T;
+    void *OUT;
+    const double *diag;
+    ColState *cs;
+    PcgFlags *fl;
+    double *part;
+    int64_t n, ldb, ldo;  // internal blocks have ld = w
+    int32_t w, lpr_log2, stage, require_active;
+    const int32_t *gate;  // if set and *gate == 0 the launch does nothing
+};
+
+template <class VT, int CPL>
+__global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
+    using V = cx<VT>;
+    using PK = Pack<VT, CPL>;
+    // In-loop stages stop after a curvature breakdown (the reference leaves
+    // the loop); set-up and final-acceptance stages always run.
+    if (a.require_active && (a.fl->broke || !a.fl->any_active)) return;
+    if (a.gate && *a.gate == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lpr = 1 << a.lpr_log2;
+    const int rpw = 64 >> a.lpr_log2, rpb = 4 * rpw;
+    const int rloc = wave * rpw + (lane >> a.lpr_log2);
+    const int cgrp = lane & (lpr - 1);
+    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+    const int64_t w = a.w;
+
+    double acc_re[CPL], acc_im[CPL];
+    double cf[CPL], cf2[CPL];
+    int32_t act[CPL], rst[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        acc_re[c] = acc_im[c] = 0.0;
+        cf[c] = cf2[c] = 0.0;
+        act[c] = rst[c] = 0;
+        if (colok) {
+            const ColState &s = a.cs[col + c];
+            act[c] = s.active;
+            rst[c] = s.restart;
+            cf[c] = (a.stage == PS_SCALE) ? 1.0 / s.safe
+                    : (a.stage == PS_UPDATE) ? s.alpha
+                    : (a.stage == PS_PUPDATE) ? s.beta
+                    : (a.stage == PS_RESCALE) ? s.safe
+                                              : 0.0;
+        }
+    }
+    const int64_t nblk = (a.n + rpb - 1) / rpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+    V *Q = static_cast<V *>(a.Q);
+    V *Y = static_cast<V *>(a.Y);
+    V *R = static_cast<V *>(a.R);
+    V *Z = static_cast<V *>(a.Z);
+    V *P = static_cast<V *>(a.P);
+    const V *SP = static_cast<const V *>(a.SP);
+    const V *T = static_cast<const V *>(a.T);
+    const V *B = static_cast<const V *>(a.B);
+    V *OUT = static_cast<V *>(a.OUT);
+    bool bad[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) bad[c] = false;
+
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t row = blk * rpb + rloc;
+        if (row >= a.n || !colok) continue;
+        const int64_t o = row * w + col;
+        switch (a.stage) {
+            case PS_ABSMAX: {
+                PK b;
+                b.load(B + row * a.ldb + col);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> v = b.get(c);
+                    acc_re[c] = fmax(acc_re[c], fmax(fabs(v.re), fabs(v.im)));
+                }
+                break;
+            }
+            case PS_SCALE: {
+                PK b, q;
+                b.load(B + row * a.ldb + col);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<double> v = b.get(c);
+                    v.re *= cf[c];
+                    v.im *= cf[c];
+                    q.set(c, v);
+                    const cx<double> qv = q.get(c);
+                    acc_re[c] += qv.re * qv.re + qv.im * qv.im;
+                }
+                q.store(Q + o);
+                q.store(R + o);
+                PK zero;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) zero.set(c, cx<double>{0.0, 0.0});
+                zero.store(Y + o);
+                break;
+            }
+            case PS_PINIT: {
+                PK z;
+                z.load(Z + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (!act[c]) z.set(c, cx<double>{0.0, 0.0});
+                z.store(P + o);
+                break;
+            }
+            case PS_UPDATE: {
+                PK p, sp, y, r;
+                p.load(P + o);
+                sp.load(SP + o);
+                y.load(Y + o);
+                r.load(R + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> pv = p.get(c), sv = sp.get(c);
+                    cx<double> yv = y.get(c), rv = r.get(c);
+                    yv.re += pv.re * cf[c];
+                    yv.im += pv.im * cf[c];
+                    rv.re -= sv.re * cf[c];
+                    rv.im -= sv.im * cf[c];
+                    y.set(c, yv);
+                    r.set(c, rv);
+                    const cx<double> rs = r.get(c);
+                    acc_re[c] += rs.re * rs.re + rs.im * rs.im;
+                }
+                y.store(Y + o);
+                r.store(R + o);
+                break;
+            }
+            case PS_RESTART: {
+                bool any = false;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) any = any || rst[c];
+                if (!any) break;
+                PK t, r;
+                t.load(T + o);
+                r.load(R + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (rst[c]) r.set(c, t.get(c));
+                r.store(R + o);
+                break;
+            }
+            case PS_DIAG: {
+                PK r, z;
+                r.load(R + o);
+                const double d = a.diag[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> rv = r.get(c);
+                    z.set(c, cx<double>{rv.re / d, rv.im / d});
+                    const cx<double> zv = z.get(c);
+                    acc_re[c] += rv.re * zv.re + rv.im * zv.im;
+                }
+                z.store(Z + o);
+                break;
+            }
+            case PS_DOT: {
+                PK r, z;
+                r.load(R + o);
+                z.load(Z + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> rv = r.get(c), zv = z.get(c);
+                    acc_re[c] += rv.re * zv.re + rv.im * zv.im;
+                }
+                break;
+            }
+            case PS_PUPDATE: {
+                PK z, p;
+                z.load(Z + o);
+                p.load(P + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<double> v{0.0, 0.0};
+                    if (act[c]) {
+                        v = z.get(c);
+                        if (!rst[c]) {
+                            const cx<double> pv = p.get(c);
+                            v.re += pv.re * cf[c];
+                            v.im += pv.im * cf[c];
+                        }
+                    }
+                    p.set(c, v);
+                }
+                p.store(P + o);
+                break;
+            }
+            case PS_TRUEGAP: {
+                PK t, r;
+                t.load(T + o);
+                r.load(R + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> tv = t.get(c), rv = r.get(c);
+                    acc_re[c] += tv.re * tv.re + tv.im * tv.im;
+                    const double dr = tv.re - rv.re, di = tv.im - rv.im;
+                    acc_im[c] += dr * dr + di * di;
+                }
+                break;
+            }
+            case PS_RESCALE: {
+                PK y, out;
+                y.load(Y + o);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> yv = y.get(c);
+                    const bool fin0 = __builtin_isfinite(yv.re) && __builtin_isfinite(yv.im);
+                    cx<double> v{yv.re * cf[c], yv.im * cf[c]};
+                    out.set(c, v);
+                    const cx<double> ov = out.get(c);
+                    const bool fin1 = __builtin_isfinite(ov.re) && __builtin_isfinite(ov.im);
+                    bad[c] = bad[c] || !fin0 || !fin1;
+                }
+                out.store(OUT + row * a.ldo + col);
+                break;
+            }
+        }
+    }
+
+    const bool reduces = a.stage == PS_ABSMAX || a.stage == PS_SCALE || a.stage == PS_UPDATE ||
+                         a.stage == PS_DIAG || a.stage == PS_DOT || a.stage == PS_TRUEGAP;
+    if (a.stage == PS_RESCALE) {
+        // nonfinite columns: flag per column (vector atomics on global memory)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+            if (colok && bad[c]) atomicAnd(&a.cs[col + c].fin, 0);
+        return;
+    }
+    if (!reduces) return;
+    const bool is_max = a.stage == PS_ABSMAX;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        for (int off = lpr; off < 64; off <<= 1) {
+            const double ore = __shfl_xor(acc_re[c], off), oim = __shfl_xor(acc_im[c], off);
+            acc_re[c] = is_max ? fmax(acc_re[c], ore) : acc_re[c] + ore;
+            acc_im[c] += oim;
+        }
+    }
+    __shared__ cx<double> red[4][64 * CPL];
+    if (lane < lpr) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = cx<double>{acc_re[c], acc_im[c]};
+    }
+    __syncthreads();
+    if (wave == 0 && lane < lpr && colok) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (col + c >= a.w) continue;
+            cx<double> s = red[0][lane * CPL + c];
+            for (int q = 1; q < 4; ++q) {
+                const cx<double> v = red[q][lane * CPL + c];
+                s.re = is_max ? fmax(s.re, v.re) : s.re + v.re;
+                s.im += v.im;
+            }
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
+        }
+    }
+}
+
+// Per-column logic stages (one workgroup per column reduces that column's
+// partials in a fixed tree order, then thread 0 applies the PCG rules of
+// the reference's Metric::solve, cpp/native.cpp:313-452).
+enum : int32_t {
+    PC_SAFE = 0,    // safe = absmax (0 -> 1)
+    PC_INIT = 1,    // qn, thr, active ; (rz from a later PC_RZ0)
+    PC_RZ0 = 2,     // rz = Re<r,z>
+    PC_ALPHA = 3,   // curvature -> breakdown / alpha
+    PC_RN = 4,      // rn, iterations, candidate, fixed-policy activity
+    PC_TRUE = 5,    // tn (+gap) -> restart / activity
+    PC_BETA = 6,    // beta, rz, activity count
+    PC_FINAL = 7,   // final true residual -> report fields
+    PC_CLEAR = 8,   // zero flag block + restart marks
+};
+
+struct ColArgs {
+    const double *part;  // [groups][w] complex
+    int64_t groups;
+    ColState *cs;
+    PcgFlags *fl;
+    double rtol, atol, gap_ratio;
+    int32_t w, stage, adaptive, refresh;
+    const int32_t *gate;  // if set and *gate == 0 the launch does nothing
+};
+
+static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
+    const int j = blockIdx.x;
+    if (j >= a.w) return;
+    if (a.gate && *a.gate == 0) return;
+    if (a.stage != PC_SAFE && a.stage != PC_FINAL && a.stage != PC_INIT && a.stage != PC_RZ0 && a.fl->broke) return;
+    __shared__ double sre[256], sim[256];
+    const bool is_max = a.stage == PC_SAFE;
+    double re = 0.0, im = 0.0;
+    if (a.part) {
+        const cx<double> *p = reinterpret_cast<const cx<double> *>(a.part);
+        for (int64_t g = threadIdx.x; g < a.groups; g += 256) {
+            const cx<double> v = p[g * a.w + j];
+            re = is_max ? fmax(re, v.re) : re + v.re;
+            im += v.im;
+        }
+    }
+    sre[threadIdx.x] = re;
+    sim[threadIdx.x] = im;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sre[threadIdx.x] = is_max ? fmax(sre[threadIdx.x], sre[threadIdx.x + off])
+                                      : sre[threadIdx.x] + sre[threadIdx.x + off];
+            sim[threadIdx.x] += sim[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double tre = sre[0], tim = sim[0];
+    ColState &s = a.cs[j];
+    PcgFlags *f = a.fl;
+    switch (a.stage) {
+        case PC_SAFE:
+            s.safe = tre == 0.0 ? 1.0 : tre;
+            s.iters = 0;
+            s.breakdown = 0;
+            s.restart = 0;
+            s.fin = 1;
+            break;
+        case PC_INIT: {
+            s.qn = sqrt(tre);
+            s.thr = fmax(a.rtol * s.qn, a.atol / s.safe);
+            s.active = s.qn > s.thr;
+            if (s.active) atomicAdd(&f->any_active, 1);
+            break;
+        }
+        case PC_RZ0:
+            s.rz = tre;
+            break;
+        case PC_ALPHA: {
+            // curvature = <p, S p>: re = tre, im = tim
+            if (s.active && (!(tre > 0) || !__builtin_isfinite(tre) || !__builtin_isfinite(tim) ||
+                             fabs(tim) > 1e-10 * fmax(fabs(tre), 2.2250738585072014e-308))) {
+                s.breakdown = 1;
+                atomicOr(&f->broke, 1);
+            }
+            s.alpha = s.active ? s.rz / tre : 0.0;
+            break;
+        }
+        case PC_RN: {
+            s.rn = sqrt(tre);
+            s.iters += s.active;
+            s.restart = 0;
+            if (s.active && s.rn <= s.thr) atomicOr(&f->need_refresh, 1);
+            if (!a.adaptive && !a.refresh) s.active = s.active && s.rn > s.thr;
+            break;
+        }
+        case PC_TRUE: {
+            if (j == 0 && f->any_active > 0) {
+                atomicAdd(reinterpret_cast<unsigned long long *>(&f->refreshes), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&f->checks), 1ull);
+            }
+            s.tn = sqrt(tre);
+            s.gap = sqrt(tim);
+            if (s.active) {
+                if (a.adaptive) {
+                    s.restart = s.tn > s.thr && (s.gap > a.gap_ratio * fmax(s.tn, s.thr) || s.rn <= s.thr);
+                } else {
+                    s.restart = 1;
+                }
+                s.active = s.tn > s.thr;
+                if (s.restart) atomicAdd(reinterpret_cast<unsigned long long *>(&f->restarts), 1ull);
+            }
+            break;
+        }
+        case PC_BETA: {
+            s.beta = s.active ? tre / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
+            s.rz = tre;
+            if (s.active) atomicAdd(&f->any_active, 1);
+            break;
+        }
+        case PC_FINAL: {
+            s.tn = sqrt(tre);
+            break;
+        }
+        case PC_CLEAR:
+            break;
+    }
+}
+
+// y128 += scale_j * d64 (refinement update), and r64 = r128 / scale_j.
+template <class VA, class VB>
+__global__ __launch_bounds__(256) void scaled_convert_kernel(const void *Xv, int64_t ldx, void *Ov, int64_t ldo,
+                                                             int64_t n, int32_t w, const ColState *cs, int32_t mode) {
+    const cx<VA> *X = static_cast<const cx<VA> *>(Xv);
+    cx<VB> *O = static_cast<cx<VB> *>(Ov);
+    const int64_t total = n * w;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t r = e / w, c = e - r * w;
+        const cx<double> x = to<double>(X[r * ldx + c]);
+        if (mode == 0) {  // O = X / safe
+            const double inv = 1.0 / cs[c].safe;
+            O[r * ldo + c] = to<VB>(cx<double>{x.re * inv, x.im * inv});
+        } else {  // O += X * safe
+            cx<double> o = to<double>(O[r * ldo + c]);
+            o.re += x.re * cs[c].safe;
+            o.im += x.im * cs[c].safe;
+            O[r * ldo + c] = to<VB>(o);
+        }
+    }
+}
+
+// Atom-block Jacobi: z_blk = L^-H L^-1 r_blk for contiguous blocks, one
+// thread per (block, column).  Factors are complex128 row-major.
+template <class VT, int MAXB>
+__global__ __launch_bounds__(256) void block_precond_kernel(const void *Rv, void *Zv, int64_t w, const int32_t *bptr,
+                                                            const int64_t *foff, const double *fac, int64_t nblocks,
+                                                            const PcgFlags *fl) {
+    if (fl && fl->broke) return;
+    const cx<VT> *R = static_cast<const cx<VT> *>(Rv);
+    cx<VT> *Z = static_cast<cx<VT> *>(Zv);
+    const cx<double> *L = reinterpret_cast<const cx<double> *>(fac);
+    const int64_t total = nblocks * w;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t q = e / w, j = e - q * w;
+        const int32_t a0 = bptr[q], m = bptr[q + 1] - a0;
+        const cx<double> *Lq = L + foff[q];
+        cx<double> v[MAXB];
+        for (int i = 0; i < m; ++i) {
+            cx<double> s = to<double>(R[int64_t(a0 + i) * w + j]);
+            for (int k = 0; k < i; ++k) {
+                const cx<double> l = Lq[i * m + k];
+                s.re -= l.re * v[k].re - l.im * v[k].im;
+                s.im -= l.re * v[k].im + l.im * v[k].re;
+            }
+            const cx<double> d = Lq[i * m + i];
+            const double den = d.re * d.re + d.im * d.im;
+            v[i] = {(s.re * d.re + s.im * d.im) / den, (s.im * d.re - s.re * d.im) / den};
+        }
+        for (int i = m - 1; i >= 0; --i) {
+            cx<double> s = v[i];
+            for (int k = i + 1; k < m; ++k) {
+                const cx<double> l = Lq[k * m + i];  // conj(L[k][i])
+                s.re -= l.re * v[k].re + l.im * v[k].im;
+                s.im -= l.re * v[k].im - l.im * v[k].re;
+            }
+            const cx<double> d = Lq[i * m + i];  // divide by conj(d)
+            const double den = d.re * d.re + d.im * d.im;
+            v[i] = {(s.re * d.re - s.im * d.im) / den, (s.im * d.re + s.re * d.im) / den};
+        }
+        for (int i = 0; i < m; ++i) Z[int64_t(a0 + i) * w + j] = to<VT>(v[i]);
+    }
+}
+
+}  // namespace kmsd

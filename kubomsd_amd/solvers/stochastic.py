@@ -1,0 +1,178 @@
+"""Sample-batch orchestration and statistics for the device solvers
+(reference src/kubo_msd/solvers/stochastic.py:21-228; device residency and
+batch admission as in the GPU edition's solvers/stochastic_gpu.py:18-145).
+
+Samples are taken in the global order of statistics blocks and batches, so
+results do not depend on the batch width; DOS and numerators are summed per
+block on the host and normalized by the paired block bootstrap.  A batch whose
+device workspace does not fit the HBM budget is halved until it does
+(never below one sample, else RESOURCE_BLOCKED).
+"""
+from dataclasses import replace
+import time
+
+import numpy as np
+
+from ..contracts import Reason, TransportBlock
+from ..errors import NumericalError, ResourceBlocked
+from ..kernels.energy import dos_floor
+from ..metric.bounds import estimate_spectral_bounds
+from ..metric.sampling import sample_metric
+from ..observables.diffusion import msd_gradient
+from ..observables.statistics import estimate_ratio
+from .spectral import spectral_samples, spectral_workspace_bytes
+from .stepwise import stepwise_samples
+
+
+def _quantity_keys(cfg, method):
+    keys = ["msd_A2/commutator"] if method == "stepwise" else []
+    if method == "spectral" or cfg.include_vac:
+        keys += ["msd_A2/vac_integral", "diffusion_A2_per_fs/vac_integral"]
+        if cfg.include_vac:
+            keys += ["vac_A2_per_fs2/vac_integral"]
+    if len(cfg.time_fs) > 1:
+        keys += ["diffusion_A2_per_fs/msd_gradient"]
+    return keys
+
+
+def batch_workspace_bytes(ops, cfg, method, nb):
+    """Device bytes a batch of nb samples needs beyond the resident operators."""
+    es = 16 if cfg.precision == "fp64" else 8
+    na = len(cfg.axes)
+    w = nb * (1 + na)
+    # total + series out + 2 recursion buffers + projection block + chain buffers + PCG work (7 blocks)
+    vec = ops.n * es * (4 * w + 3 * na * nb + (0 if ops.metric.identity else 7 * w))
+    if method == "spectral" or cfg.include_vac:
+        vec = max(vec, spectral_workspace_bytes(ops, cfg, nb))
+    return int(vec)
+
+
+def admitted_batch(ops, cfg, method):
+    mem = ops.ctx.memory()
+    free = mem["budget_bytes"] - mem["used_bytes"]
+    nb = cfg.sample_batch
+    while nb >= 1:
+        if batch_workspace_bytes(ops, cfg, method, nb) <= free:
+            return nb
+        nb //= 2
+    raise ResourceBlocked("RESOURCE_BLOCKED: one sample's device workspace exceeds the HBM budget")
+
+
+def _host(x):
+    return x.detach().cpu().numpy() if hasattr(x, "detach") else np.asarray(x)
+
+
+def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch):
+    diagnostics = []
+    primary = "commutator" if method == "stepwise" else "vac_integral"
+    ns = cfg.nsamples if samples is None else samples.shape[1]
+    nbk = dos_blocks.shape[0]
+    for block in range(nbk):
+        first_id = block * cfg.statistics_block
+        end_id = min(ns, first_id + cfg.statistics_block)
+        for first in range(first_id, end_id, batch):
+            last = min(first + batch, end_id)
+            ids = np.arange(first, last)
+            B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else ops.asarray(samples[:, first:last])
+            if method == "stepwise":
+                dos, nums, diag = stepwise_samples(ops, cfg, B)
+                if cfg.include_vac:
+                    _, extra, extra_diag = spectral_samples(ops, cfg, B)
+                    nums.update(extra)
+                    diag["additional_vac_path"] = extra_diag
+                    diag["additional_cost_recorded"] = True
+            else:
+                dos, nums, diag = spectral_samples(ops, cfg, B)
+            nums = {k: _host(v) for k, v in nums.items()}
+            if len(cfg.time_fs) > 1:
+                nums["diffusion_A2_per_fs/msd_gradient"] = msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
+                                                                        axis=2)
+            diagnostics.append(diag)
+            dh = _host(dos)
+            if not np.isfinite(dh).all():
+                raise NumericalError("nonfinite stochastic DOS")
+            for local in range(len(ids)):
+                dos_blocks[block] += dh[local]
+            for key, vh in nums.items():
+                if not np.isfinite(vh).all():
+                    raise NumericalError("nonfinite stochastic numerator")
+                for local in range(len(ids)):
+                    accum[key][block] += vh[local]
+    return diagnostics
+
+
+def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics, batch):
+    nbk = len(counts)
+    primary = "commutator" if method == "stepwise" else "vac_integral"
+    rho, stderr_dos, stat = estimate_ratio(dos_blocks, counts[:, None], counts)
+    threshold = dos_floor(ops.n, cfg.spectral_bounds[1] - cfg.spectral_bounds[0], cfg)
+    valid = np.isfinite(rho) & (rho > threshold)
+    reason = np.where(valid, 0, int(Reason.LOW_DOS)).astype("uint32")
+    if stderr_dos is not None:
+        uncertain = ~np.isfinite(stderr_dos) | (rho - 1.96 * stderr_dos <= 0)
+        valid &= ~uncertain
+        reason[uncertain] |= int(Reason.UNCERTAIN_TRACE)
+    if nbk < 8:
+        reason |= int(Reason.INSUFFICIENT_BLOCKS)
+    static = {"dos_states_per_eV": rho, "dos_block_sums": dos_blocks, "block_counts": counts,
+              "sample_count": np.array(ns), "sample_ids": np.arange(ns)}
+    if stderr_dos is not None:
+        static["stderr_dos_states_per_eV"] = stderr_dos
+    metadata = {
+        "primary_estimator": primary,
+        "primary_diffusion_estimator": "msd_gradient" if method == "stepwise" else "vac_integral",
+        "resolved_config": cfg.as_dict(), "spectral_report": report, "dos_floor": threshold,
+        "statistical_method": stat,
+        "sample_generator": "NumPy PCG64 SeedSequence([seed,sample_id]); unit modulus complex phases",
+        "sampling": getattr(getattr(ops, "sampler", None), "report", {"source": "caller supplied samples"}),
+        "sample_batch_diagnostics": diagnostics, "admitted_sample_batch": batch,
+        "raw_accumulation": "numerator_* are means over nsamples; *_block_sums are sums",
+        "backend": {"name": "hip", "precision": cfg.precision, "memory": ops.ctx.memory(),
+                    "metric": ops.metric.diagnostics(), "h_action_calls": ops.h_calls},
+    }
+    return valid, reason, static, metadata
+
+
+def _normalize_slab(raw, dos_blocks, counts, ns, valid, key):
+    value, se, _ = estimate_ratio(raw, dos_blocks, counts)
+    value[..., ~valid] = np.nan
+    arrays = {key: value, "numerator_" + key: raw.sum(0) / ns, "numerator_block_sums/" + key: raw.copy()}
+    if se is not None:
+        se[..., ~valid] = np.nan
+        arrays["stderr_" + key] = se
+    return arrays
+
+
+def iter_stochastic_blocks(ops, config, method, samples=None):
+    cfg = config
+    report = estimate_spectral_bounds(ops.host_H, ops.host_S, ops.metric.s_lower,
+                                      ops.metric.bounds_report["certified"], cfg.spectral_bounds)
+    cfg = replace(cfg, spectral_bounds=report["bounds"])
+    ns = cfg.nsamples if samples is None else samples.shape[1]
+    nbk = (ns + cfg.statistics_block - 1) // cfg.statistics_block
+    counts = np.array([min(cfg.statistics_block, ns - i * cfg.statistics_block) for i in range(nbk)])
+    keys = _quantity_keys(cfg, method)
+    shape = (nbk, len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV))
+    accum = {key: np.zeros(shape) for key in keys}
+    dos_blocks = np.zeros((nbk, len(cfg.energy_eV)))
+    batch = admitted_batch(ops, cfg, method)
+    t0 = time.perf_counter()
+    diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch)
+    seconds = time.perf_counter() - t0
+    valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
+                                                     diagnostics, batch)
+    metadata["sample_seconds"] = seconds
+    for start in range(0, len(cfg.time_fs), cfg.time_block):
+        stop = min(start + cfg.time_block, len(cfg.time_fs))
+        arrays = {}
+        for key in keys:
+            arrays.update(_normalize_slab(accum[key][:, :, start:stop, :], dos_blocks, counts, ns, valid, key))
+        for estimator in {k.split("/")[-1] for k in keys}:
+            r = reason.copy()
+            if estimator == "vac_integral":
+                r |= int(Reason.INTEGRATION_UNVERIFIED)
+            dims = (len(cfg.axes), stop - start, len(cfg.energy_eV))
+            arrays["valid/" + estimator] = np.broadcast_to(valid, dims).copy()
+            arrays["reason_code/" + estimator] = np.broadcast_to(r, dims).copy()
+        yield TransportBlock(start, cfg.time_fs[start:stop], arrays, static if start == 0 else {},
+                             metadata if start == 0 else {})

@@ -1,0 +1,177 @@
+"""Device batched PCG against dense solves, the reference's known answers
+(tests/unit/test_native.py:57-93, 218-255; tests/unit/test_metric_adaptive.py)
+and the FP64 oracle; nonorthogonal engine actions against the oracle."""
+import numpy as np
+import pytest
+import scipy.linalg as la
+import scipy.sparse as sp
+
+from helpers import dense_model, hip_engine, host, oracle_engine, random_block, rel_err, to_dev
+from oracle import native as O
+
+pytestmark = pytest.mark.gpu
+
+
+def metric(ctx, S, blocks=()):
+    from kubomsd_amd.backends.hip import HipMetric
+    S = sp.csr_matrix(S, dtype=np.complex128)
+    return HipMetric(ctx, S, S.diagonal().real, blocks)
+
+
+def block_factors(S, slices):
+    S = np.asarray(S.todense() if sp.issparse(S) else S)
+    return [(a, b, la.cholesky(S[a:b, a:b], lower=True)) for a, b in slices]
+
+
+@pytest.mark.parametrize("pre", ["diagonal", "block-jacobi"])
+def test_pcg_scaled_rhs_and_true_residual(hip_ctx, pre):
+    rng = np.random.default_rng(97)
+    A = rng.normal(size=(11, 11)) + 1j * rng.normal(size=(11, 11))
+    S = A.conj().T @ A + 2 * np.eye(11)
+    B = rng.normal(size=(11, 4)) + 1j * rng.normal(size=(11, 4))
+    B *= [0, 1e-180, 1, 1e180]
+    blocks = block_factors(S, [(0, 3), (3, 7), (7, 11)]) if pre == "block-jacobi" else ()
+    m = metric(hip_ctx, S, blocks)
+    out, rep = m.solve(to_dev(hip_ctx, B), 2e-13, 0.0, 1000)
+    out = host(out)
+    for j in (1, 2, 3):
+        scale = np.max(np.abs(B[:, j]))
+        np.testing.assert_allclose(out[:, j] / scale, la.solve(S, B[:, j] / scale), rtol=3e-11, atol=2e-12)
+        res = la.norm(B[:, j] / scale - S @ (out[:, j] / scale))
+        assert res / la.norm(B[:, j] / scale) <= 1.1 * 2e-13
+    np.testing.assert_array_equal(out[:, 0], 0)
+    assert all(rep["converged"])
+    assert max(rep["relative_residual"]) <= 2e-13
+    # iteration counts agree with the FP64 oracle up to rounding at the
+    # threshold (the reference does not pin iteration counts either)
+    om = O.Metric(O.Csr(S), np.real(np.diag(S)), blocks)
+    _, orep = om.solve(B, 2e-13, 0.0, 1000)
+    assert all(abs(a - b) <= 1 for a, b in zip(rep["iterations"], orep["iterations"]))
+
+
+def test_pcg_failure_report_and_atol(hip_ctx):
+    from kubomsd_amd.errors import NumericalError
+    S = np.array([[1, .2j], [-.2j, 2]])
+    m = metric(hip_ctx, S)
+    with pytest.raises(NumericalError) as failed:
+        m.solve(to_dev(hip_ctx, np.array([[0, 1], [0, 1]], complex)), 1e-10, 0.0, 1)
+    d = failed.value.diagnostics
+    assert d["converged"] == [True, False]
+    assert d["iterations"] == [0, 1]
+    assert d["relative_residual"][1] > 1e-10
+    out, rep = metric(hip_ctx, S).solve(to_dev(hip_ctx, np.full((2, 1), 1e-180, complex)), 1e-10, 1e-100, 1000)
+    np.testing.assert_array_equal(host(out), 0)
+    assert all(rep["converged"])
+
+
+def test_pcg_known_answer_one_jacobi_iteration(hip_ctx):
+    # One Jacobi-preconditioned CG step from p = (1, .5): y = (15/17, 15/34).
+    m = metric(hip_ctx, np.array([[1., .2], [.2, 2.]]))
+    out, rep = m.solve(to_dev(hip_ctx, np.ones((2, 1), complex)), 0.5, 0.0, 1)
+    np.testing.assert_allclose(host(out)[:, 0], [15 / 17, 15 / 34], rtol=1e-14, atol=1e-14)
+    assert rep["converged"] == [True] and rep["iterations"] == [1]
+    assert 1e-12 < rep["relative_residual"][0] < .5
+
+
+def test_pcg_rejects_unrepresentable_solution(hip_ctx):
+    from kubomsd_amd.errors import NumericalError
+    m = metric(hip_ctx, .25 * np.eye(2))
+    B = np.array([[1e308, 1.], [1., 1.]], dtype=complex)
+    with pytest.raises(NumericalError, match="finite|representable") as failed:
+        m.solve(to_dev(hip_ctx, B), 1e-10, 0.0, 1000)
+    assert failed.value.diagnostics["converged"] == [False, True]
+
+
+def test_pcg_refresh_and_counters(hip_ctx):
+    n = 96
+    S = sp.diags([-np.ones(n - 1), 2.03 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csr", dtype=complex)
+    B = random_block(n, 3, 88)
+    m = metric(hip_ctx, S)
+    out, rep = m.solve(to_dev(hip_ctx, B), 1e-11, 0.0, 1000)
+    assert m.diagnostics()["true_residual_refreshes"] >= 1
+    np.testing.assert_allclose(S @ host(out), B, rtol=1e-9, atol=1e-9)
+    assert max(rep["relative_residual"]) <= 1e-11
+    om = O.Metric(O.Csr(S), np.full(n, 2.03), ())
+    _, orep = om.solve(B, 1e-11, 0.0, 1000)
+    assert all(abs(a - b) <= 1 for a, b in zip(rep["iterations"], orep["iterations"]))
+    assert abs(m.diagnostics()["true_residual_refreshes"] - om.diagnostics()["true_residual_refreshes"]) <= 1
+
+
+@pytest.mark.parametrize("interval", [4, 10000])
+def test_pcg_adaptive_policy(hip_ctx, interval):
+    rng = np.random.default_rng(4)
+    U, _ = np.linalg.qr(rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32)))
+    S = (U * np.geomspace(1e-4, 1, 32)) @ U.conj().T
+    S = (S + S.conj().T) / 2
+    B = rng.normal(size=(32, 3)) + 1j * rng.normal(size=(32, 3))
+    B[:, 0] = 0
+    m = metric(hip_ctx, S)
+    m.set_residual_policy("adaptive", interval, 0.1)
+    y, rep = m.solve(to_dev(hip_ctx, B), 1e-12, 0.0, 1000)
+    y = host(y)
+    assert all(rep["converged"]) and rep["iterations"][0] == 0
+    np.testing.assert_array_equal(y[:, 0], 0)
+    for j in (1, 2):
+        assert np.linalg.norm(B[:, j] - S @ y[:, j]) / np.linalg.norm(B[:, j]) <= 1e-12
+    om = O.Metric(O.Csr(S), np.real(np.diag(S)), ())
+    om.set_residual_policy("adaptive", interval, 0.1)
+    _, orep = om.solve(B, 1e-12, 0.0, 1000)
+    # rounding differs from the serial oracle; iteration counts agree closely
+    assert all(abs(a - b) <= 2 for a, b in zip(rep["iterations"], orep["iterations"]))
+
+
+def test_pcg_complex64_refinement_reaches_fp64_tolerance(hip_ctx):
+    from kubomsd_amd.synthetic import cubic_orbital_model
+    model = cubic_orbital_model(5, seed=31)
+    S = model.S
+    B = random_block(model.n, 8, 3)
+    blocks = block_factors(S, model.atom_slices)
+    m = metric(hip_ctx, S, blocks)
+    out, rep = m.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 1000)
+    Bc = B.astype(np.complex64).astype(np.complex128)
+    ref = sp.linalg.spsolve(S.tocsc(), Bc)
+    assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+    # the stored solution is rounded once to complex64
+    assert rel_err(host(out), ref) < 2e-7
+
+
+@pytest.mark.parametrize("kind", ["velocity", "position", "composite"])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_nonorthogonal_engine_matches_oracle(hip_ctx, kind, dtype):
+    from kubomsd_amd.backends.hip import HipEngine, HipMetric
+    base = dense_model()
+    H, S, X = base.H.toarray(), base.S.toarray(), base.velocity["x"].toarray()
+    n = H.shape[0]
+    if kind == "velocity":
+        defs = [(0, [X]), (-1, []), (-1, [])]
+    elif kind == "position":
+        defs = [(1, [X]), (-1, []), (-1, [])]
+    else:
+        r = np.linspace(0, 2, n)
+        defs = [(2, [H * (r[None, :] - r[:, None]), X - r[:, None] * S, X - S * r[None, :]]), (-1, []), (-1, [])]
+    rt = 2e-13
+    oe = O.Engine(O.Csr(H), O.Metric(O.Csr(S), np.real(np.diag(S)), ()),
+                  [(k, [O.Csr(mm) for mm in ms]) for k, ms in defs], rt, 0.0, 1000)
+    he = HipEngine(hip_ctx, sp.csr_matrix(H), HipMetric(hip_ctx, sp.csr_matrix(S), np.real(np.diag(S))),
+                   [(k, [sp.csr_matrix(mm) for mm in ms]) for k, ms in defs], rt, 0.0, 1000)
+    rng = np.random.default_rng(18)
+    B = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    tol = 1e-9 if dtype == "c128" else 2e-5
+    assert rel_err(host(he.h_action(to_dev(hip_ctx, B, dtype))), oe.h_action(B)) < tol
+    assert rel_err(host(he.v_cov(to_dev(hip_ctx, B, dtype), [1.0, 0, 0])), oe.v_cov(B, [1.0, 0, 0])) < tol
+    assert rel_err(host(he.chain(to_dev(hip_ctx, B, dtype), 12, 0.0, 2.0)), oe.chain(B, 12, 0.0, 2.0)) < tol * 10
+    assert rel_err(host(he.moments(to_dev(hip_ctx, B, dtype), 20, 0.0, 2.0)), oe.moments(B, 20, 0.0, 2.0)) < tol * 10
+    coeff = np.array([.9, .3j, -.05, .01j, .002])
+    Ba = np.hstack([B, 0.1 * B])
+    got = host(he.series(to_dev(hip_ctx, Ba, dtype), coeff, 0.0, 2.0, [[1.0, 0, 0]]))
+    assert rel_err(got, oe.series(Ba, coeff, 0.0, 2.0, [[1.0, 0, 0]])) < tol * 10
+
+
+def test_fused_failure_keeps_per_column_diagnostics(hip_ctx):
+    from kubomsd_amd.errors import NumericalError
+    base = dense_model()
+    he = hip_engine(hip_ctx, base, maxiter=1)
+    with pytest.raises(NumericalError) as failed:
+        he.series(to_dev(hip_ctx, np.ones((base.n, 2), complex)), np.array([1, .5, .1]), 0.0, 2.0)
+    assert failed.value.diagnostics["converged"] == [False, False]
+    assert failed.value.diagnostics["iterations"] == [1, 1]

@@ -1,0 +1,135 @@
+"""Device stepwise / spectral / stochastic orchestration against the FP64
+oracle orchestration (oracle/kpm.py) on identical models and samples.
+
+Tolerances: fp64 runs within 1e-9 of the largest entry (the reference's own
+cpp-vs-reference stepwise bound is 2e-9, tests/unit/test_native.py:139-143);
+mixed (complex64 storage) within 2e-5.
+"""
+import numpy as np
+import pytest
+
+from helpers import dense_model, phases, random_block
+from kubomsd_amd.synthetic import honeycomb_lattice, square_lattice
+from oracle import kpm
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-9, "mixed": 2e-5}
+
+
+def run_config(**kw):
+    from kubomsd_amd.config import RunConfig
+    base = dict(solver="stochastic-kpm", kernel="jackson", backend="hip", metric_rtol=1e-13,
+                include_vac=False, segment_fs=.3)
+    base.update(kw)
+    return RunConfig(**base)
+
+
+def bundle(ctx, H, S, vel, cfg, atom_slices=()):
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.operators.velocity import build_operators
+    return build_operators(ModelBundle(H, S, velocity=vel, atom_slices=list(atom_slices)), cfg, ctx)
+
+
+def close(got, want, tol):
+    got = got.detach().cpu().numpy() if hasattr(got, "detach") else np.asarray(got)
+    scale = max(np.abs(want).max(), 1e-300)
+    assert np.max(np.abs(got - want)) <= tol * scale, np.max(np.abs(got - want)) / scale
+
+
+def small_model():
+    m = dense_model(9, 903)
+    H, S, X = m.H.toarray(), m.S.toarray(), m.velocity["x"].toarray()
+    eps = np.linalg.eigvalsh(np.linalg.solve(S, H))
+    return H, S, X, (float(eps.min()) - .3, float(eps.max()) + .3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stepwise_nonorthogonal_matches_oracle(hip_ctx, precision):
+    from kubomsd_amd.solvers.stepwise import stepwise_samples
+    H, S, X, bounds = small_model()
+    E = np.linspace(bounds[0] + .2, bounds[1] - .2, 7)
+    cfg = run_config(energy_eV=E, time_fs=[0, .2, .5], spectral_bounds=bounds, n_moments=40, axes=("x",),
+                     precision=precision)
+    B = random_block(9, 3, 5)
+    dos, nums, diag = stepwise_samples(bundle(hip_ctx, H, S, {"x": X}, cfg), cfg, B)
+    kcfg = kpm.config(energy_eV=E, time_fs=cfg.time_fs, spectral_bounds=bounds, n_moments=40, segment_fs=.3)
+    edos, enums = kpm.stepwise_samples(kpm.Ops(H, S, {"x": X}, rtol=1e-13), kcfg, B)
+    close(dos, edos, TOL[precision])
+    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], TOL[precision] * 10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stepwise_lattice_two_axes_matches_oracle(hip_ctx, precision):
+    from kubomsd_amd.solvers.stepwise import stepwise_samples
+    m = square_lattice(20, seed=7)
+    bounds = (-5.1, 5.1)
+    E = np.linspace(-4, 4, 33)
+    cfg = run_config(energy_eV=E, time_fs=[0, .5, 1.0, 1.5], spectral_bounds=bounds, n_moments=128,
+                     axes=("x", "y"), precision=precision, segment_fs=1.0)
+    B = phases(m.n, 4, 3)
+    dos, nums, _ = stepwise_samples(bundle(hip_ctx, m.H, m.S, m.velocity, cfg), cfg, B)
+    kcfg = kpm.config(energy_eV=E, time_fs=cfg.time_fs, spectral_bounds=bounds, n_moments=128, segment_fs=1.0,
+                      axes_vectors=np.eye(3)[:2])
+    edos, enums = kpm.stepwise_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
+    close(dos, edos, TOL[precision])
+    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], TOL[precision] * 10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spectral_matches_oracle(hip_ctx, precision):
+    from kubomsd_amd.solvers.spectral import spectral_samples
+    m = honeycomb_lattice(8, 6, seed=5)
+    bounds = (-8.8, 8.8)
+    E = np.linspace(-8, 8, 21)
+    cfg = run_config(energy_eV=E, time_fs=[0, .3, .8], spectral_bounds=bounds, n_moments=64, axes=("x",),
+                     precision=precision, include_vac=True, integration_dt_fs=.02, segment_fs=.5)
+    B = phases(m.n, 3, 4)
+    dos, nums, diag = spectral_samples(bundle(hip_ctx, m.H, m.S, m.velocity, cfg), cfg, B)
+    kcfg = kpm.config(energy_eV=E, time_fs=cfg.time_fs, spectral_bounds=bounds, n_moments=64, segment_fs=.5,
+                      integration_dt_fs=.02)
+    edos, enums = kpm.spectral_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
+    close(dos, edos, TOL[precision])
+    for key in enums:
+        close(nums[key], enums[key], TOL[precision] * 10)
+
+
+def test_stochastic_blocks_batch_invariance_and_oracle_sums(hip_ctx):
+    """Global sample order makes results independent of the batch width
+    (tests/unit/test_shared_axes.py:5-16); block sums equal the oracle's."""
+    from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
+    m = square_lattice(12, seed=9)
+    bounds = (-5.1, 5.1)
+    E = np.linspace(-4, 4, 17)
+    B = phases(m.n, 9, 8)
+    out = []
+    for batch in (4, 1):
+        cfg = run_config(energy_eV=E, time_fs=[0, .5, 1.0], spectral_bounds=bounds, n_moments=64, axes=("x", "y"),
+                         nsamples=9, statistics_block=2, sample_batch=batch, segment_fs=1.0)
+        ops = bundle(hip_ctx, m.H, m.S, m.velocity, cfg)
+        out.append(list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=B))[0])
+    a, b = out
+    np.testing.assert_allclose(a.arrays["msd_A2/commutator"], b.arrays["msd_A2/commutator"], rtol=1e-10,
+                               atol=1e-12)
+    np.testing.assert_allclose(a.static["block_counts"], [2, 2, 2, 2, 1])
+    kcfg = kpm.config(energy_eV=E, time_fs=[0, .5, 1.0], spectral_bounds=bounds, n_moments=64, segment_fs=1.0,
+                      axes_vectors=np.eye(3)[:2])
+    edos, enums = kpm.stepwise_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
+    blocks = [slice(i, min(i + 2, 9)) for i in range(0, 9, 2)]
+    want = np.array([enums["msd_A2/commutator"][s].sum(0) for s in blocks])
+    close(a.arrays["numerator_block_sums/msd_A2/commutator"], want, 1e-9)
+    close(a.static["dos_block_sums"], np.array([edos[s].sum(0) for s in blocks]), 1e-10)
+    assert np.all(np.isfinite(a.arrays["diffusion_A2_per_fs/msd_gradient"][a.arrays["valid/commutator"]]))
+
+
+def test_sampler_covariance_on_device(hip_ctx):
+    """Device Clenshaw sampler: C C^H = S^-1 (tests/unit/test_sampling.py:4-14)."""
+    import scipy.sparse as sp
+    from kubomsd_amd.metric.sampling import MetricSampler
+    S = np.array([[1, .2j], [-.2j, 2]])
+    cfg = run_config(energy_eV=[0.], time_fs=[0.], spectral_bounds=(-1, 1), sampling_tol=1e-12,
+                     axes=("x",))
+    ops = bundle(hip_ctx, np.zeros((2, 2)), S, {"x": np.array([[0, 1j], [-1j, 0]])}, cfg)
+    sampler = MetricSampler(ops.metric, cfg)
+    C = sampler.transform(ops.asarray(np.eye(2, dtype=complex))).cpu().numpy()
+    np.testing.assert_allclose(C @ C.conj().T, np.linalg.inv(S), rtol=1e-11, atol=1e-12)
