@@ -111,6 +111,10 @@ struct Solver {
         a.rtol = rtol;
         a.atol = atol;
         a.gap_ratio = m->gap_ratio;
+        // The reference's 1e-10 curvature test is an FP64 criterion; complex64
+        // inner solves round every stored vector at ~6e-8, so their test
+        // tolerates imaginary parts up to 1e-5 of the real part.
+        a.imag_tol = dtype == KMSD_C128 ? 1e-10 : 1e-5;
         a.w = w;
         a.stage = stage;
         a.adaptive = m->adaptive;

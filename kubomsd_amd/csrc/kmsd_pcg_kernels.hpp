@@ -321,6 +321,7 @@ struct ColArgs {
     ColState *cs;
     PcgFlags *fl;
     double rtol, atol, gap_ratio;
+    double imag_tol;      // breakdown: |Im <p,Sp>| > imag_tol |Re <p,Sp>|
     int32_t w, stage, adaptive, refresh;
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
@@ -377,7 +378,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         case PC_ALPHA: {
             // curvature = <p, S p>: re = tre, im = tim
             if (s.active && (!(tre > 0) || !__builtin_isfinite(tre) || !__builtin_isfinite(tim) ||
-                             fabs(tim) > 1e-10 * fmax(fabs(tre), 2.2250738585072014e-308))) {
+                             fabs(tim) > a.imag_tol * fmax(fabs(tre), 2.2250738585072014e-308))) {
                 s.breakdown = 1;
                 atomicOr(&f->broke, 1);
             }
