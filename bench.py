@@ -198,6 +198,18 @@ def main():
     step_s = (ms_per_step / 1e3) / (M - 1)
     achieved = per_launch / step_s / 1e9
 
+    # Same moments from the Chebyshev doubling identities (about half the
+    # orders, no projection stream): reported beside the direct recursion.
+    e2, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    mu_d = eng.moments(B, M, c, a, doubling=True)
+    e2.record(stream)
+    for _ in range(args.steps):
+        mu_d = eng.moments(B, M, c, a, doubling=True)
+    e3.record(stream)
+    torch.cuda.synchronize()
+    dbl_ms = e2.elapsed_time(e3) / args.steps
+    dbl_orders = (M - 1) // 2 + 1
+
     err = None
     if not args.no_fp64_check and args.precision == "mixed":
         B64 = torch.from_numpy(Bh).to(ctx.torch_device)
@@ -206,8 +218,12 @@ def main():
         from kubomsd_amd.kernels.energy import reconstruction_matrix
         G = reconstruction_matrix(np.linspace(-8.0, 8.0, 161), (lo, hi), M)
         dos64, dos32 = G @ mu64.mean(1), G @ mu32.mean(1)
+        mud = mu_d.cpu().numpy()
+        dosd = G @ mud.mean(1)
         err = dict(mu_max_rel=float(np.max(np.abs(mu32 - mu64)) / np.max(np.abs(mu64))),
-                   dos_max_rel=float(np.max(np.abs(dos32 - dos64)) / np.max(np.abs(dos64))))
+                   dos_max_rel=float(np.max(np.abs(dos32 - dos64)) / np.max(np.abs(dos64))),
+                   doubling_mu_max_rel=float(np.max(np.abs(mud - mu64)) / np.max(np.abs(mu64))),
+                   doubling_dos_max_rel=float(np.max(np.abs(dosd - dos64)) / np.max(np.abs(dos64))))
         del B64
     cpu = None
     if not args.no_cpu_baseline:
@@ -235,6 +251,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": per_launch,
                      "kernel": "step_kernel (fused CSR product + 3-term update + moment partials)"},
+        "moments_per_s": nb * M / (ms_per_step / 1e3),
+        "moment_doubling": {"moments_per_s": nb * M / (dbl_ms / 1e3), "ms_per_chain": dbl_ms,
+                            "orders": dbl_orders,
+                            "achieved_GBps": step_bytes(model.H.nnz, n, nb, s_val, s_vec, d=0) * dbl_orders
+                            / (dbl_ms / 1e3) / 1e9},
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }

@@ -160,6 +160,15 @@ int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, 
 int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
                  double half);
 
+/* No reference counterpart: the same moments as kmsd_moments (S = I only),
+ * from the Chebyshev doubling identities T_2k = 2 T_k^2 - 1 and
+ * T_2k+1 = 2 T_k T_k+1 - T_1: mu_2k = 2<T_k B, T_k B> - mu_0 and
+ * mu_2k+1 = 2 Re<T_k B, T_k+1 B> - mu_1.  About L/2 Chebyshev orders instead of
+ * L - 1, and no projection-block stream.  Exact in arithmetic; rounding
+ * differs from the direct recursion. */
+int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
+                          double half);
+
 /* ---- spectral double moments -------------------------------------------- */
 /* Reference: einsum('jnb,knb->bjk', left.conj(), cov) (src/kubo_msd/solvers/
  * spectral.py:37; GPU edition solvers/spectral.py:65).  left: (m, N, nb),

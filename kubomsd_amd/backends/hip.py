@@ -59,6 +59,7 @@ SIGNATURES = {
     "kmsd_series": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D, _P, _I64],
     "kmsd_chain": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
+    "kmsd_moments_doubling": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_mu_table": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I],
 }
 
@@ -431,13 +432,16 @@ class HipEngine:
                                             int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
         return out
 
-    def moments(self, B, length, center, half):
-        """Device float64 (length, nb): mu[l, j] = Re <S B_j, T_l B_j>."""
+    def moments(self, B, length, center, half, doubling=False):
+        """Device float64 (length, nb): mu[l, j] = Re <S B_j, T_l B_j>.
+        doubling=True (S = I only) uses the Chebyshev product identities: about
+        half the orders and no projection stream."""
         B = self._prep(B)
         torch = _torch()
         mu = torch.empty((int(length), B.shape[1]), dtype=torch.float64, device=B.device)
-        self._check(self.ctx.lib.kmsd_moments(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(mu.data_ptr()),
-                                              int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
+        fn = self.ctx.lib.kmsd_moments_doubling if doubling else self.ctx.lib.kmsd_moments
+        self._check(fn(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(mu.data_ptr()), int(length), B.shape[1],
+                       _dtype_code(B), float(center), float(half)))
         return mu
 
 

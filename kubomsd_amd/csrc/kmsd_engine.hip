@@ -437,3 +437,69 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
 }
+
+extern "C" int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
+                                     double center, double half) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, mu, nb, dtype);
+        if (L < 1) throw Error(KMSD_EINVAL, "moment count must be positive");
+        if (!e->metric->identity) throw Error(KMSD_EINVAL, "moment doubling requires an identity metric");
+        check_half(half);
+        const int64_t n = n_of(e);
+        const int32_t w = int32_t(nb);
+        require_finite(c, dtype, B, nb, n, w);
+        const size_t bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
+        void *t1 = e->t0.ensure(c, bytes);
+        void *spare = e->t1.ensure(c, bytes);
+        const int cap = cpl_cap_for(dtype, {{B, nb}, {t1, nb}, {spare, nb}});
+        const int64_t G = step_groups(c, n, w, dtype, cap);
+        const int64_t steps = (L - 1) / 2 + 1;  // k = 0 .. floor((L-1)/2)
+        const int64_t chunk = std::min<int64_t>(c->moment_chunk, steps);
+        const size_t slot_doubles = size_t(G) * size_t(w) * 2;
+        double *part = static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
+        double *ab = static_cast<double *>(e->wbuf.ensure(c, sizeof(double) * 2 * size_t(steps) * size_t(w)));
+        int64_t slot = 0, base = 0;
+        auto flush = [&] {
+            launch_reduce(c, part, G, w, slot, ab + 2 * base * nb, nb, 1);
+            base += slot;
+            slot = 0;
+        };
+        auto next_part = [&] {
+            if (slot == chunk) flush();
+            return part + size_t(slot++) * slot_doubles;
+        };
+        const void *prev = nullptr;
+        const void *cur = B;
+        for (int64_t k = 0; k < steps; ++k) {
+            void *o = (k == 0) ? t1 : (prev == B ? spare : const_cast<void *>(prev));
+            Step q;
+            q.X = cur;
+            q.ldx = nb;
+            q.OUT = o;
+            q.ldo = nb;
+            q.alpha = Cplx{(k == 0 ? 1.0 : 2.0) / half, 0};
+            q.beta = Cplx{-(k == 0 ? 1.0 : 2.0) * center / half, 0};
+            q.use_beta = true;
+            if (k > 0) {
+                q.P = prev;
+                q.ldp = nb;
+                q.gamma = -1.0;
+                q.use_p = true;
+            }
+            q.doubling = true;
+            q.n = n;
+            q.w = w;
+            q.cpl_cap = cap;
+            q.expect_groups = G;
+            q.part = next_part();
+            cheb(e, dtype, q);
+            prev = cur;
+            cur = o;
+        }
+        flush();
+        launch_doubling_finish(c, ab, mu, L, w);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}

@@ -137,6 +137,7 @@ struct Step {
     int32_t w = 0;
     int cpl_cap = 4;            // columns-per-lane ceiling (keeps the shape fixed across a chain)
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
+    bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
     int64_t expect_groups = 0;  // if nonzero, the launch must use exactly this many workgroups
 };
 // Launches one fused step; returns the number of workgroups used (the
@@ -161,6 +162,7 @@ void spmm(kmsd_ctx *c, int dtype, const kmsd_matrix *A, const void *X, int64_t l
 void require_finite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
 // Device copy of S scaled by congruence with diag^-1/2 (the sampler's W).
 kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const double *diag);
+void launch_doubling_finish(kmsd_ctx *c, const double *ab, double *mu, int64_t L, int32_t w);
 void launch_sample_finish(kmsd_ctx *c, int dtype, void *out, const void *xi, int64_t ld, int64_t n, int32_t w,
                           const double *diag, double c0);
 

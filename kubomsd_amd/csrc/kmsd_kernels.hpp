@@ -63,6 +63,7 @@ enum : uint32_t {
     F_ACC = 1u << 5,    // ACC_i += coef * t
     F_DIAG = 1u << 6,   // the product includes the stored diagonal d_i X_i
     F_ELL = 1u << 7,    // off-diagonal part is ELL (ell entries per row, row-major)
+    F_MOM2 = 1u << 8,   // partial.re += |X_i|^2, partial.im += Re conj(X_i) t (moment doubling)
 };
 
 struct StepArgs {
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     const MT *val = static_cast<const MT *>(a.val);
     const MT *dval = static_cast<const MT *>(a.dval);
     const uint32_t fl = a.flags;
-    const bool need_xi = (fl & F_BETA) || ((fl & F_SPMM) && (fl & F_DIAG));
+    const bool need_xi = (fl & F_BETA) || (fl & F_MOM2) || ((fl & F_SPMM) && (fl & F_DIAG));
 
     cx<double> mom[CPL];
 #pragma unroll
@@ -255,6 +256,14 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
             }
         }
         if (fl & F_STORE) ts.store_nt(OUT + row * a.ldo + col);
+        if (fl & F_MOM2) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const cx<double> xv = xi.get(c), tv = ts.get(c);
+                mom[c].re = __builtin_fma(xv.re, xv.re, __builtin_fma(xv.im, xv.im, mom[c].re));
+                mom[c].im = __builtin_fma(xv.re, tv.re, __builtin_fma(xv.im, tv.im, mom[c].im));
+            }
+        }
         if (fl & F_MOM) {
             // Products of the stored (rounded) values, accumulated in FP64;
             // FP32 x FP32 products are exact in FP64.
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
         }
     }
 
-    if (!(fl & F_MOM)) return;
+    if (!(fl & (F_MOM | F_MOM2))) return;
     // Rows sharing a column: lanes differing in bits >= lpr_log2.
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -420,6 +429,28 @@ __global__ __launch_bounds__(256) void sample_finish_kernel(void *Ov, const void
         const double z = 1.0 / sqrt(d[r]);
         const cx<double> o = to<double>(O[r * ld + c]), x = to<double>(X[r * ld + c]);
         O[r * ld + c] = to<VT>(cx<double>{z * (o.re + c0 * x.re), z * (o.im + c0 * x.im)});
+    }
+}
+
+}  // namespace kmsd
+
+namespace kmsd {
+
+// Moment doubling: ab[k][j] = (<T_k,T_k>, Re<T_k,T_{k+1}>) -> mu[2k] = 2a_k - mu_0,
+// mu[2k+1] = 2b_k - mu_1 with mu_0 = a_0, mu_1 = b_0 (T_{2k} = 2T_k^2 - 1,
+// T_{2k+1} = 2T_kT_{k+1} - T_1).
+static __global__ __launch_bounds__(256) void doubling_finish_kernel(const double *ab, double *mu, int64_t L, int32_t w) {
+    const cx<double> *AB = reinterpret_cast<const cx<double> *>(ab);
+    const int64_t total = L * w;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t l = e / w, j = e - l * w, k = l / 2;
+        const double mu0 = AB[j].re, mu1 = AB[j].im;
+        if (l == 0)
+            mu[e] = mu0;
+        else if (l == 1)
+            mu[e] = mu1;
+        else
+            mu[e] = (l & 1) ? 2.0 * AB[k * w + j].im - mu1 : 2.0 * AB[k * w + j].re - mu0;
     }
 }
 

@@ -280,6 +280,7 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     if (s.use_p) fl |= F_P;
     if (s.OUT) fl |= F_STORE;
     if (s.W) fl |= F_MOM;
+    if (s.doubling) fl |= F_MOM2;
     if (s.ACC) fl |= F_ACC;
     a.X = s.X;
     a.Z = s.Z;
@@ -451,6 +452,12 @@ kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const dou
         throw;
     }
     return W;
+}
+
+void launch_doubling_finish(kmsd_ctx *c, const double *ab, double *mu, int64_t L, int32_t w) {
+    const int64_t g = blocks_for(L * w);
+    hipLaunchKernelGGL(doubling_finish_kernel, dim3(unsigned(g)), dim3(256), 0, c->stream, ab, mu, L, w);
+    HIPCHECK(hipGetLastError());
 }
 
 void launch_sample_finish(kmsd_ctx *c, int dtype, void *out, const void *xi, int64_t ld, int64_t n, int32_t w,

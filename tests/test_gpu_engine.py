@@ -122,3 +122,16 @@ def test_budget_rejection(hip_ctx, square):
     m, _ = square
     with pytest.raises(ResourceBlocked, match="RESOURCE_BLOCKED"):
         HipMatrix(small, m.H)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("L", [1, 2, 3, 64, 301])
+def test_moment_doubling_matches_direct_oracle(hip_ctx, honey, dtype, L):
+    """mu_2k = 2<T_k,T_k> - mu_0, mu_2k+1 = 2<T_k,T_k+1> - mu_1 equals the
+    reference's direct recursion (exact identity; rounding differs)."""
+    m, oe = honey
+    B = phases(m.n, 8, 13)
+    want = oe.moments(B, L, 0.05, 8.8)
+    got = host(hip_engine(hip_ctx, m).moments(to_dev(hip_ctx, B, dtype), L, 0.05, 8.8, doubling=True))
+    assert got.shape == want.shape
+    assert rel_err(got, want) < (1e-11 if dtype == "c128" else 2e-5)
