@@ -46,3 +46,12 @@ class TransportBlock:
     arrays: dict
     static: dict = field(default_factory=dict)
     metadata: dict = field(default_factory=dict)
+
+
+@dataclass
+class RunSummary:
+    output_path: str
+    run_status: str
+    method: str
+    backend: str
+    diagnostics: dict = field(default_factory=dict)

@@ -79,3 +79,27 @@ def estimate_spectral_bounds(host_H, host_S, s_lower, s_certified, spectral_boun
         source = "two-start generalized Ritz extrema plus margin"
     return {"bounds": tuple(bounds), "source": source, "certified": certified, "residuals": residuals,
             "conservative_bounds": (-b, b), "extrema": [lo, hi]}
+
+
+def certified_bounds(host_H, host_S, s_lower, margin=1e-3):
+    """Bounds that provably contain the generalized spectrum of (H, S) without
+    an eigensolver: Gershgorin discs of H when S = I, otherwise
+    +-||H||_inf / lambda_min(S) (the reference's conservative bound,
+    bounds.py:28-32), widened by a relative margin."""
+    H = sp.csr_matrix(host_H)
+    S = sp.csr_matrix(host_S)
+    n = H.shape[0]
+    identity = S.nnz == n and np.all(S.diagonal() == 1)
+    if identity:
+        d = H.diagonal().real
+        r = np.asarray(abs(H).sum(axis=1)).ravel() - np.abs(H.diagonal())
+        lo, hi = float(np.min(d - r)), float(np.max(d + r))
+        source = "Gershgorin discs of H (S = I)"
+    else:
+        b = float(np.asarray(abs(H).sum(axis=1)).max()) / s_lower
+        lo, hi = -b, b
+        source = "||H||_inf / lambda_min(S)"
+    w = max(hi - lo, 1e-12)
+    bounds = (lo - margin * w, hi + margin * w)
+    return {"bounds": bounds, "source": source, "certified": True, "residuals": [], "conservative_bounds": bounds,
+            "extrema": [lo, hi]}

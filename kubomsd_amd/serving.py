@@ -1,0 +1,265 @@
+"""Continuous-batching transport service on one MI355X.
+
+Requests (transport jobs of different sizes) arrive over time.  The server
+keeps ``slots`` workers, each with its own HIP stream and kmsd context; a job is
+admitted when a slot is free AND its estimated device footprint fits the HBM
+budget left by the jobs already on the device.  Admitted jobs run concurrently:
+their kernels interleave on the GPU through the per-slot streams, so small
+jobs that cannot fill 256 CUs alone overlap with each other and with large
+ones.  Waiting jobs are kept in size classes (by estimated bytes of one
+Chebyshev step); the scheduler takes the oldest waiting job whose footprint
+fits, scanning classes smallest-first when the oldest does not fit, so a large
+job never blocks small ones that fit (and is guaranteed to run once enough of
+the budget drains, because no new work is admitted past a job that has waited
+longer than ``max_wait_s``).
+
+Every kernel on the path reduces deterministically (fixed-order partial sums,
+launch shapes fixed by the job alone), so a job's result is bitwise the same
+alone and in any mix; tests/test_gpu_serving.py checks it.
+
+The reference has no serving layer; its closest analogue is the GPU edition's
+per-run admission (require_bytes, batch halving: kubo_msd_gpu/src/kubo_msd/
+backends/gpu_context.py:128-136, solvers/stochastic_gpu.py:31-52), which the
+per-job budget check here generalizes to many jobs.
+"""
+from collections import deque
+from dataclasses import dataclass, field
+import threading
+import time
+
+import numpy as np
+
+from .errors import ResourceBlocked
+
+
+@dataclass
+class TransportJob:
+    """One request: a model (or a callable that builds it), a RunConfig and
+    optional explicit samples.  method: 'stepwise' or 'spectral'."""
+    job_id: int
+    model: object
+    cfg: object
+    method: str = "stepwise"
+    samples: object = None
+    arrival_s: float = 0.0
+    est_bytes: int = 0
+    size_class: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    arrival_s: float
+    start_s: float
+    finish_s: float
+    slot: int
+    dos: np.ndarray = None
+    diffusion: np.ndarray = None
+    blocks: list = None
+    error: str = None
+
+    @property
+    def latency_s(self):
+        return self.finish_s - self.arrival_s
+
+
+def estimate_job_bytes(n, nnz, nb, na, precision, nonorthogonal, n_velocity=1):
+    """Device bytes of a stepwise job: matrices (FP64 + FP32 values, int32
+    indices, for H, S and the velocities) and the vector blocks of one batch."""
+    es = 16 if precision == "fp64" else 8
+    mats = (2 + n_velocity) * nnz * (8 + 4 + 4) + 64 * n
+    w = nb * (1 + na)
+    vec = n * es * (4 * w + 3 * na * nb + (7 * w + 12 * nb if nonorthogonal else 0))
+    return int(1.15 * (mats + vec)) + (64 << 20)
+
+
+class TransportServer:
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=None, class_edges=(1 << 26, 1 << 30),
+                 max_wait_s=30.0, keep_blocks=False):
+        import torch
+        self.torch = torch
+        self.device = device
+        self.slots = slots
+        if hbm_budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(device)
+            hbm_budget_bytes = int(0.85 * free)
+        self.budget = int(hbm_budget_bytes)
+        self.used = 0
+        self.class_edges = tuple(class_edges)
+        self.queues = [deque() for _ in range(len(class_edges) + 1)]
+        self.max_wait_s = max_wait_s
+        self.keep_blocks = keep_blocks
+        self.cv = threading.Condition()
+        self.results = {}
+        self.pending = 0
+        self.closed = False
+        self.t0 = None
+        self.peak_used = 0
+        self.peak_running = 0
+        self.running = 0
+
+    # -- admission -------------------------------------------------------------
+    def size_class(self, est_bytes):
+        return int(np.searchsorted(self.class_edges, est_bytes, side="right"))
+
+    def submit(self, job):
+        if job.est_bytes > self.budget:
+            raise ResourceBlocked(f"RESOURCE_BLOCKED: job {job.job_id} needs {job.est_bytes} bytes, "
+                                  f"budget {self.budget}")
+        job.size_class = self.size_class(job.est_bytes)
+        with self.cv:
+            self.queues[job.size_class].append(job)
+            self.pending += 1
+            self.cv.notify_all()
+
+    def _pick(self, now):
+        heads = [q[0] for q in self.queues if q]
+        if not heads:
+            return None
+        oldest = min(heads, key=lambda j: j.arrival_s)
+        free = self.budget - self.used
+        if oldest.est_bytes <= free:
+            return oldest
+        if now - oldest.arrival_s > self.max_wait_s:
+            return None  # drain: let the starving job get the budget next
+        for q in self.queues:  # smallest classes first
+            for job in q:
+                if job.est_bytes <= free:
+                    return job
+        return None
+
+    def _take(self):
+        with self.cv:
+            while True:
+                if self.closed and self.pending == 0:
+                    return None
+                job = self._pick(self.clock())
+                if job is not None:
+                    self.queues[job.size_class].remove(job)
+                    self.pending -= 1
+                    self.used += job.est_bytes
+                    self.running += 1
+                    self.peak_used = max(self.peak_used, self.used)
+                    self.peak_running = max(self.peak_running, self.running)
+                    return job
+                self.cv.wait(0.05)
+
+    def _release(self, job):
+        with self.cv:
+            self.used -= job.est_bytes
+            self.running -= 1
+            self.cv.notify_all()
+
+    def clock(self):
+        return time.perf_counter() - self.t0
+
+    # -- execution -------------------------------------------------------------
+    def _worker(self, slot):
+        torch = self.torch
+        from .backends.hip import HipContext
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)  # default priority
+        with torch.cuda.stream(stream):
+            ctx = HipContext(self.device, hbm_budget_bytes=self.budget)
+            while True:
+                job = self._take()
+                if job is None:
+                    return
+                start = self.clock()
+                try:
+                    res = run_job(job, ctx, self.keep_blocks)
+                    err = None
+                except Exception as exc:  # reported per job, the server keeps serving
+                    res, err = {}, f"{type(exc).__name__}: {exc}"
+                finish = self.clock()
+                self._release(job)
+                self.results[job.job_id] = JobResult(job.job_id, job.arrival_s, start, finish, slot,
+                                                     res.get("dos"), res.get("diffusion"), res.get("blocks"), err)
+
+    def run(self, jobs, realtime=True):
+        """Feeds jobs at their arrival times (seconds from start) and returns
+        {job_id: JobResult} once all have finished."""
+        self.t0 = time.perf_counter()
+        workers = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in range(self.slots)]
+        for w in workers:
+            w.start()
+        for job in sorted(jobs, key=lambda j: j.arrival_s):
+            if realtime:
+                delay = job.arrival_s - self.clock()
+                if delay > 0:
+                    time.sleep(delay)
+            self.submit(job)
+        with self.cv:
+            self.closed = True
+            self.cv.notify_all()
+        for w in workers:
+            w.join()
+        return self.results
+
+    @staticmethod
+    def summary(results, wall_s):
+        lat = np.array([r.latency_s for r in results.values() if r.error is None])
+        failed = [r.job_id for r in results.values() if r.error is not None]
+        return dict(jobs=len(results), failed=failed, wall_s=wall_s,
+                    jobs_per_s=len(lat) / wall_s if wall_s > 0 else None,
+                    p50_latency_s=float(np.percentile(lat, 50)) if len(lat) else None,
+                    p99_latency_s=float(np.percentile(lat, 99)) if len(lat) else None,
+                    mean_latency_s=float(lat.mean()) if len(lat) else None)
+
+
+def run_job(job, ctx, keep_blocks=False):
+    """Builds the job's operators on the slot's context and runs the
+    stochastic estimator; returns DOS and the diffusion estimate D(E, t)."""
+    from .metric.bounds import certified_bounds
+    from .operators.velocity import build_operators
+    from .solvers.stochastic import iter_stochastic_blocks
+    model = job.model() if callable(job.model) else job.model
+    ops = build_operators(model, job.cfg, ctx)
+    report = None if job.cfg.spectral_bounds is not None else \
+        certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower)
+    blocks = list(iter_stochastic_blocks(ops, job.cfg, job.method, samples=job.samples, bounds_report=report))
+    first = blocks[0]
+    key = "numerator_diffusion_A2_per_fs/msd_gradient"
+    diff = np.concatenate([b.arrays[key] for b in blocks], axis=1) if key in first.arrays else None
+    out = {"dos": first.static["dos_states_per_eV"], "diffusion": diff}
+    if keep_blocks:
+        out["blocks"] = blocks
+    del ops
+    return out
+
+
+def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256, 512, 1024, 2048, 4096),
+                    vectors=(4, 16, 64), nonorthogonal_fraction=0.2, precision="mixed", times=(0.0, 0.5, 1.0),
+                    seed=20261208):
+    """The serving workload: Poisson arrivals; honeycomb flakes with N
+    log-uniform in n_range, W ~ U[0.5, 3] eV, M and the vector count drawn from
+    the given sets, a fraction with a nonorthogonal overlap; each job is a
+    stepwise transport run (DOS + MSD + D(E) at `times`) on one batch."""
+    from .config import RunConfig
+    from .contracts import ModelBundle
+    from .synthetic import honeycomb_flake
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, n_jobs))
+    arrivals -= arrivals[0]
+    jobs = []
+    for i in range(n_jobs):
+        n = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        W = float(rng.uniform(0.5, 3.0))
+        M = int(rng.choice(moments))
+        nb = int(rng.choice(vectors))
+        nonorth = bool(rng.random() < nonorthogonal_fraction)
+        L = max(2, int(round(np.sqrt(n / 2))))
+        n_act = 2 * L * L
+
+        def build(L=L, W=W, nonorth=nonorth, i=i):
+            m = honeycomb_flake(2 * L * L, W=W, s_scale=0.08 if nonorth else 0.0, seed=seed + 17 * i)
+            return ModelBundle(m.H, m.S, velocity={"x": m.velocity["x"]})
+        cfg = RunConfig(energy_eV=np.linspace(-7.5, 7.5, 61), time_fs=list(times), solver="stochastic-kpm",
+                        kernel="jackson", backend="hip", n_moments=M, axes=("x",), nsamples=nb, sample_batch=nb,
+                        statistics_block=max(1, nb // 4), include_vac=False, precision=precision,
+                        metric_rtol=1e-10, seed=20261019 + i)
+        est = estimate_job_bytes(n_act, 4 * n_act, nb, 1, precision, nonorth)
+        jobs.append(TransportJob(i, build, cfg, "stepwise", None, float(arrivals[i]), est,
+                                 meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth)))
+    return jobs

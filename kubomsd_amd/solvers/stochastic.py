@@ -143,10 +143,12 @@ def _normalize_slab(raw, dos_blocks, counts, ns, valid, key):
     return arrays
 
 
-def iter_stochastic_blocks(ops, config, method, samples=None):
+def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None):
+    """bounds_report: precomputed bounds (e.g. metric.bounds.certified_bounds)
+    used instead of the generalized Ritz check."""
     cfg = config
-    report = estimate_spectral_bounds(ops.host_H, ops.host_S, ops.metric.s_lower,
-                                      ops.metric.bounds_report["certified"], cfg.spectral_bounds)
+    report = bounds_report or estimate_spectral_bounds(ops.host_H, ops.host_S, ops.metric.s_lower,
+                                                       ops.metric.bounds_report["certified"], cfg.spectral_bounds)
     cfg = replace(cfg, spectral_bounds=report["bounds"])
     ns = cfg.nsamples if samples is None else samples.shape[1]
     nbk = (ns + cfg.statistics_block - 1) // cfg.statistics_block

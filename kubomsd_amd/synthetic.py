@@ -185,3 +185,20 @@ def gershgorin_lower(S):
     d = S.diagonal().real
     radii = np.asarray(abs(S).sum(axis=1)).ravel() - np.abs(d)
     return float(np.min(d - radii))
+
+
+def honeycomb_flake(n_target, W=1.0, t=-2.7, a=1.42, s_scale=0.0, seed=LATTICE_SEED + 5):
+    """Open-boundary honeycomb flake with about n_target orbitals; with
+    s_scale > 0 a nonorthogonal overlap S = I + s_scale * U[-1,1] on the
+    nearest-neighbour bonds (diagonally dominant for s_scale < 1/3)."""
+    L = max(2, int(round(np.sqrt(n_target / 2))))
+    m = honeycomb_lattice(L, L, t=t, W=W, a=a, periodic=False, seed=seed)
+    if s_scale > 0:
+        rng = np.random.default_rng(seed + 1)
+        off = sp.triu(m.H, k=1).tocoo()
+        vals = s_scale * rng.uniform(-1, 1, off.nnz)
+        U = sp.coo_matrix((vals, (off.row, off.col)), shape=m.H.shape)
+        m.S = (sp.identity(m.n, format="csr") + U + U.T).astype(np.complex128).tocsr()
+        m.S.sort_indices()
+        m.description += f" nonorthogonal S (bond overlap {s_scale} U[-1,1])"
+    return m

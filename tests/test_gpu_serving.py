@@ -1,0 +1,55 @@
+"""Continuous batching: jobs admitted by slots and HBM budget overlap on the
+device, and every job's result is bitwise identical alone and in the mix
+(deterministic reductions); mixed-precision D(E) stays near the FP64 run."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from kubomsd_amd.serving import TransportServer, synthetic_trace
+
+pytestmark = pytest.mark.gpu
+
+
+def trace(precision="mixed"):
+    return synthetic_trace(6, rate_per_s=50.0, n_range=(800, 6000), moments=(64, 128), vectors=(4, 8),
+                           nonorthogonal_fraction=0.34, precision=precision, times=(0.0, 0.5, 1.0), seed=11)
+
+
+def test_alone_equals_mix_and_overlap():
+    alone = {}
+    for job in trace():
+        srv = TransportServer(slots=1, hbm_budget_bytes=8 << 30)
+        job.arrival_s = 0.0
+        alone.update(srv.run([job], realtime=False))
+    srv = TransportServer(slots=4, hbm_budget_bytes=8 << 30)
+    jobs = trace()
+    for j in jobs:
+        j.arrival_s = 0.0
+    mix = srv.run(jobs, realtime=False)
+    assert srv.peak_running >= 2, "jobs never overlapped"
+    assert any(j.meta["nonorthogonal"] for j in jobs)
+    for jid, r in mix.items():
+        assert r.error is None and alone[jid].error is None, (r.error, alone[jid].error)
+        np.testing.assert_array_equal(r.dos, alone[jid].dos)
+        np.testing.assert_array_equal(r.diffusion, alone[jid].diffusion)
+
+
+def test_mixed_precision_diffusion_vs_fp64():
+    srv = TransportServer(slots=3, hbm_budget_bytes=8 << 30)
+    mixed = srv.run(trace("mixed"), realtime=False)
+    srv = TransportServer(slots=3, hbm_budget_bytes=8 << 30)
+    fp64 = srv.run(trace("fp64"), realtime=False)
+    for jid in mixed:
+        a, b = mixed[jid].diffusion, fp64[jid].diffusion
+        assert mixed[jid].error is None and fp64[jid].error is None
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        assert err < 1e-4, (jid, err)
+
+
+def test_budget_admission_rejects_oversized_job():
+    from kubomsd_amd.errors import ResourceBlocked
+    jobs = trace()
+    srv = TransportServer(slots=1, hbm_budget_bytes=1 << 20)
+    with pytest.raises(ResourceBlocked, match="RESOURCE_BLOCKED"):
+        srv.submit(jobs[0])

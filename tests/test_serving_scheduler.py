@@ -1,0 +1,62 @@
+"""Admission logic of the transport server (host only, no device calls)."""
+import numpy as np
+
+from kubomsd_amd.serving import TransportJob, TransportServer, estimate_job_bytes, synthetic_trace
+
+
+class _Srv(TransportServer):
+    def __init__(self, budget):  # no torch device needed for admission
+        self.budget, self.used, self.class_edges = budget, 0, (100, 1000)
+        self.queues = [__import__("collections").deque() for _ in range(3)]
+        self.max_wait_s, self.cv = 5.0, __import__("threading").Condition()
+        self.pending, self.closed, self.running, self.peak_used, self.peak_running = 0, False, 0, 0, 0
+        self.t0 = 0.0
+
+    def clock(self):
+        return self.now
+
+
+def job(i, b, t):
+    return TransportJob(i, None, None, arrival_s=t, est_bytes=b)
+
+
+def test_oldest_fitting_job_first_and_small_jobs_pass_a_blocked_large_one():
+    s = _Srv(1000)
+    s.now = 1.0
+    for j in (job(0, 900, 0.0), job(1, 50, 0.1), job(2, 40, 0.2), job(3, 700, 0.3)):
+        s.submit(j)
+    assert s._take().job_id == 0          # oldest fits: admitted
+    assert s._pick(1.0).job_id == 1       # 900 used: small ones pass the waiting 700
+    assert s._take().job_id == 1
+    assert s._take().job_id == 2
+    assert s.used == 990
+    # 700 does not fit until the big one releases
+    assert s._pick(1.0) is None
+    s._release(job(0, 900, 0.0))
+    assert s._take().job_id == 3
+
+
+def test_starving_job_blocks_new_admissions():
+    s = _Srv(1000)
+    s.submit(job(0, 800, 0.0))
+    s.now = 0.5
+    assert s._take().job_id == 0
+    s.submit(job(1, 900, 1.0))  # cannot fit while job 0 runs
+    s.submit(job(2, 10, 2.0))
+    s.now = 2.0
+    assert s._pick(s.now).job_id == 2   # not yet starving: small job passes
+    s.now = 10.0
+    assert s._pick(s.now) is None       # job 1 waited > max_wait_s: drain for it
+
+
+def test_size_classes_and_trace_shape():
+    s = _Srv(10 ** 12)
+    assert [s.size_class(b) for b in (50, 100, 500, 5000)] == [0, 1, 1, 2]
+    jobs = synthetic_trace(32, rate_per_s=8.0, n_range=(1e3, 1e4), seed=3)
+    assert len(jobs) == 32 and jobs[0].arrival_s == 0.0
+    assert np.all(np.diff([j.arrival_s for j in jobs]) >= 0)
+    ns = [j.meta["n"] for j in jobs]
+    assert 400 <= min(ns) and max(ns) <= 2 * 10 ** 4
+    assert {j.meta["M"] for j in jobs} <= {256, 512, 1024, 2048, 4096}
+    assert estimate_job_bytes(10 ** 6, 4 * 10 ** 6, 64, 1, "mixed", True) > \
+        estimate_job_bytes(10 ** 6, 4 * 10 ** 6, 64, 1, "mixed", False)
