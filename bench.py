@@ -42,14 +42,15 @@ def parse():
     p.add_argument("--steps", type=int, default=5, help="timed moment chains")
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--cells", type=int, default=1000, help="honeycomb cells per side")
-    p.add_argument("--vectors", type=int, default=64)
-    p.add_argument("--moments", type=int, default=2048)
+    p.add_argument("--vectors", type=int, default=None, help="default 64 (honeycomb), 32 (nonorthogonal)")
+    p.add_argument("--moments", type=int, default=None, help="default 2048 (honeycomb), 1024 (nonorthogonal)")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     p.add_argument("--no-fp64-check", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=5.0, help="target seconds per CPU baseline run")
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
+    p.add_argument("--pcg-poll", type=int, default=0, help="override PCG iterations between host polls")
     p.add_argument("--serving", action="store_true",
                    help="run the continuous-batching serving trace instead of the chain benchmark")
     p.add_argument("--jobs", type=int, default=64)
@@ -57,9 +58,20 @@ def parse():
     p.add_argument("--n-max", type=float, default=4e5, help="largest job size (orbitals); reference: 4e6")
     p.add_argument("--slots", type=int, default=6)
     p.add_argument("--check-jobs", type=int, default=6, help="jobs re-run alone in FP64 for the D(E) error")
+    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu"], default="honeycomb",
+                   help="honeycomb: orthogonal moment chain (default); nonorthogonal: simple-cubic 64^3 x 4 "
+                        "orbitals, S^-1 H chain with batched block-Jacobi PCG; mu: square 2048^2 double-moment "
+                        "tables mu_mn on the matrix cores")
+    p.add_argument("--side", type=int, default=2048, help="mu: square lattice side")
+    p.add_argument("--samples", type=int, default=32, help="mu: random vectors (one table each)")
+    p.add_argument("--atoms", type=int, default=64, help="nonorthogonal: atoms per cube side")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 1 GiB device copy (torch copy kernel), to scale HBM counters")
-    return p.parse_args()
+    a = p.parse_args()
+    nonorth = a.workload == "nonorthogonal"
+    a.vectors = a.vectors or (32 if nonorth else 64)
+    a.moments = a.moments or (1024 if a.workload != "honeycomb" else 2048)
+    return a
 
 
 def calibrate_copy(reps=5):
@@ -115,6 +127,256 @@ def serving(args):
     print(json.dumps(line))
 
 
+# |lambda(S^-1 H)| <= |H|_2 / lambda_min(S) for the seeded simple-cubic model
+# (64^3 atoms, 4 orbitals): |H|_2 and lambda_min(S) from host Lanczos (scipy
+# eigsh, tol 1e-6 / 1e-4: |H|_2 = 9.6389, lambda_min(S) = 0.31957), widened
+# by 2%.  A chain run with bounds that miss the spectrum grows exponentially;
+# the bench checks |mu_l| <= mu_0.
+CUBIC64_BOUND = 1.02 * 9.6389 / 0.31957
+
+
+def atom_block_factors(S, norb):
+    """Lower Cholesky factors of the norb x norb on-site blocks of S, as the
+    reference's block-Jacobi input [(a, b, L)] (metric/setup.py:73-76)."""
+    coo = S.tocoo()
+    nat = S.shape[0] // norb
+    keep = coo.row // norb == coo.col // norb
+    blk = np.zeros((nat, norb, norb), np.complex128)
+    blk[coo.row[keep] // norb, coo.row[keep] % norb, coo.col[keep] % norb] = coo.data[keep]
+    L = np.linalg.cholesky(blk)
+    return [(q * norb, (q + 1) * norb, L[q]) for q in range(nat)]
+
+
+def pcg_iteration_bytes(nnz_s, n, nb, s_val, s_vec):
+    """Lower bound of HBM bytes per PCG iteration on an nb-column block: the
+    S product (values, indices, row pointers) and six block streams (read P;
+    read and write R; read and write Y; write P)."""
+    return nnz_s * (s_val + 4) + 4 * (n + 1) + 6 * n * nb * s_vec
+
+
+def nonorthogonal(args):
+    """Simple-cubic 64^3 atoms x 4 orbitals (N = 1,048,576), 4x4 blocks to the
+    second shell, nonorthogonal S; 32 vectors; M = 1024 moments of S^-1 H, each
+    order one H product plus a block-Jacobi PCG solve of S (FP32 inner solves
+    to 1e-6 inside FP64 refinement to 1e-10).  Reports vector-steps/s and PCG
+    iterations/s (inner iterations, max over columns per solve)."""
+    import torch
+    from kubomsd_amd.backends.hip import HipContext, HipEngine, HipMetric
+    from kubomsd_amd.synthetic import cubic_orbital_model
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    ctx = HipContext(0)
+    # 4 complex columns per lane: the fastest BSR4 product shape for 32
+    # complex64 columns in a sweep of cpl x workgroups on MI355X.
+    ctx.set_tuning("cpl", args.cpl or 4)
+    if args.groups:
+        ctx.set_tuning("groups", args.groups)
+    if args.pcg_poll:
+        ctx.set_tuning("pcg_poll", args.pcg_poll)
+    t_build = time.perf_counter()
+    model = cubic_orbital_model(args.atoms)
+    n, nb, M = model.n, args.vectors, args.moments
+    blocks = atom_block_factors(model.S, 4)
+    metric = HipMetric(ctx, model.S, model.S.diagonal().real, blocks)
+    eng = HipEngine(ctx, model.H, metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
+    bound = CUBIC64_BOUND if args.atoms == 64 and CUBIC64_BOUND else None
+    if bound is None:
+        import scipy.sparse.linalg as sla
+        lmin = sla.eigsh(model.S.real.tocsr(), k=1, which="SA", tol=1e-6, return_eigenvectors=False,
+                         v0=np.random.default_rng(1).normal(size=n))[0]
+        hmax = abs(sla.eigsh(model.H.real.tocsr(), k=1, which="LM", tol=1e-4, return_eigenvectors=False,
+                             v0=np.random.default_rng(2).normal(size=n))[0])
+        bound = 1.02 * hmax / lmin
+    c, a = 0.0, bound
+    Bh = random_phases(n, SEED, range(nb))
+    B = torch.from_numpy(Bh).to(torch.complex64).to(ctx.torch_device)
+    print(f"# setup {time.perf_counter() - t_build:.1f}s: N={n} nnz(H)={model.H.nnz} nnz(S)={model.S.nnz} nb={nb} "
+          f"M={M} bounds=+-{bound:.3f}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        eng.moments(B, 8, c, a)
+    torch.cuda.synchronize()
+    d0 = metric.diagnostics()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        mu = eng.moments(B, M, c, a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    d1 = metric.diagnostics()
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    its = d1["max_iterations_sum"] - d0["max_iterations_sum"]
+    solves = d1["calls"] - d0["calls"]
+    mu_h = mu.cpu().numpy()
+    growth = float(np.max(np.abs(mu_h)) / np.max(np.abs(mu_h[0])))
+    print(f"# timed {args.steps} chains: {gpu_ms:.1f} ms, {solves} solves, {its} PCG iterations, "
+          f"max|mu_l|/mu_0 = {growth:.3f}", file=sys.stderr, flush=True)
+    if growth > 1.01:
+        raise SystemExit("moments grow: spectral bounds do not enclose the spectrum")
+    value = nb * (M - 1) * args.steps / (gpu_ms / 1e3)
+    s_vec = 8
+    step_b = step_bytes(model.H.nnz, n, nb, 4, s_vec)
+    pcg_b = pcg_iteration_bytes(model.S.nnz, n, nb, 4, s_vec)
+    alg = (M - 1) * args.steps * step_b + its * pcg_b
+    achieved = alg / (gpu_ms / 1e3) / 1e9
+    err = None
+    if not args.no_fp64_check:
+        L = min(64, M)
+        B64 = torch.from_numpy(Bh).to(ctx.torch_device)
+        mu64 = eng.moments(B64, L, c, a).cpu().numpy()
+        err = dict(moments_compared=L, mu_max_rel=float(np.max(np.abs(mu_h[:L] - mu64)) / np.max(np.abs(mu64))))
+        del B64
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(model, Bh[:, :8], c, a, args.cpu_seconds, blocks=blocks)
+    line = {
+        "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
+        "value": value, "unit": "vector-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64+f64", "data": "synthetic (seeded simple-cubic orbital model, NumPy PCG64 random phases)",
+        "config": {"workload": f"simple cubic {args.atoms}^3 atoms x 4 orbitals (N={n}), nonorthogonal S, "
+                               f"{nb} vectors, M={M} moments of S^-1 H, block-Jacobi PCG (FP32 inner to 1e-6, "
+                               f"FP64 refinement to 1e-10), one moment chain per step",
+                   "n": n, "nnz_H": int(model.H.nnz), "nnz_S": int(model.S.nnz), "vectors": nb, "moments": M,
+                   "precision": "mixed", "parallelism": "single GPU"},
+        "pcg": {"solves": solves, "inner_iterations": its, "iterations_per_solve": its / max(solves, 1),
+                "iterations_per_s": its / (gpu_ms / 1e3),
+                "algorithmic_bytes_per_iteration": pcg_b},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_chain": alg / args.steps,
+                     "kernel": "whole chain: H products, recursion and inner PCG iterations (lower-bound bytes "
+                               "over event time; FP64 refinement passes not counted)"},
+        "err_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}
+
+
+def double_moments(args):
+    """Square 2048 x 2048 periodic lattice (N = 4,194,304), t = -1 eV, Anderson
+    W = 1.5 eV; per random vector r: A = [T_n(H~) V r] and B = [V T_m(H~) r]
+    for n, m < M = 1024 (complex64 chains, (M, N) each), then the table
+    mu_nm = sum_i conj(A_ni) B_mi on the matrix cores, FP32 MFMA and the bf16x3
+    split variant.  Algorithmic flops per table 8 N M^2 (complex MAC = 8 real
+    flops).  Errors against FP64: the same GEMM in complex128 on the same
+    inputs (all entries), and the full FP64 pipeline (complex128 chains and
+    GEMM) for the columns m < 64 of the first vector."""
+    import torch
+    from kubomsd_amd.backends.hip import (HipContext, HipEngine, HipMetric, KMSD_MU_BF16X3, KMSD_MU_FP32,
+                                          mu_table)
+    from kubomsd_amd.synthetic import square_lattice
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    ctx = HipContext(0)
+    t_build = time.perf_counter()
+    model = square_lattice(args.side, t=-1.0, W=1.5)
+    n, M = model.n, args.moments
+    bound = 4.0 + 0.75 + 0.05  # Gershgorin |E| <= 4|t| + W/2, plus margin
+    c, a = 0.0, bound
+    metric = HipMetric(ctx, model.S, model.S.diagonal().real)
+    V = model.velocity["x"]
+    eng = HipEngine(ctx, model.H, metric, [(0, [V]), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
+    from kubomsd_amd.backends.hip import HipMatrix
+    Vd = HipMatrix(ctx, V)
+    flops = 8.0 * n * M * M
+    print(f"# setup {time.perf_counter() - t_build:.1f}s: N={n} nnz={model.H.nnz} M={M} samples={args.samples}",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    times = {"chains_ms": 0.0, "velocity_ms": 0.0, "fp32_ms": 0.0, "bf16x3_ms": 0.0}
+    err = None
+    t_all = time.perf_counter()
+    for sid in range(args.samples):
+        r = torch.from_numpy(random_phases(n, SEED, [sid])).to(torch.complex64).to(ctx.torch_device)
+        e = [ev() for _ in range(7)]
+        e[0].record(stream)
+        A = eng.chain(eng.v_cov(r, (1.0, 0.0, 0.0)), M, c, a)
+        Cc = eng.chain(r, M, c, a)
+        e[1].record(stream)
+        Ct = Cc.view(M, n).t().contiguous()
+        del Cc
+        VCt = Vd.matmul(Ct)
+        del Ct
+        Bm = VCt.t().contiguous().view(M, n, 1)
+        del VCt
+        e[2].record(stream)
+        mu32 = mu_table(ctx, A, Bm, KMSD_MU_FP32)
+        e[3].record(stream)
+        mub = mu_table(ctx, A, Bm, KMSD_MU_BF16X3)
+        e[4].record(stream)
+        torch.cuda.synchronize()
+        times["chains_ms"] += e[0].elapsed_time(e[1])
+        times["velocity_ms"] += e[1].elapsed_time(e[2])
+        times["fp32_ms"] += e[2].elapsed_time(e[3])
+        times["bf16x3_ms"] += e[3].elapsed_time(e[4])
+        if sid == 0 and not args.no_fp64_check:
+            err = mu_errors(ctx, eng, Vd, r, A, Bm, mu32[0], mub[0], M, c, a)
+        del A, Bm
+        print(f"# sample {sid}: fp32 {e[2].elapsed_time(e[3]):.1f} ms, bf16x3 {e[3].elapsed_time(e[4]):.1f} ms",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t_all
+    ns = args.samples
+    tf32 = flops * ns / (times["fp32_ms"] / 1e3) / 1e12
+    tfb = flops * ns / (times["bf16x3_ms"] / 1e3) / 1e12
+    line = {
+        "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
+        "value": tf32, "unit": "TFLOP/s (mu_nm table, FP32 MFMA)", "n_gpus": 1, "steps": ns, "warmup": 0,
+        "ms_per_step": times["fp32_ms"] / ns, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (FP32 MFMA, FP64 split-sum)", "data": "synthetic (seeded square lattice, NumPy PCG64 phases)",
+        "config": {"workload": f"square {args.side}^2 (N={n}), t=-1 eV, W=1.5 eV, {ns} random vectors, "
+                               f"mu_nm {M}x{M} complex per vector", "n": n, "moments": M, "samples": ns,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": tf32, "peak": MFMA_PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": tf32 / MFMA_PEAK_TFLOPS["fp32"], "traffic": None,
+                     "algorithmic_flops_per_launch": flops, "kernel": "mu_mfma_f32_kernel (+ split sum)"},
+        "bf16x3": {"achieved_TFLOPs": tfb, "mfma_executed_TFLOPs": 3 * tfb,
+                   "frac_of_bf16_peak": 3 * tfb / MFMA_PEAK_TFLOPS["bf16"], "ms_per_table": times["bf16x3_ms"] / ns},
+        "pipeline_ms_per_vector": {k: v / ns for k, v in times.items()},
+        "wall_s": wall,
+        "err_vs_fp64": err,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
+def mu_errors(ctx, eng, Vd, r, A, Bm, mu32, mub, M, c, a):
+    """Max-entry relative errors of the FP32 and bf16x3 tables against FP64:
+    (1) the complex128 GEMM of the same complex64 chains, over N in chunks;
+    (2) the complex128 pipeline (chains, velocity, GEMM) for columns m < 64."""
+    import torch
+    n = A.shape[1]
+    A2, B2 = A.view(M, n), Bm.view(M, n)
+    ref = torch.zeros((M, M), dtype=torch.complex128, device=A.device)
+    step = 1 << 18
+    for i0 in range(0, n, step):
+        a64 = A2[:, i0:i0 + step].to(torch.complex128)
+        b64 = B2[:, i0:i0 + step].to(torch.complex128)
+        ref += a64.conj() @ b64.transpose(0, 1).contiguous()
+        del a64, b64
+    scale = ref.abs().max().item()
+    out = {"gemm_fp32_max_rel": (mu32 - ref).abs().max().item() / scale,
+           "gemm_bf16x3_max_rel": (mub - ref).abs().max().item() / scale}
+    k = 64
+    r64 = r.to(torch.complex128)
+    A64 = eng.chain(eng.v_cov(r64, (1.0, 0.0, 0.0)), M, c, a).view(M, n)
+    C64 = eng.chain(r64, k, c, a).view(k, n)
+    B64 = Vd.matmul(C64.t().contiguous()).t().contiguous()
+    del C64
+    full = torch.zeros((M, k), dtype=torch.complex128, device=A.device)
+    for i0 in range(0, n, step):
+        full += A64[:, i0:i0 + step].conj().resolve_conj() @ B64[:, i0:i0 + step].transpose(0, 1).contiguous()
+    del A64, B64
+    s2 = full.abs().max().item()
+    out["pipeline_fp32_max_rel_cols64"] = (mu32[:, :k] - full).abs().max().item() / s2
+    out["pipeline_bf16x3_max_rel_cols64"] = (mub[:, :k] - full).abs().max().item() / s2
+    return out
+
+
 def random_phases(n, seed, ids):
     """exp(2 pi i u), u = PCG64(SeedSequence([seed, id])).random(n) per column
     (reference src/kubo_msd/metric/sampling.py:8-9)."""
@@ -141,7 +403,7 @@ def cpu_threads():
     return max(1, min(aff, int(env))) if env else aff
 
 
-def cpu_baseline(model, B, c, a, target_s):
+def cpu_baseline(model, B, c, a, target_s, blocks=()):
     """Times the first orders of the FP64 DOS moment chain on the host cores
     (median of 3 after a warm-up, as benchmarks/benchmark_native.py:68-75)."""
     threads = cpu_threads()
@@ -155,12 +417,12 @@ def cpu_baseline(model, B, c, a, target_s):
             m = sp.csr_matrix(m, dtype=np.complex128)
             return ref.CSR(m.indptr.astype(np.int64), m.indices.astype(np.int64), m.data, m.shape[0], threads)
         S = csr(model.S)
-        metric = ref.Metric(S, model.S.diagonal().real.copy(), [], threads)
+        metric = ref.Metric(S, model.S.diagonal().real.copy(), list(blocks), threads)
         eng = ref.Engine(csr(model.H), metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000,
                          0.6582119569509067)
         kind = "reference"
     else:
-        metric = O.Metric(O.Csr(model.S, threads), model.S.diagonal().real, (), threads)
+        metric = O.Metric(O.Csr(model.S, threads), model.S.diagonal().real, blocks, threads)
         eng = O.Engine(O.Csr(model.H, threads), metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
         kind = "port"
     t0 = time.perf_counter()
@@ -187,6 +449,16 @@ def main():
         return
     if args.serving:
         serving(args)
+        return
+    if args.workload == "mu":
+        if args.gpus != 1:
+            sys.exit("this build runs on one GPU only")
+        double_moments(args)
+        return
+    if args.workload == "nonorthogonal":
+        if args.gpus != 1:
+            sys.exit("this build runs on one GPU only")
+        nonorthogonal(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sample sharding is a design "

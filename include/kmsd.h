@@ -35,7 +35,7 @@ extern "C" {
 
 enum { KMSD_OK = 0, KMSD_EINVAL = 1, KMSD_ENUMERIC = 2, KMSD_ERESOURCE = 3, KMSD_EDEVICE = 4 };
 enum { KMSD_C64 = 1, KMSD_C128 = 2 };
-enum { KMSD_CSR = 0, KMSD_BSR4 = 1 };
+enum { KMSD_CSR = 0, KMSD_BSR4 = 1, KMSD_AUTO = 2 };
 enum { KMSD_PRECOND_DIAGONAL = 0, KMSD_PRECOND_BLOCK_JACOBI = 1 };
 enum { KMSD_MU_FP32 = 0, KMSD_MU_BF16X3 = 1, KMSD_MU_FP64 = 2 };
 
@@ -67,8 +67,9 @@ int kmsd_ctx_synchronize(kmsd_ctx *ctx);
 /* bytes currently charged to the budget and the budget itself */
 int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
 /* Launch tuning: key "groups" (workgroups per launch, multiple of 8),
- * "cpl" (complex columns per lane for complex64 blocks: 1 or 2),
- * "moment_chunk" (Chebyshev orders per partial-reduction launch). */
+ * "cpl" (complex columns per lane: 1, 2 or 4), "moment_chunk" (Chebyshev
+ * orders per partial-reduction launch), "fp32_math" (complex64 blocks: FP32
+ * values and row sums), "pcg_poll" (PCG iterations between host polls). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* Message of the last failure in ctx (or of the last failed create when ctx
  * is NULL).  Valid until the next call on ctx. */
@@ -82,10 +83,19 @@ int kmsd_build_info(char *buf, int64_t len);
  * threads) (cpp/native.cpp:54-74, 640-646).  Same validation: monotone
  * pointers starting at 0 and ending at nnz, indices in [0,n), finite values.
  * values: complex128 interleaved (2*nnz doubles).  Purely real or purely
- * imaginary matrices are detected and stored as one real array. */
+ * imaginary matrices are detected and stored as one real array.
+ * fmt is the device storage: KMSD_CSR (diagonal split off, ELL when rows
+ * are near-uniform), KMSD_BSR4 (4x4 blocks; n a multiple of 4; the input is
+ * still CSR and is blocked here) or KMSD_AUTO (BSR4 when n is a multiple of
+ * 4 and the nonzero 4x4 blocks are at least 75% full, else CSR). */
 int kmsd_matrix_create(kmsd_ctx *ctx, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
                        const int64_t *colidx, const double *values, kmsd_matrix **out);
 int kmsd_matrix_destroy(kmsd_matrix *m);
+/* No reference counterpart (the reference's CSR exposes no layout): out[6] =
+ * {storage format (KMSD_CSR/KMSD_BSR4), n, nnz as given, stored entries
+ * (off-diagonal or block entries), device bytes, value kind (0 real,
+ * 1 imaginary, 2 complex)}. */
+int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out);
 /* Reference: CSR.matmul(B) (cpp/native.cpp:75-89, 642-646): out = A B. */
 int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int dtype);
 

@@ -23,7 +23,7 @@ from ..units import HBAR_EV_FS
 LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libkmsd.so"
 KMSD_OK, KMSD_EINVAL, KMSD_ENUMERIC, KMSD_ERESOURCE, KMSD_EDEVICE = range(5)
 KMSD_C64, KMSD_C128 = 1, 2
-KMSD_CSR, KMSD_BSR4 = 0, 1
+KMSD_CSR, KMSD_BSR4, KMSD_AUTO = 0, 1, 2
 KMSD_MU_FP32, KMSD_MU_BF16X3, KMSD_MU_FP64 = 0, 1, 2
 
 _lib = None
@@ -41,6 +41,7 @@ SIGNATURES = {
     "kmsd_build_info": [_P, _I64],
     "kmsd_matrix_create": [_P, _I, _I64, _I64, _P, _P, _P, _P],
     "kmsd_matrix_destroy": [_P],
+    "kmsd_matrix_info": [_P, _P],
     "kmsd_matrix_matmul": [_P, _P, _P, _I64, _I],
     "kmsd_metric_create": [_P, _P, _P, _I64, _P, _P, _P],
     "kmsd_metric_destroy": [_P],
@@ -205,9 +206,11 @@ def _csr_host(matrix):
 
 
 class HipMatrix:
-    """Device CSR matrix (reference CSR, cpp/native.cpp:54-90)."""
+    """Device sparse matrix (reference CSR, cpp/native.cpp:54-90).  fmt picks
+    the device storage: KMSD_AUTO (default: 4x4 blocks when they are at least
+    75% full, else CSR), KMSD_CSR or KMSD_BSR4."""
 
-    def __init__(self, ctx, matrix):
+    def __init__(self, ctx, matrix, fmt=KMSD_AUTO):
         self.ctx = ctx
         m = _csr_host(matrix)
         if m.shape[0] != m.shape[1]:
@@ -218,10 +221,16 @@ class HipMatrix:
         idx = np.ascontiguousarray(m.indices, np.int64)
         val = np.ascontiguousarray(m.data, np.complex128)
         h = C.c_void_p()
-        ctx.check(ctx.lib.kmsd_matrix_create(ctx.h, KMSD_CSR, self.n, self.nnz, _np_ptr(ptr), _np_ptr(idx),
+        ctx.check(ctx.lib.kmsd_matrix_create(ctx.h, int(fmt), self.n, self.nnz, _np_ptr(ptr), _np_ptr(idx),
                                              _np_ptr(val), C.byref(h)))
         self.h = h
         self.host = m
+
+    def info(self):
+        v = np.zeros(6, np.int64)
+        self.ctx.check(self.ctx.lib.kmsd_matrix_info(self.h, _np_ptr(v)))
+        return dict(format={KMSD_CSR: "csr", KMSD_BSR4: "bsr4"}[int(v[0])], n=int(v[1]), nnz=int(v[2]),
+                    stored=int(v[3]), device_bytes=int(v[4]), values=("real", "imag", "complex")[int(v[5])])
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -96,6 +96,7 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
+
         } else {
             throw Error(KMSD_EINVAL, "unknown tuning key");
         }
@@ -138,6 +139,17 @@ extern "C" int kmsd_matrix_destroy(kmsd_matrix *m) {
     if (!m) return KMSD_OK;
     matrix_release(m);
     delete m;
+    return KMSD_OK;
+}
+
+extern "C" int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out) {
+    if (!m || !out) return KMSD_EINVAL;
+    out[0] = m->fmt;
+    out[1] = m->n;
+    out[2] = m->nnz;
+    out[3] = m->noff;
+    out[4] = int64_t(m->bytes);
+    out[5] = m->vkind;
     return KMSD_OK;
 }
 

@@ -164,7 +164,7 @@ extern "C" int kmsd_engine_create(kmsd_ctx *c, kmsd_matrix *H, kmsd_metric *metr
     kmsd_engine *e = new kmsd_engine();
     const int rc = guard(c, [&] {
         if (!H || !metric || !kinds || H->n != metric->n) throw Error(KMSD_EINVAL, "invalid engine operators");
-        if (H->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "engine H must be CSR");
+        if (H->fmt != KMSD_CSR && H->fmt != KMSD_BSR4) throw Error(KMSD_EINVAL, "engine H must be CSR or BSR4");
         e->ctx = c;
         e->H = H;
         e->metric = metric;
@@ -174,7 +174,7 @@ extern "C" int kmsd_engine_create(kmsd_ctx *c, kmsd_matrix *H, kmsd_metric *metr
             if (kinds[a] < -1 || kinds[a] > 2) throw Error(KMSD_EINVAL, "invalid velocity kind");
             for (int k = 0; k < cnt; ++k) {
                 kmsd_matrix *m = mats ? mats[3 * a + k] : nullptr;
-                if (!m || m->n != H->n || m->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "invalid engine operators");
+                if (!m || m->n != H->n) throw Error(KMSD_EINVAL, "invalid engine operators");
                 e->mats[a][k] = m;
             }
         }

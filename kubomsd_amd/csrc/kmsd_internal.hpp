@@ -32,9 +32,11 @@ struct kmsd_matrix {
     kmsd_ctx *ctx = nullptr;
     int fmt = KMSD_CSR;
     int64_t n = 0, nnz = 0;  // nnz of the full matrix as given
-    // Device layout: the diagonal as a dense array plus the off-diagonal part
-    // as ELL (when rows have near-uniform length) or CSR.
-    int64_t noff = 0;        // stored off-diagonal entries (ELL: n*ell incl. padding)
+    // Device layout (KMSD_CSR): the diagonal as a dense array plus the
+    // off-diagonal part as ELL (when rows have near-uniform length) or CSR.
+    // KMSD_BSR4: ptr/idx are block-row pointers and block columns of the
+    // nonzero 4x4 blocks, val* 16 row-major entries per block, no split diagonal.
+    int64_t noff = 0;        // stored off-diagonal entries (ELL: n*ell incl. padding; BSR4: 16 * blocks)
     int32_t ell = 0;         // ELL width, 0 = CSR
     int32_t *ptr = nullptr;  // CSR row pointers of the off-diagonal part (CSR only)
     int32_t *idx = nullptr;
@@ -139,6 +141,7 @@ struct Step {
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
     int64_t expect_groups = 0;  // if nonzero, the launch must use exactly this many workgroups
+    bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
 };
 // Launches one fused step; returns the number of workgroups used (the
 // partial-sum row count) so callers can size/reduce partials.

@@ -116,6 +116,96 @@ struct Pack {
     }
 };
 
+// Epilogue of one output row: t = alpha y (+ beta X_i) (+ gamma P_i), then the
+// optional store, series accumulation and moment partials.  Arithmetic in
+// FP64 whatever the storage type: one rounding per stored element, and
+// alpha/beta/gamma are never rounded to FP32.
+template <class VT, class AT, int CPL>
+__device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, int64_t row, int col,
+                                              const cx<AT> (&y)[CPL], const Pack<VT, CPL> &xi,
+                                              const Pack<VT, CPL> &pv, const Pack<VT, CPL> &wv,
+                                              cx<double> (&mom)[CPL]) {
+    using V = cx<VT>;
+    Pack<VT, CPL> ts;
+    V *ACC = static_cast<V *>(a.ACC);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        cx<double> t = cmul<double>(a.alpha_re, a.alpha_im, to<double>(y[c]));
+        if (fl & F_BETA) {
+            const cx<double> xs = cmul<double>(a.beta_re, a.beta_im, xi.get(c));
+            t.re += xs.re;
+            t.im += xs.im;
+        }
+        if (fl & F_P) {
+            const cx<double> p = pv.get(c);
+            t.re = __builtin_fma(a.gamma, p.re, t.re);
+            t.im = __builtin_fma(a.gamma, p.im, t.im);
+        }
+        ts.set(c, t);
+        if (fl & F_ACC) {
+            cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
+            const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t);
+            acc.re += add.re;
+            acc.im += add.im;
+            ACC[row * a.lda + col + c] = to<VT>(acc);
+        }
+    }
+    if (fl & F_STORE) ts.store_nt(static_cast<V *>(a.OUT) + row * a.ldo + col);
+    if (fl & F_MOM2) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const cx<double> xv = xi.get(c), tv = ts.get(c);
+            mom[c].re = __builtin_fma(xv.re, xv.re, __builtin_fma(xv.im, xv.im, mom[c].re));
+            mom[c].im = __builtin_fma(xv.re, tv.re, __builtin_fma(xv.im, tv.im, mom[c].im));
+        }
+    }
+    if (fl & F_MOM) {
+        // Products of the stored (rounded) values, accumulated in FP64;
+        // FP32 x FP32 products are exact in FP64.
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const cx<double> wd = wv.get(c), tv = ts.get(c);
+            mom[c].re = __builtin_fma(wd.re, tv.re, mom[c].re);
+            mom[c].re = __builtin_fma(wd.im, tv.im, mom[c].re);
+            mom[c].im = __builtin_fma(wd.re, tv.im, mom[c].im);
+            mom[c].im = __builtin_fma(-wd.im, tv.re, mom[c].im);
+        }
+    }
+}
+
+// Workgroup partial moments: lanes sharing a column (bits >= lpr_log2 of the
+// lane index) are summed by shuffles, the four wavefronts through LDS, and
+// one partial per (workgroup, column) is written.  Fixed order.
+template <int CPL>
+__device__ __forceinline__ void step_partials(const StepArgs &a, cx<double> (&mom)[CPL], int lane, int wave, int lpr,
+                                              int col, bool colok) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        for (int off = lpr; off < 64; off <<= 1) {
+            mom[c].re += __shfl_xor(mom[c].re, off);
+            mom[c].im += __shfl_xor(mom[c].im, off);
+        }
+    }
+    __shared__ cx<double> red[4][64 * CPL];
+    if (lane < lpr) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = mom[c];
+    }
+    __syncthreads();
+    if (wave == 0 && lane < lpr && colok) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (col + c >= a.w) continue;
+            cx<double> s = red[0][lane * CPL + c];
+            for (int q = 1; q < 4; ++q) {
+                s.re += red[q][lane * CPL + c].re;
+                s.im += red[q][lane * CPL + c].im;
+            }
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
+        }
+    }
+}
+
 // VT: vector element real type (float -> complex64, double -> complex128)
 // MR: matrix value real type; MC: complex matrix values
 // AT: arithmetic type of the row sums (double for FP64 vectors; float or
@@ -144,9 +234,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     const V *X = static_cast<const V *>(a.X);
     const V *Z = static_cast<const V *>(a.Z);
     const V *P = static_cast<const V *>(a.P);
-    V *OUT = static_cast<V *>(a.OUT);
     const V *W = static_cast<const V *>(a.W);
-    V *ACC = static_cast<V *>(a.ACC);
     const MT *val = static_cast<const MT *>(a.val);
     const MT *dval = static_cast<const MT *>(a.dval);
     const uint32_t fl = a.flags;
@@ -230,81 +318,174 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 #pragma unroll
             for (int c = 0; c < CPL; ++c) y[c] = zv.template geta<AT>(c);
         }
-        // Epilogue in FP64 whatever the storage type: one rounding per stored
-        // element, and alpha/beta/gamma are never rounded to FP32.
-        PK ts;
+        step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
+    }
+
+    if (!(fl & (F_MOM | F_MOM2))) return;
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
+}
+
+// Broadcast of lane E of every 16-lane row to the whole row (DPP
+// row_newbcast, one VALU move per 32-bit word).
+template <int E, class T>
+__device__ __forceinline__ T row_bcast(T v) {
+    constexpr int NW = sizeof(T) / 4;
+    struct Words {
+        int w[NW];
+    };
+    const Words in = __builtin_bit_cast(Words, v);
+    Words out;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            cx<double> t = cmul<double>(a.alpha_re, a.alpha_im, to<double>(y[c]));
-            if (fl & F_BETA) {
-                const cx<double> xs = cmul<double>(a.beta_re, a.beta_im, xi.get(c));
-                t.re += xs.re;
-                t.im += xs.im;
+    for (int i = 0; i < NW; ++i) out.w[i] = __builtin_amdgcn_update_dpp(0, in.w[i], 0x150 + E, 0xF, 0xF, false);
+    return __builtin_bit_cast(T, out);
+}
+
+template <class T>
+__device__ __forceinline__ void row_bcast16(T mine, T (&v)[16]) {
+    v[0] = row_bcast<0>(mine);
+    v[1] = row_bcast<1>(mine);
+    v[2] = row_bcast<2>(mine);
+    v[3] = row_bcast<3>(mine);
+    v[4] = row_bcast<4>(mine);
+    v[5] = row_bcast<5>(mine);
+    v[6] = row_bcast<6>(mine);
+    v[7] = row_bcast<7>(mine);
+    v[8] = row_bcast<8>(mine);
+    v[9] = row_bcast<9>(mine);
+    v[10] = row_bcast<10>(mine);
+    v[11] = row_bcast<11>(mine);
+    v[12] = row_bcast<12>(mine);
+    v[13] = row_bcast<13>(mine);
+    v[14] = row_bcast<14>(mine);
+    v[15] = row_bcast<15>(mine);
+}
+
+// 4x4 block-sparse (BSR4) variant of step_kernel.  The matrix is stored as
+// block rows of four consecutive rows: ptr[q]..ptr[q+1] index the nonzero 4x4
+// blocks of block row q, idx their block columns, val 16 row-major values per
+// block.  Each lane owns CPL columns of all four rows of a block row, so one
+// gather of the four contiguous rows X[4j..4j+3] feeds 16 products per
+// column.  When a block row's lane group spans whole 16-lane rows
+// (lanes per block row >= 16), lane l loads only value l%16 of the block and
+// the row shares them by DPP broadcasts, so the 64 values cost one narrow
+// load per lane instead of 16 values per lane through the vector cache.
+// The epilogue is the CSR kernel's, row by row.  n must be a multiple of 4.
+template <class MT>
+struct alignas(16) Block16 {
+    MT v[16];
+};
+
+template <class VT, class MR, bool MC, class AT, int CPL, bool DPP>
+__global__ __launch_bounds__(256) void bsr4_step_kernel(const StepArgs a) {
+    using V = cx<VT>;
+    using MV = MatVal<MR, MC>;
+    using MT = typename MV::type;
+    using PK = Pack<VT, CPL>;
+    using BK = Block16<MT>;
+    if (a.gate && *a.gate == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lpr = 1 << a.lpr_log2;
+    const int qpw = 64 >> a.lpr_log2;           // block rows per wavefront
+    const int qpb = 4 * qpw;                    // block rows per workgroup pass
+    const int qloc = wave * qpw + (lane >> a.lpr_log2);
+    const int cgrp = lane & (lpr - 1);
+    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+    constexpr bool dpp = DPP;  // requires lanes per block row >= 16
+
+    const V *X = static_cast<const V *>(a.X);
+    const V *Z = static_cast<const V *>(a.Z);
+    const V *P = static_cast<const V *>(a.P);
+    const V *W = static_cast<const V *>(a.W);
+    const BK *val = static_cast<const BK *>(a.val);
+    const MT *vals = static_cast<const MT *>(a.val);
+    const uint32_t fl = a.flags;
+    const bool need_xi = (fl & F_BETA) || (fl & F_MOM2);
+
+    cx<double> mom[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) mom[c] = {0.0, 0.0};
+
+    const int64_t nq = a.n >> 2;
+    const int64_t nblk = (nq + qpb - 1) / qpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t q = blk * qpb + qloc;
+        // The whole lane group of a block row leaves together; on the DPP path
+        // lanes past the last column stay to share their block values.
+        if (q >= nq || (!dpp && !colok)) continue;
+        const int64_t row0 = q * 4;
+        cx<AT> y[4][CPL];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) y[r][c] = {AT(0), AT(0)};
+        if (fl & F_SPMM) {
+            const int64_t k0 = a.ptr[q], e = a.ptr[q + 1];
+            if constexpr (dpp) {
+                for (int64_t k = k0; k < e; ++k) {
+                    const int64_t j = int64_t(a.idx[k]) * 4;
+                    PK x0[4];
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc) {
+                        if (colok)
+                            x0[cc].load(X + (j + cc) * a.ldx + col);
+                        else
+                            x0[cc].v = {};
+                    }
+                    MT v[16];
+                    row_bcast16(vals[k * 16 + (lane & 15)], v);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                            for (int c = 0; c < CPL; ++c) MV::fma(y[r][c], v[r * 4 + cc], x0[cc].template geta<AT>(c));
+                }
+            } else {
+                for (int64_t k = k0; k < e; ++k) {
+                    const int64_t j = int64_t(a.idx[k]) * 4;
+                    PK x0[4];
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc) x0[cc].load(X + (j + cc) * a.ldx + col);
+                    const BK v0 = val[k];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                            for (int c = 0; c < CPL; ++c)
+                                MV::fma(y[r][c], v0.v[r * 4 + cc], x0[cc].template geta<AT>(c));
+                }
             }
-            if (fl & F_P) {
-                const cx<double> p = pv.get(c);
-                t.re = __builtin_fma(a.gamma, p.re, t.re);
-                t.im = __builtin_fma(a.gamma, p.im, t.im);
-            }
-            ts.set(c, t);
-            if (fl & F_ACC) {
-                cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
-                const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t);
-                acc.re += add.re;
-                acc.im += add.im;
-                ACC[row * a.lda + col + c] = to<VT>(acc);
+        } else if (colok) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                PK zv;
+                zv.load(Z + (row0 + r) * a.ldz + col);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) y[r][c] = zv.template geta<AT>(c);
             }
         }
-        if (fl & F_STORE) ts.store_nt(OUT + row * a.ldo + col);
-        if (fl & F_MOM2) {
+        if (!colok) continue;
+        // Row-local streams after the gathers: fewer live registers across
+        // the block loop (higher occupancy hides the gather latency).
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                const cx<double> xv = xi.get(c), tv = ts.get(c);
-                mom[c].re = __builtin_fma(xv.re, xv.re, __builtin_fma(xv.im, xv.im, mom[c].re));
-                mom[c].im = __builtin_fma(xv.re, tv.re, __builtin_fma(xv.im, tv.im, mom[c].im));
-            }
-        }
-        if (fl & F_MOM) {
-            // Products of the stored (rounded) values, accumulated in FP64;
-            // FP32 x FP32 products are exact in FP64.
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                const cx<double> wd = wv.get(c), tv = ts.get(c);
-                mom[c].re = __builtin_fma(wd.re, tv.re, mom[c].re);
-                mom[c].re = __builtin_fma(wd.im, tv.im, mom[c].re);
-                mom[c].im = __builtin_fma(wd.re, tv.im, mom[c].im);
-                mom[c].im = __builtin_fma(-wd.im, tv.re, mom[c].im);
-            }
+        for (int r = 0; r < 4; ++r) {
+            PK xi, pv, wv;
+            if (need_xi) xi.load(X + (row0 + r) * a.ldx + col);
+            if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
+            if (fl & F_MOM) wv.load_nt(W + (row0 + r) * a.ldw + col);
+            step_epilogue<VT, AT, CPL>(a, fl, row0 + r, col, y[r], xi, pv, wv, mom);
         }
     }
 
     if (!(fl & (F_MOM | F_MOM2))) return;
-    // Rows sharing a column: lanes differing in bits >= lpr_log2.
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-        for (int off = lpr; off < 64; off <<= 1) {
-            mom[c].re += __shfl_xor(mom[c].re, off);
-            mom[c].im += __shfl_xor(mom[c].im, off);
-        }
-    }
-    __shared__ cx<double> red[4][64 * CPL];
-    if (lane < lpr) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = mom[c];
-    }
-    __syncthreads();
-    if (wave == 0 && lane < lpr && colok) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            if (col + c >= a.w) continue;
-            cx<double> s = red[0][lane * CPL + c];
-            for (int q = 1; q < 4; ++q) {
-                s.re += red[q][lane * CPL + c].re;
-                s.im += red[q][lane * CPL + c].im;
-            }
-            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
-        }
-    }
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
 }
 
 // out[s][c] = sum_g part[s][g][c] in a fixed tree order (deterministic).
@@ -413,6 +594,31 @@ __global__ __launch_bounds__(256) void congruence_scale_kernel(void *offv, void 
             dg[r].im = R(double(dg[r].im) * s);
         } else {
             dg[r] = R(double(dg[r]) * s);
+        }
+    }
+}
+
+// BSR4 form of the congruence scaling: one thread per stored block entry.
+template <class R, bool MC>
+__global__ __launch_bounds__(256) void bsr4_congruence_kernel(void *valv, const int32_t *ptr, const int32_t *idx,
+                                                              const double *d, int64_t nq, int64_t nnzb) {
+    using MT = typename MatVal<R, MC>::type;
+    MT *val = static_cast<MT *>(valv);
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nnzb * 16; e += int64_t(gridDim.x) * 256) {
+        const int64_t k = e >> 4;
+        const int r = int(e & 15) >> 2, cc = int(e & 3);
+        // block row of entry k: largest q with ptr[q] <= k (binary search)
+        int64_t lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (ptr[mid] <= k) lo = mid; else hi = mid;
+        }
+        const double s = 1.0 / sqrt(d[lo * 4 + r] * d[int64_t(idx[k]) * 4 + cc]);
+        if constexpr (MC) {
+            val[e].re = R(double(val[e].re) * s);
+            val[e].im = R(double(val[e].im) * s);
+        } else {
+            val[e] = R(double(val[e]) * s);
         }
     }
 }

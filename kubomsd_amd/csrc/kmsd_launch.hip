@@ -86,9 +86,58 @@ T *upload(kmsd_ctx *c, kmsd_matrix *m, const std::vector<T> &h) {
 }
 }  // namespace
 
+namespace {
+
+// Blocks a validated CSR matrix into 4x4 blocks.  Returns false (and builds
+// nothing) when want_auto is set and the blocks would be less than 75% full.
+bool bsr4_upload(kmsd_ctx *c, kmsd_matrix *m, bool want_auto, int64_t n, int64_t nnz, const int64_t *rowptr,
+                 const int64_t *colidx, const double *values, int comps, int off) {
+    const int64_t nq = n / 4;
+    std::vector<int32_t> bptr(nq + 1, 0), bidx;
+    bidx.reserve(size_t(nnz / 8 + 16));
+    std::vector<int32_t> cols;
+    for (int64_t q = 0; q < nq; ++q) {
+        cols.clear();
+        for (int64_t k = rowptr[4 * q]; k < rowptr[4 * q + 4]; ++k) cols.push_back(int32_t(colidx[k] >> 2));
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        bidx.insert(bidx.end(), cols.begin(), cols.end());
+        bptr[q + 1] = int32_t(bidx.size());
+    }
+    const int64_t nnzb = int64_t(bidx.size());
+    if (want_auto && 4 * nnz < 3 * 16 * nnzb) return false;
+    if (16 * nnzb >= (int64_t(1) << 31)) throw Error(KMSD_EINVAL, "matrix exceeds 32-bit device indexing");
+    std::vector<double> v64(size_t(std::max<int64_t>(1, 16 * nnzb)) * comps, 0.0);
+    for (int64_t q = 0; q < nq; ++q) {
+        const int32_t *b0 = bidx.data() + bptr[q], *b1 = bidx.data() + bptr[q + 1];
+        for (int r = 0; r < 4; ++r)
+            for (int64_t k = rowptr[4 * q + r]; k < rowptr[4 * q + r + 1]; ++k) {
+                const int32_t jb = int32_t(colidx[k] >> 2);
+                const int64_t pos = bptr[q] + (std::lower_bound(b0, b1, jb) - b0);
+                const size_t e = size_t(pos * 16 + r * 4 + (colidx[k] & 3)) * comps;
+                for (int t = 0; t < comps; ++t) v64[e + t] += values[2 * k + off + t];
+            }
+    }
+    std::vector<float> v32(v64.size());
+    for (size_t e = 0; e < v64.size(); ++e) v32[e] = float(v64[e]);
+    m->fmt = KMSD_BSR4;
+    m->noff = 16 * nnzb;
+    m->ell = 0;
+    m->has_diag = false;
+    m->identity = false;
+    m->ptr = upload(c, m, bptr);
+    m->idx = upload(c, m, bidx);
+    m->val64 = upload(c, m, v64);
+    m->val32 = upload(c, m, v32);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return true;
+}
+
+}  // namespace
+
 void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
                    const int64_t *colidx, const double *values) {
-    if (fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "unknown matrix format");
+    if (fmt != KMSD_CSR && fmt != KMSD_BSR4 && fmt != KMSD_AUTO) throw Error(KMSD_EINVAL, "unknown matrix format");
     if (n < 1 || nnz < 0 || !rowptr || (nnz && (!colidx || !values)))
         throw Error(KMSD_EINVAL, "invalid CSR dimensions/threads");
     if (n >= (int64_t(1) << 31) || nnz >= (int64_t(1) << 31))
@@ -104,12 +153,18 @@ void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz,
         any_re |= re != 0.0;
         any_im |= im != 0.0;
     }
-    m->fmt = fmt;
+    m->fmt = KMSD_CSR;
     m->n = n;
     m->nnz = nnz;
     m->vkind = any_im ? (any_re ? VK_COMPLEX : VK_IMAG) : VK_REAL;
     const int comps = m->vkind == VK_COMPLEX ? 2 : 1;
     const int off = m->vkind == VK_IMAG ? 1 : 0;
+    if (fmt == KMSD_BSR4 && n % 4) throw Error(KMSD_EINVAL, "BSR4 storage needs a multiple of 4 rows");
+    // A 75%-full 4x4 block holds 12 entries, three per row: below 3 entries
+    // per row on average AUTO cannot choose BSR4.
+    if ((fmt == KMSD_BSR4 || (fmt == KMSD_AUTO && n % 4 == 0 && nnz >= 3 * n)) &&
+        bsr4_upload(c, m, fmt == KMSD_AUTO, n, nnz, rowptr, colidx, values, comps, off))
+        return;
     // Split: dense diagonal (duplicates summed) + off-diagonal entries.
     std::vector<double> dg(size_t(n) * comps, 0.0);
     std::vector<int64_t> cnt(n, 0);
@@ -223,17 +278,22 @@ int cpl_limit(const Step &s, int dtype) {
 }
 
 template <class VT, class MR, bool MC, class AT, int CPL>
-void go(kmsd_ctx *c, const Shape &sh, const StepArgs &a) {
-    hipLaunchKernelGGL((step_kernel<VT, MR, MC, AT, CPL>), dim3(unsigned(sh.gx), unsigned(sh.gy)), dim3(256), 0,
-                       c->stream, a);
+void go(kmsd_ctx *c, const Shape &sh, const StepArgs &a, bool bsr) {
+    const dim3 grid(unsigned(sh.gx), unsigned(sh.gy));
+    if (bsr && a.lpr_log2 >= 4)
+        hipLaunchKernelGGL((bsr4_step_kernel<VT, MR, MC, AT, CPL, true>), grid, dim3(256), 0, c->stream, a);
+    else if (bsr)
+        hipLaunchKernelGGL((bsr4_step_kernel<VT, MR, MC, AT, CPL, false>), grid, dim3(256), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((step_kernel<VT, MR, MC, AT, CPL>), grid, dim3(256), 0, c->stream, a);
 }
 
 template <class VT, class MR, class AT, int CPL>
-void go_vk(kmsd_ctx *c, const Shape &sh, const StepArgs &a, bool complex_values) {
+void go_vk(kmsd_ctx *c, const Shape &sh, const StepArgs &a, bool complex_values, bool bsr) {
     if (complex_values)
-        go<VT, MR, true, AT, CPL>(c, sh, a);
+        go<VT, MR, true, AT, CPL>(c, sh, a, bsr);
     else
-        go<VT, MR, false, AT, CPL>(c, sh, a);
+        go<VT, MR, false, AT, CPL>(c, sh, a, bsr);
 }
 
 int64_t blocks_for(int64_t total) { return std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192)); }
@@ -263,15 +323,15 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     uint32_t fl = 0;
     Cplx alpha = s.alpha;
     const kmsd_matrix *A = s.A;
+    const bool bsr = A && A->fmt == KMSD_BSR4;
+    const bool f32 = A && dtype == KMSD_C64 && (c->fp32_math || s.f32_matrix);
     if (A) {
-        if (A->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "block-sparse products use the BSR path");
         fl |= F_SPMM;
         a.ptr = A->ptr;
         a.idx = A->idx;
-        a.ell = A->ell;
-        if (A->ell) fl |= F_ELL;
-        if (A->has_diag) fl |= F_DIAG;
-        const bool f32 = dtype == KMSD_C64 && c->fp32_math;
+        a.ell = bsr ? 0 : A->ell;
+        if (a.ell) fl |= F_ELL;
+        if (!bsr && A->has_diag) fl |= F_DIAG;
         a.val = f32 ? A->val32 : A->val64;
         a.dval = f32 ? A->dval32 : A->dval64;
         if (A->vkind == VK_IMAG) alpha = Cplx{-s.alpha.im, s.alpha.re};  // A = i * stored
@@ -310,23 +370,30 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     const bool cv = A && A->vkind == VK_COMPLEX;
     if (dtype == KMSD_C128) {
         if (sh.cpl == 2)
-            go_vk<double, double, double, 2>(c, sh, a, cv);
+            go_vk<double, double, double, 2>(c, sh, a, cv, bsr);
         else
-            go_vk<double, double, double, 1>(c, sh, a, cv);
+            go_vk<double, double, double, 1>(c, sh, a, cv, bsr);
     } else if (c->fp32_math) {
         if (sh.cpl == 4)
-            go_vk<float, float, float, 4>(c, sh, a, cv);
+            go_vk<float, float, float, 4>(c, sh, a, cv, bsr);
         else if (sh.cpl == 2)
-            go_vk<float, float, float, 2>(c, sh, a, cv);
+            go_vk<float, float, float, 2>(c, sh, a, cv, bsr);
         else
-            go_vk<float, float, float, 1>(c, sh, a, cv);
+            go_vk<float, float, float, 1>(c, sh, a, cv, bsr);
+    } else if (f32) {
+        if (sh.cpl == 4)
+            go_vk<float, float, double, 4>(c, sh, a, cv, bsr);
+        else if (sh.cpl == 2)
+            go_vk<float, float, double, 2>(c, sh, a, cv, bsr);
+        else
+            go_vk<float, float, double, 1>(c, sh, a, cv, bsr);
     } else {
         if (sh.cpl == 4)
-            go_vk<float, double, double, 4>(c, sh, a, cv);
+            go_vk<float, double, double, 4>(c, sh, a, cv, bsr);
         else if (sh.cpl == 2)
-            go_vk<float, double, double, 2>(c, sh, a, cv);
+            go_vk<float, double, double, 2>(c, sh, a, cv, bsr);
         else
-            go_vk<float, double, double, 1>(c, sh, a, cv);
+            go_vk<float, double, double, 1>(c, sh, a, cv, bsr);
     }
     HIPCHECK(hipGetLastError());
     return sh.gx;
@@ -424,6 +491,29 @@ kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const dou
             return d;
         };
         const size_t nv = size_t(std::max<int64_t>(1, S->noff)) * comps;
+        if (S->fmt == KMSD_BSR4) {
+            const int64_t nq = rows / 4, nnzb = S->noff / 16;
+            W->ptr = static_cast<int32_t *>(dup(S->ptr, sizeof(int32_t) * size_t(nq + 1)));
+            W->idx = static_cast<int32_t *>(dup(S->idx, sizeof(int32_t) * size_t(std::max<int64_t>(1, nnzb))));
+            W->val64 = dup(S->val64, sizeof(double) * nv);
+            W->val32 = dup(S->val32, sizeof(float) * nv);
+            const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((S->noff + 255) / 256, 8192)));
+            if (S->vkind == VK_COMPLEX) {
+                hipLaunchKernelGGL((bsr4_congruence_kernel<double, true>), dim3(g), dim3(256), 0, c->stream, W->val64,
+                                   W->ptr, W->idx, diag, nq, nnzb);
+                hipLaunchKernelGGL((bsr4_congruence_kernel<float, true>), dim3(g), dim3(256), 0, c->stream, W->val32,
+                                   W->ptr, W->idx, diag, nq, nnzb);
+            } else {
+                hipLaunchKernelGGL((bsr4_congruence_kernel<double, false>), dim3(g), dim3(256), 0, c->stream,
+                                   W->val64, W->ptr, W->idx, diag, nq, nnzb);
+                hipLaunchKernelGGL((bsr4_congruence_kernel<float, false>), dim3(g), dim3(256), 0, c->stream,
+                                   W->val32, W->ptr, W->idx, diag, nq, nnzb);
+            }
+            HIPCHECK(hipGetLastError());
+            W->identity = false;
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            return W;
+        }
         if (S->ptr) W->ptr = static_cast<int32_t *>(dup(S->ptr, sizeof(int32_t) * size_t(rows + 1)));
         W->idx = static_cast<int32_t *>(dup(S->idx, sizeof(int32_t) * size_t(std::max<int64_t>(1, S->noff))));
         W->val64 = dup(S->val64, sizeof(double) * nv);

@@ -16,7 +16,7 @@ extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *dia
     kmsd_metric *m = new kmsd_metric();
     m->ctx = c;
     const int rc = guard(c, [&] {
-        if (!S || !diag || S->fmt != KMSD_CSR) throw Error(KMSD_EINVAL, "invalid metric diagonal/threads");
+        if (!S || !diag) throw Error(KMSD_EINVAL, "invalid metric diagonal/threads");
         const int64_t n = S->n;
         m->S = S;
         m->n = n;

@@ -101,6 +101,37 @@ struct Solver {
         HIPCHECK(hipGetLastError());
     }
 
+    // Fused iteration stages: diagonal preconditioner in the row kernel,
+    // uniform 4x4 blocks in the quad kernel.
+    void fused(int stage) {
+        if (m->nblocks == 0) {
+            row(stage, nullptr, 0, nullptr, 0, true);
+            return;
+        }
+        PcgArgs a{};
+        a.Y = Y;
+        a.R = R;
+        a.P = P;
+        a.SP = SP;
+        a.fac = m->factors;
+        a.cs = cs;
+        a.fl = fl;
+        a.part = part;
+        a.n = n;
+        a.w = w;
+        a.lpr_log2 = rs.lpr_log2;
+        a.stage = stage;
+        a.require_active = 1;
+        const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
+        if (dtype == KMSD_C128)
+            hipLaunchKernelGGL((pcg_quad_kernel<double, 1>), grid, dim3(256), 0, c->stream, a);
+        else if (rs.cpl == 2)
+            hipLaunchKernelGGL((pcg_quad_kernel<float, 2>), grid, dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((pcg_quad_kernel<float, 1>), grid, dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+    }
+
     void col(int stage, const double *p, int64_t groups, double rtol, double atol, bool refresh = false,
              const int32_t *gate = nullptr) {
         ColArgs a{};
@@ -148,6 +179,9 @@ struct Solver {
         s.n = n;
         s.w = w;
         s.gate = gate;
+        // complex64 solves are the inner solves of an FP64 refinement (or a
+        // caller's FP32 request): FP32 matrix values halve the S stream.
+        s.f32_matrix = dtype == KMSD_C64;
         g2 = launch_step(c, dtype, s);
     }
 
@@ -245,6 +279,10 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     const int32_t *any = &S.fl->any_active;
     const int32_t *cand = &S.fl->need_refresh;
     const int poll = std::max(1, c->pcg_poll);
+    // Fixed residual policy with a diagonal or uniform 4x4 preconditioner:
+    // iterations without a true-residual refresh run fused (Y, R update with
+    // z = M^-1 R and both reductions in one pass, P update recomputing z).
+    const bool fusable = !m->adaptive && (m->nblocks == 0 || m->uniform4);
     for (int k = 0; k < maxiter; ++k) {
         if (k % poll == 0) {
             const PcgFlags h = S.read_flags();
@@ -254,6 +292,13 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
         // sp = S p ; curvature <p, sp>
         S.spmv(S.P, S.SP, nullptr, 1.0, 0.0, S.P, any);
         S.col(PC_ALPHA, S.part2, S.g2, rtol, atol, false, any);
+        if (fusable && !refresh) {
+            S.fused(PS_UPD_PRE);
+            S.clear_flag(offsetof(PcgFlags, any_active), sizeof(int32_t));
+            S.col(PC_RN_BETA, S.part, S.rs.gx, rtol, atol);
+            S.fused(PS_PUPD_PRE);
+            continue;
+        }
         // y += alpha p ; r -= alpha sp ; |r|
         S.row(PS_UPDATE, nullptr, 0, nullptr, 0, true);
         S.clear_flag(offsetof(PcgFlags, need_refresh), sizeof(int32_t));

@@ -32,6 +32,10 @@ enum : int32_t {
     PS_PUPDATE = 7,  // P = active ? Z + (restart ? 0 : beta P) : 0
     PS_TRUEGAP = 8,  // partial |T|^2 (re) and |T-R|^2 (im)
     PS_RESCALE = 9,  // OUT = Y * safe ; flags nonfinite columns
+    // Fused iteration (fixed residual policy, no refresh this iteration);
+    // z = M^-1 R is recomputed where needed instead of stored.
+    PS_UPD_PRE = 10,   // Y += alpha P, R -= alpha SP ; partial |R|^2 (re), Re<R,z> (im)
+    PS_PUPD_PRE = 11,  // P = active ? z + (restart ? 0 : beta P) : 0
 };
 
 struct PcgArgs {
@@ -40,11 +44,12 @@ struct PcgArgs {
     const void *SP, *T;
     void *OUT;
     const double *diag;
+    const double *fac;  // uniform 4x4 block factors, 16 complex entries per block (quad kernel)
     ColState *cs;
     PcgFlags *fl;
     double *part;
     int64_t n, ldb, ldo;  // internal blocks have ld = w
-    int32_t w, lpr_log2, stage, require_active;
+    int32_t w, lpr_log2, stage, require_active;  // require_active: skip after breakdown or with no active column
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
 
@@ -78,8 +83,8 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
             act[c] = s.active;
             rst[c] = s.restart;
             cf[c] = (a.stage == PS_SCALE) ? 1.0 / s.safe
-                    : (a.stage == PS_UPDATE) ? s.alpha
-                    : (a.stage == PS_PUPDATE) ? s.beta
+                    : (a.stage == PS_UPDATE || a.stage == PS_UPD_PRE) ? s.alpha
+                    : (a.stage == PS_PUPDATE || a.stage == PS_PUPD_PRE) ? s.beta
                     : (a.stage == PS_RESCALE) ? s.safe
                                               : 0.0;
         }
@@ -167,6 +172,54 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
                 }
                 y.store(Y + o);
                 r.store(R + o);
+                break;
+            }
+            case PS_UPD_PRE: {
+                PK p, sp, y, r;
+                p.load(P + o);
+                sp.load(SP + o);
+                y.load(Y + o);
+                r.load(R + o);
+                const double d = a.diag[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> pv = p.get(c), sv = sp.get(c);
+                    cx<double> yv = y.get(c), rv = r.get(c);
+                    yv.re += pv.re * cf[c];
+                    yv.im += pv.im * cf[c];
+                    rv.re -= sv.re * cf[c];
+                    rv.im -= sv.im * cf[c];
+                    y.set(c, yv);
+                    r.set(c, rv);
+                    const cx<double> rs = r.get(c);
+                    const double rr = rs.re * rs.re + rs.im * rs.im;
+                    acc_re[c] += rr;
+                    acc_im[c] += rr / d;
+                }
+                y.store(Y + o);
+                r.store(R + o);
+                break;
+            }
+            case PS_PUPD_PRE: {
+                PK r, p;
+                r.load(R + o);
+                p.load(P + o);
+                const double d = a.diag[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<double> v{0.0, 0.0};
+                    if (act[c]) {
+                        const cx<double> rv = r.get(c);
+                        v = cx<double>{rv.re / d, rv.im / d};
+                        if (!rst[c]) {
+                            const cx<double> pv = p.get(c);
+                            v.re += pv.re * cf[c];
+                            v.im += pv.im * cf[c];
+                        }
+                    }
+                    p.set(c, v);
+                }
+                p.store(P + o);
                 break;
             }
             case PS_RESTART: {
@@ -261,7 +314,7 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
     }
 
     const bool reduces = a.stage == PS_ABSMAX || a.stage == PS_SCALE || a.stage == PS_UPDATE ||
-                         a.stage == PS_DIAG || a.stage == PS_DOT || a.stage == PS_TRUEGAP;
+                         a.stage == PS_DIAG || a.stage == PS_DOT || a.stage == PS_TRUEGAP || a.stage == PS_UPD_PRE;
     if (a.stage == PS_RESCALE) {
         // nonfinite columns: flag per column (vector atomics on global memory)
 #pragma unroll
@@ -313,6 +366,7 @@ enum : int32_t {
     PC_BETA = 6,    // beta, rz, activity count
     PC_FINAL = 7,   // final true residual -> report fields
     PC_CLEAR = 8,   // zero flag block + restart marks
+    PC_RN_BETA = 9, // fused iteration: rn, iterations, activity (re = |r|^2), then beta, rz (im = Re<r,z>)
 };
 
 struct ColArgs {
@@ -417,6 +471,19 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             if (s.active) atomicAdd(&f->any_active, 1);
             break;
         }
+        case PC_RN_BETA: {
+            // PC_RN (fixed policy, no refresh) followed by PC_BETA; runs on
+            // columns that are inactive too, where it only rewrites values
+            // that inactive columns never read again.
+            s.rn = sqrt(tre);
+            s.iters += s.active;
+            s.restart = 0;
+            s.active = s.active && s.rn > s.thr;
+            s.beta = s.active ? tim / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
+            s.rz = tim;
+            if (s.active) atomicAdd(&f->any_active, 1);
+            break;
+        }
         case PC_FINAL: {
             s.tn = sqrt(tre);
             break;
@@ -444,6 +511,177 @@ __global__ __launch_bounds__(256) void scaled_convert_kernel(const void *Xv, int
             o.re += x.re * cs[c].safe;
             o.im += x.im * cs[c].safe;
             O[r * ldo + c] = to<VB>(o);
+        }
+    }
+}
+
+// z = L^-H L^-1 r for one 4x4 block (L lower triangular, complex, row-major),
+// the arithmetic of block_precond_kernel for m = 4.
+__device__ __forceinline__ void blk4_solve(const cx<double> *L, cx<double> (&v)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        cx<double> s = v[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            const cx<double> l = L[i * 4 + k];
+            s.re -= l.re * v[k].re - l.im * v[k].im;
+            s.im -= l.re * v[k].im + l.im * v[k].re;
+        }
+        const cx<double> d = L[i * 4 + i];
+        const double den = d.re * d.re + d.im * d.im;
+        v[i] = {(s.re * d.re + s.im * d.im) / den, (s.im * d.re - s.re * d.im) / den};
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        cx<double> s = v[i];
+#pragma unroll
+        for (int k = i + 1; k < 4; ++k) {
+            const cx<double> l = L[k * 4 + i];
+            s.re -= l.re * v[k].re + l.im * v[k].im;
+            s.im -= l.re * v[k].im - l.im * v[k].re;
+        }
+        const cx<double> d = L[i * 4 + i];
+        const double den = d.re * d.re + d.im * d.im;
+        v[i] = {(s.re * d.re - s.im * d.im) / den, (s.im * d.re + s.re * d.im) / den};
+    }
+}
+
+// Fused iteration stages PS_UPD_PRE / PS_PUPD_PRE for a uniform 4x4
+// block-Jacobi preconditioner: each lane owns CPL columns of the four rows of
+// one block, so z = L^-H L^-1 r is formed in registers.  Partials as in
+// pcg_row_kernel ([gridDim.x][w], re = |R|^2, im = Re<R,z>).
+template <class VT, int CPL>
+__global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
+    using V = cx<VT>;
+    using PK = Pack<VT, CPL>;
+    if (a.require_active && (a.fl->broke || !a.fl->any_active)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lpr = 1 << a.lpr_log2;
+    const int qpw = 64 >> a.lpr_log2, qpb = 4 * qpw;
+    const int qloc = wave * qpw + (lane >> a.lpr_log2);
+    const int cgrp = lane & (lpr - 1);
+    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+    const int64_t w = a.w;
+    const bool upd = a.stage == PS_UPD_PRE;
+    double acc_re[CPL], acc_im[CPL], cf[CPL];
+    int32_t act[CPL], rst[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        acc_re[c] = acc_im[c] = cf[c] = 0.0;
+        act[c] = rst[c] = 0;
+        if (colok) {
+            const ColState &s = a.cs[col + c];
+            act[c] = s.active;
+            rst[c] = s.restart;
+            cf[c] = upd ? s.alpha : s.beta;
+        }
+    }
+    const int64_t nq = a.n >> 2;
+    const int64_t nblk = (nq + qpb - 1) / qpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+    V *Y = static_cast<V *>(a.Y);
+    V *R = static_cast<V *>(a.R);
+    V *P = static_cast<V *>(a.P);
+    const V *SP = static_cast<const V *>(a.SP);
+    const cx<double> *F = reinterpret_cast<const cx<double> *>(a.fac);
+
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t q = blk * qpb + qloc;
+        if (q >= nq || !colok) continue;
+        const int64_t o0 = q * 4 * w + col;
+        PK r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i].load(R + o0 + i * w);
+        if (upd) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                PK p, sp, y;
+                p.load(P + o0 + i * w);
+                sp.load(SP + o0 + i * w);
+                y.load(Y + o0 + i * w);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> pv = p.get(c), sv = sp.get(c);
+                    cx<double> yv = y.get(c), rv = r[i].get(c);
+                    yv.re += pv.re * cf[c];
+                    yv.im += pv.im * cf[c];
+                    rv.re -= sv.re * cf[c];
+                    rv.im -= sv.im * cf[c];
+                    y.set(c, yv);
+                    r[i].set(c, rv);
+                }
+                y.store(Y + o0 + i * w);
+                r[i].store(R + o0 + i * w);
+            }
+        }
+        const cx<double> *L = F + q * 16;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            cx<double> v[4], rv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = rv[i] = r[i].get(c);
+            blk4_solve(L, v);
+            if (upd) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    acc_re[c] += rv[i].re * rv[i].re + rv[i].im * rv[i].im;
+                    acc_im[c] += rv[i].re * v[i].re + rv[i].im * v[i].im;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) r[i].set(c, v[i]);  // r now holds z
+            }
+        }
+        if (!upd) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                PK p;
+                p.load(P + o0 + i * w);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<double> v{0.0, 0.0};
+                    if (act[c]) {
+                        v = r[i].get(c);
+                        if (!rst[c]) {
+                            const cx<double> pv = p.get(c);
+                            v.re += pv.re * cf[c];
+                            v.im += pv.im * cf[c];
+                        }
+                    }
+                    p.set(c, v);
+                }
+                p.store(P + o0 + i * w);
+            }
+        }
+    }
+    if (!upd) return;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        for (int off = lpr; off < 64; off <<= 1) {
+            acc_re[c] += __shfl_xor(acc_re[c], off);
+            acc_im[c] += __shfl_xor(acc_im[c], off);
+        }
+    }
+    __shared__ cx<double> red[4][64 * CPL];
+    if (lane < lpr) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = cx<double>{acc_re[c], acc_im[c]};
+    }
+    __syncthreads();
+    if (wave == 0 && lane < lpr && colok) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (col + c >= a.w) continue;
+            cx<double> s = red[0][lane * CPL + c];
+            for (int qq = 1; qq < 4; ++qq) {
+                s.re += red[qq][lane * CPL + c].re;
+                s.im += red[qq][lane * CPL + c].im;
+            }
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
         }
     }
 }
