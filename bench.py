@@ -333,7 +333,7 @@ def double_moments(args):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": tf32, "peak": MFMA_PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                      "frac": tf32 / MFMA_PEAK_TFLOPS["fp32"], "traffic": None,
-                     "algorithmic_flops_per_launch": flops, "kernel": "mu_mfma_f32_kernel (+ split sum)"},
+                     "algorithmic_flops_per_launch": flops, "kernel": "mu_f32_kernel (+ split sum)"},
         "bf16x3": {"achieved_TFLOPs": tfb, "mfma_executed_TFLOPs": 3 * tfb,
                    "frac_of_bf16_peak": 3 * tfb / MFMA_PEAK_TFLOPS["bf16"], "ms_per_table": times["bf16x3_ms"] / ns},
         "pipeline_ms_per_vector": {k: v / ns for k, v in times.items()},

@@ -1,4 +1,25 @@
 // MFMA kernels for square double-moment tables (see kmsd_mu_mfma.hpp).
+//
+// Workgroup tile 128 (j) x 128 (q) of the complex table, four wavefronts in
+// a 2 x 2 arrangement, each owning 64 x 64 = 2 x 2 blocks of 32 x 32 MFMA
+// outputs for the real and the imaginary part (128 accumulator registers per
+// lane).  K (the N orbitals) is walked in slabs of 16 complex entries staged
+// through LDS; the next slab's global loads are issued before the current
+// slab's MFMAs (register prefetch).  The N range is split across workgroups
+// into FP64 partial tables that a second kernel sums in a fixed order.
+//
+// Operand layout: the sum over K is order-free, so each lane half h takes
+// its own complex entries (FP32: entries 4u+2h, 4u+2h+1 of a group of four;
+// bf16: entries 8u+4h .. 8u+4h+3 of a group of eight) and reads them as one
+// 16-byte LDS access.  With (Lr, Li), (Rr, Ri) the parts of one entry:
+//   Re C += Lr Rr + Li Ri        Im C += Lr Ri - Li Rr
+// FP32: four v_mfma_f32_32x32x2_f32 per entry pair (products of single
+// parts); bf16x3: the interleaved (re, im) pairs are the K slots of
+// v_mfma_f32_32x32x16_bf16, the imaginary product uses (Ri, -Rr) pairs.
+//
+// Workgroups are numbered so that each XCD receives a contiguous range of
+// (split, tile) work items: the 64 workgroups an XCD holds at once share one
+// N range, so their L and R rows come from that XCD's L2.
 #include <algorithm>
 
 #include "kmsd_mu_mfma.hpp"
@@ -8,13 +29,132 @@ namespace kmsd {
 namespace {
 
 typedef float f16x __attribute__((ext_vector_type(16)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
-constexpr int TILE = 64;         // output tile (j and q)
-constexpr int KT = 32;           // complex entries per K slab
-constexpr int KF = 2 * KT;       // floats per row of a slab
-constexpr int FSTRIDE = KF + 1;  // padded LDS row stride (floats)
-constexpr int BSTRIDE = KF + 8;  // padded LDS row stride (bf16 elements)
+constexpr int TILE = 128;  // workgroup tile (j and q)
+constexpr int KT = 16;     // complex entries per slab
+constexpr int LSF = 36;    // FP32 LDS row stride (floats; 16-byte rows)
+constexpr int LSB = 40;    // bf16 LDS row stride (16-bit values; 16-byte rows)
+
+struct Work {
+    int x, y;
+    int64_t z;
+};
+
+__device__ __forceinline__ Work decode(int gx, int gy, int64_t total) {
+    const int64_t id = blockIdx.x;
+    int64_t w = id;
+    if ((total & 7) == 0) w = (id & 7) * (total >> 3) + (id >> 3);
+    const int64_t tiles = int64_t(gx) * gy;
+    Work r;
+    r.z = w / tiles;
+    const int64_t t = w - r.z * tiles;
+    r.x = int(t % gx);
+    r.y = int(t / gx);
+    return r;
+}
+
+// Slab loads: thread t covers tile row t/2 and complex entries
+// [ib + 8(t%2), ib + 8(t%2) + 8) as four float4 (zeros outside the matrix).
+__device__ __forceinline__ void load_slab(const float *M, int64_t n, int rows, int r0, int64_t ib, int64_t iend,
+                                          f4 (&v)[4]) {
+    const int t = threadIdx.x;
+    const int r = r0 + (t >> 1);
+    const int64_t e0 = ib + 8 * (t & 1);
+    const float *p = M + int64_t(r) * 2 * n + 2 * e0;
+    if (r < rows && e0 + 8 <= iend && (n & 1) == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const f4 *>(p + 4 * q);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int64_t e = e0 + (4 * q + s) / 2;
+                v[q][s] = (r < rows && e < iend) ? p[4 * q + s] : 0.0f;
+            }
+    }
+}
+
+__device__ __forceinline__ void store_partials(double *part, int64_t z, int m, int k, int j0, int q0,
+                                               const f16x (&cre)[2][2], const f16x (&cim)[2][2]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int wj = (wave >> 1) * 64, wq = (wave & 1) * 64;
+    cx<double> *P = reinterpret_cast<cx<double> *>(part) + z * int64_t(m) * k;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int j = j0 + wj + 32 * a + 8 * (i / 4) + 4 * half + (i % 4);
+                const int q = q0 + wq + 32 * b + l32;
+                if (j < m && q < k) P[int64_t(j) * k + q] = cx<double>{double(cre[a][b][i]), double(cim[a][b][i])};
+            }
+}
+
+__global__ __launch_bounds__(256, 2) void mu_f32_kernel(const float *L, const float *R, double *part, int64_t n,
+                                                        int m, int k, int gx, int gy, int64_t total,
+                                                        int64_t rows_per_split) {
+    __shared__ float la[TILE][LSF], ra[TILE][LSF];
+    const Work wk = decode(gx, gy, total);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int j0 = wk.x * TILE, q0 = wk.y * TILE;
+    const int wj = (wave >> 1) * 64, wq = (wave & 1) * 64;
+    const int64_t ib0 = wk.z * rows_per_split;
+    const int64_t iend = ib0 + rows_per_split < n ? ib0 + rows_per_split : n;
+    f16x cre[2][2], cim[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) cre[a][b] = cim[a][b] = f16x{};
+    f4 pl[4], pr[4];
+    if (ib0 < iend) {
+        load_slab(L, n, m, j0, ib0, iend, pl);
+        load_slab(R, n, k, q0, ib0, iend, pr);
+    }
+    const int sr = t >> 1, sc = 16 * (t & 1);
+    for (int64_t ib = ib0; ib < iend; ib += KT) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            *reinterpret_cast<f4 *>(&la[sr][sc + 4 * q]) = pl[q];
+            *reinterpret_cast<f4 *>(&ra[sr][sc + 4 * q]) = pr[q];
+        }
+        __syncthreads();
+        if (ib + KT < iend) {
+            load_slab(L, n, m, j0, ib + KT, iend, pl);
+            load_slab(R, n, k, q0, ib + KT, iend, pr);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int f = 8 * u + 4 * half;
+            f4 A[2], B[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) A[a] = *reinterpret_cast<const f4 *>(&la[wj + 32 * a + l32][f]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) B[b] = *reinterpret_cast<const f4 *>(&ra[wq + 32 * b + l32][f]);
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const float lr = A[a][2 * p], li = A[a][2 * p + 1];
+                        const float rr = B[b][2 * p], ri = B[b][2 * p + 1];
+                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(lr, rr, cre[a][b], 0, 0, 0);
+                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(li, ri, cre[a][b], 0, 0, 0);
+                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(lr, ri, cim[a][b], 0, 0, 0);
+                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(li, -rr, cim[a][b], 0, 0, 0);
+                    }
+        }
+    }
+    store_partials(part, wk.z, m, k, j0, q0, cre, cim);
+}
 
 __device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
     const unsigned u = __builtin_bit_cast(unsigned, x);
@@ -24,118 +164,99 @@ __device__ __forceinline__ float bf16_to_float(unsigned short b) {
     return __builtin_bit_cast(float, static_cast<unsigned>(b) << 16);
 }
 
-// Loads the slab rows [r0, r0+64) x complex [ib, ib+KT) of a (rows x n) complex64
-// matrix (as floats) into per-thread registers: thread t covers row t/4 and
-// floats [16*(t%4), 16*(t%4)+16).
-__device__ __forceinline__ void load_slab(const float *M, int64_t n, int rows, int r0, int64_t ib, int64_t iend,
-                                          float (&v)[16]) {
-    const int t = threadIdx.x;
-    const int r = r0 + (t >> 2);
-    const int f0 = (t & 3) * 16;
+// Eight floats -> eight bf16 high parts and eight bf16 residuals, packed.
+__device__ __forceinline__ void split8(f4 a, f4 b, u4 &hi, u4 &lo) {
+    float x[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int64_t ci = ib + (f0 + e) / 2;
-        v[e] = (r < rows && ci < iend) ? M[int64_t(r) * 2 * n + 2 * ci + (e & 1)] : 0.0f;
+    for (int e = 0; e < 4; ++e) {
+        const unsigned short h0 = bf16_bits_rne(x[2 * e]), h1 = bf16_bits_rne(x[2 * e + 1]);
+        const unsigned short l0 = bf16_bits_rne(x[2 * e] - bf16_to_float(h0));
+        const unsigned short l1 = bf16_bits_rne(x[2 * e + 1] - bf16_to_float(h1));
+        hi[e] = unsigned(h0) | (unsigned(h1) << 16);
+        lo[e] = unsigned(l0) | (unsigned(l1) << 16);
     }
 }
 
-__global__ __launch_bounds__(256) void mu_mfma_f32_kernel(const float *L, const float *R, double *part, int64_t n,
-                                                          int m, int k, int64_t rows_per_split) {
-    __shared__ float la[TILE][FSTRIDE], ra[TILE][FSTRIDE];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int j0 = blockIdx.x * TILE, q0 = blockIdx.y * TILE;
-    const int wj = (wave >> 1) * 32, wq = (wave & 1) * 32;
-    const int64_t ib0 = int64_t(blockIdx.z) * rows_per_split;
-    const int64_t iend = ib0 + rows_per_split < n ? ib0 + rows_per_split : n;
-    f16x cre = {}, cim = {};
-    const int half = lane >> 5, l32 = lane & 31;
-    for (int64_t ib = ib0; ib < iend; ib += KT) {
-        float vl[16], vr[16];
-        load_slab(L, n, m, j0, ib, iend, vl);
-        load_slab(R, n, k, q0, ib, iend, vr);
-        __syncthreads();
+// (re, im) bf16 pairs -> (im, -re)
+__device__ __forceinline__ u4 swap_neg(u4 v) {
+    u4 r;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            la[t >> 2][(t & 3) * 16 + e] = vl[e];
-            ra[t >> 2][(t & 3) * 16 + e] = vr[e];
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int s = 0; s < KT; ++s) {
-            const int f = 2 * s + half;
-            const float a = la[wj + l32][f];
-            const float bre = ra[wq + l32][f];
-            const float bim = half ? -ra[wq + l32][f - 1] : ra[wq + l32][f + 1];
-            cre = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bre, cre, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bim, cim, 0, 0, 0);
-        }
-    }
-    cx<double> *P = reinterpret_cast<cx<double> *>(part) + int64_t(blockIdx.z) * m * k;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int j = j0 + wj + 8 * (i / 4) + 4 * half + (i % 4);
-        const int q = q0 + wq + l32;
-        if (j < m && q < k) P[int64_t(j) * k + q] = cx<double>{double(cre[i]), double(cim[i])};
-    }
+    for (int e = 0; e < 4; ++e) r[e] = ((v[e] >> 16) | (v[e] << 16)) ^ 0x80000000u;
+    return r;
 }
 
-__global__ __launch_bounds__(256) void mu_mfma_bf16x3_kernel(const float *L, const float *R, double *part, int64_t n,
-                                                             int m, int k, int64_t rows_per_split) {
-    __shared__ unsigned short lh[TILE][BSTRIDE], ll[TILE][BSTRIDE], rh[TILE][BSTRIDE], rl[TILE][BSTRIDE];
+__global__ __launch_bounds__(256, 2) void mu_bf16x3_kernel(const float *L, const float *R, double *part, int64_t n,
+                                                           int m, int k, int gx, int gy, int64_t total,
+                                                           int64_t rows_per_split) {
+    __shared__ unsigned short lh[TILE][LSB], ll[TILE][LSB], rh[TILE][LSB], rl[TILE][LSB];
+    const Work wk = decode(gx, gy, total);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int j0 = blockIdx.x * TILE, q0 = blockIdx.y * TILE;
-    const int wj = (wave >> 1) * 32, wq = (wave & 1) * 32;
-    const int64_t ib0 = int64_t(blockIdx.z) * rows_per_split;
-    const int64_t iend = ib0 + rows_per_split < n ? ib0 + rows_per_split : n;
-    f16x cre = {}, cim = {};
     const int half = lane >> 5, l32 = lane & 31;
+    const int j0 = wk.x * TILE, q0 = wk.y * TILE;
+    const int wj = (wave >> 1) * 64, wq = (wave & 1) * 64;
+    const int64_t ib0 = wk.z * rows_per_split;
+    const int64_t iend = ib0 + rows_per_split < n ? ib0 + rows_per_split : n;
+    f16x cre[2][2], cim[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) cre[a][b] = cim[a][b] = f16x{};
+    f4 pl[4], pr[4];
+    if (ib0 < iend) {
+        load_slab(L, n, m, j0, ib0, iend, pl);
+        load_slab(R, n, k, q0, ib0, iend, pr);
+    }
+    const int sr = t >> 1, sc = 16 * (t & 1);
     for (int64_t ib = ib0; ib < iend; ib += KT) {
-        float vl[16], vr[16];
-        load_slab(L, n, m, j0, ib, iend, vl);
-        load_slab(R, n, k, q0, ib, iend, vr);
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int col = (t & 3) * 16 + e;
-            const unsigned short h1 = bf16_bits_rne(vl[e]);
-            lh[t >> 2][col] = h1;
-            ll[t >> 2][col] = bf16_bits_rne(vl[e] - bf16_to_float(h1));
-            const unsigned short h2 = bf16_bits_rne(vr[e]);
-            rh[t >> 2][col] = h2;
-            rl[t >> 2][col] = bf16_bits_rne(vr[e] - bf16_to_float(h2));
+        for (int q = 0; q < 2; ++q) {
+            u4 h, l;
+            split8(pl[2 * q], pl[2 * q + 1], h, l);
+            *reinterpret_cast<u4 *>(&lh[sr][sc + 8 * q]) = h;
+            *reinterpret_cast<u4 *>(&ll[sr][sc + 8 * q]) = l;
+            split8(pr[2 * q], pr[2 * q + 1], h, l);
+            *reinterpret_cast<u4 *>(&rh[sr][sc + 8 * q]) = h;
+            *reinterpret_cast<u4 *>(&rl[sr][sc + 8 * q]) = l;
         }
         __syncthreads();
+        if (ib + KT < iend) {
+            load_slab(L, n, m, j0, ib + KT, iend, pl);
+            load_slab(R, n, k, q0, ib + KT, iend, pr);
+        }
 #pragma unroll
-        for (int kk = 0; kk < KF / 16; ++kk) {
-            const int f0 = 16 * kk + 8 * half;
-            bf8 ah, al, bhr, blr, bhi, bli;
+        for (int u = 0; u < 2; ++u) {
+            const int f = 16 * u + 8 * half;
+            u4 Ah[2], Al[2], Bh[2], Bl[2];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                ah[e] = __builtin_bit_cast(__bf16, lh[wj + l32][f0 + e]);
-                al[e] = __builtin_bit_cast(__bf16, ll[wj + l32][f0 + e]);
-                bhr[e] = __builtin_bit_cast(__bf16, rh[wq + l32][f0 + e]);
-                blr[e] = __builtin_bit_cast(__bf16, rl[wq + l32][f0 + e]);
-                // imaginary operand: pairs (re, im) -> (im, -re)
-                const int src = (e & 1) ? e - 1 : e + 1;
-                const unsigned short sgn = (e & 1) ? 0x8000u : 0u;
-                bhi[e] = __builtin_bit_cast(__bf16, static_cast<unsigned short>(rh[wq + l32][f0 + src] ^ sgn));
-                bli[e] = __builtin_bit_cast(__bf16, static_cast<unsigned short>(rl[wq + l32][f0 + src] ^ sgn));
+            for (int a = 0; a < 2; ++a) {
+                Ah[a] = *reinterpret_cast<const u4 *>(&lh[wj + 32 * a + l32][f]);
+                Al[a] = *reinterpret_cast<const u4 *>(&ll[wj + 32 * a + l32][f]);
             }
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhr, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blr, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhr, cre, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bli, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi, cim, 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                Bh[b] = *reinterpret_cast<const u4 *>(&rh[wq + 32 * b + l32][f]);
+                Bl[b] = *reinterpret_cast<const u4 *>(&rl[wq + 32 * b + l32][f]);
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const bf8 bh = __builtin_bit_cast(bf8, Bh[b]), bl = __builtin_bit_cast(bf8, Bl[b]);
+                const bf8 bhi = __builtin_bit_cast(bf8, swap_neg(Bh[b]));
+                const bf8 bli = __builtin_bit_cast(bf8, swap_neg(Bl[b]));
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const bf8 ah = __builtin_bit_cast(bf8, Ah[a]), al = __builtin_bit_cast(bf8, Al[a]);
+                    cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, cre[a][b], 0, 0, 0);
+                    cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, cre[a][b], 0, 0, 0);
+                    cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, cre[a][b], 0, 0, 0);
+                    cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi, cim[a][b], 0, 0, 0);
+                    cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bli, cim[a][b], 0, 0, 0);
+                    cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi, cim[a][b], 0, 0, 0);
+                }
+            }
         }
     }
-    cx<double> *P = reinterpret_cast<cx<double> *>(part) + int64_t(blockIdx.z) * m * k;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int j = j0 + wj + 8 * (i / 4) + 4 * half + (i % 4);
-        const int q = q0 + wq + l32;
-        if (j < m && q < k) P[int64_t(j) * k + q] = cx<double>{double(cre[i]), double(cim[i])};
-    }
+    store_partials(part, wk.z, m, k, j0, q0, cre, cim);
 }
 
 __global__ __launch_bounds__(256) void mu_sum_kernel(const double *part, double *table, int64_t count, int32_t splits,
@@ -160,9 +281,12 @@ void mu_mfma(kmsd_ctx *c, const void *left, const void *right, double *table, in
     if (m >= (int64_t(1) << 30) || k >= (int64_t(1) << 30)) throw Error(KMSD_EINVAL, "mu table too large");
     const int64_t gx = (m + TILE - 1) / TILE, gy = (k + TILE - 1) / TILE;
     const int64_t count = m * k;
-    int64_t splits = std::max<int64_t>(1, (2048 + gx * gy - 1) / (gx * gy));
-    splits = std::min<int64_t>(splits, std::max<int64_t>(1, (n + KT - 1) / KT));
-    splits = std::max<int64_t>(1, std::min<int64_t>(splits, (int64_t(1) << 31) / (16 * count)));
+    // Splits of the N range: enough workgroups to fill the chip twice over,
+    // at most 2^17 entries per FP32 accumulation run, partials <= 1 GiB.
+    const int64_t slabs = (n + KT - 1) / KT;
+    int64_t splits = std::max<int64_t>((2048 + gx * gy - 1) / (gx * gy), (n + (int64_t(1) << 17) - 1) >> 17);
+    splits = std::min<int64_t>(splits, slabs);
+    splits = std::max<int64_t>(1, std::min<int64_t>(splits, (int64_t(1) << 30) / (16 * count)));
     int64_t rows = (n + splits - 1) / splits;
     rows = ((rows + KT - 1) / KT) * KT;
     splits = (n + rows - 1) / rows;
@@ -178,13 +302,16 @@ void mu_mfma(kmsd_ctx *c, const void *left, const void *right, double *table, in
         c->mu_part_bytes = bytes;
     }
     double *part = static_cast<double *>(c->mu_part);
-    const dim3 grid{unsigned(gx), unsigned(gy), unsigned(splits)};
+    const int64_t total = gx * gy * splits;
+    if (total >= (int64_t(1) << 31)) throw Error(KMSD_EINVAL, "mu table too large");
     const float *L = static_cast<const float *>(left);
     const float *R = static_cast<const float *>(right);
     if (mode == KMSD_MU_BF16X3)
-        hipLaunchKernelGGL(mu_mfma_bf16x3_kernel, grid, dim3(256), 0, c->stream, L, R, part, n, int(m), int(k), rows);
+        hipLaunchKernelGGL(mu_bf16x3_kernel, dim3(unsigned(total)), dim3(256), 0, c->stream, L, R, part, n, int(m),
+                           int(k), int(gx), int(gy), total, rows);
     else
-        hipLaunchKernelGGL(mu_mfma_f32_kernel, grid, dim3(256), 0, c->stream, L, R, part, n, int(m), int(k), rows);
+        hipLaunchKernelGGL(mu_f32_kernel, dim3(unsigned(total)), dim3(256), 0, c->stream, L, R, part, n, int(m),
+                           int(k), int(gx), int(gy), total, rows);
     HIPCHECK(hipGetLastError());
     const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 8192)));
     hipLaunchKernelGGL(mu_sum_kernel, dim3(g), dim3(256), 0, c->stream, part, table, count, int32_t(splits),

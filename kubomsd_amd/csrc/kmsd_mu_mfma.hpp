@@ -1,12 +1,11 @@
 // Square double-moment tables on the matrix cores (gfx950 MFMA).
 //
 // C[j][q] = sum_i conj(L[j][i]) R[q][i] for complex64 L (m x N) and R (k x N)
-// with one batch column.  Reading the interleaved complex rows as real rows
-// of length 2N (L', R'), the real part is L' R'^T and the imaginary part is
-// L' R''^T with R''[q][2i] = Im R, R''[q][2i+1] = -Re R, so one pass over the
-// operands feeds two real GEMMs.  Workgroup tile 64 x 64 (four wavefronts of
-// 32 x 32), K staged through LDS in slabs of KT complex entries; the N
-// reduction is split across workgroups into partial tables.
+// with one batch column, as a real part sum_i (Lr Rr + Li Ri) and an
+// imaginary part sum_i (Lr Ri - Li Rr).  Workgroup tiles of 128 x 128 with the
+// N reduction split across workgroups into FP64 partial tables (tiling,
+// operand layout and the XCD-aware numbering are described in
+// kmsd_mu_mfma.hip).
 //   KMSD_MU_FP32  : v_mfma_f32_32x32x2_f32, FP32 products and accumulation
 //   KMSD_MU_BF16X3: operands split x = hi + lo in bf16; three products
 //                   hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16
