@@ -282,9 +282,11 @@ void mu_mfma(kmsd_ctx *c, const void *left, const void *right, double *table, in
     const int64_t gx = (m + TILE - 1) / TILE, gy = (k + TILE - 1) / TILE;
     const int64_t count = m * k;
     // Splits of the N range: enough workgroups to fill the chip twice over,
-    // at most 2^17 entries per FP32 accumulation run, partials <= 1 GiB.
+    // at most 2^16 entries per FP32 accumulation run (the FP32 rounding of the
+    // accumulators grows with the run length; partial sums are FP64),
+    // partials <= 1 GiB.
     const int64_t slabs = (n + KT - 1) / KT;
-    int64_t splits = std::max<int64_t>((2048 + gx * gy - 1) / (gx * gy), (n + (int64_t(1) << 17) - 1) >> 17);
+    int64_t splits = std::max<int64_t>((2048 + gx * gy - 1) / (gx * gy), (n + (int64_t(1) << 16) - 1) >> 16);
     splits = std::min<int64_t>(splits, slabs);
     splits = std::max<int64_t>(1, std::min<int64_t>(splits, (int64_t(1) << 30) / (16 * count)));
     int64_t rows = (n + splits - 1) / splits;
