@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     p.add_argument("--no-fp64-check", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-plain-baseline", action="store_true",
+                   help="skip the FP64 GPU comparison paths (fused complex128 chain, unfused torch path)")
     p.add_argument("--cpu-seconds", type=float, default=5.0, help="target seconds per CPU baseline run")
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
@@ -165,11 +167,6 @@ def nonorthogonal(args):
     from kubomsd_amd.synthetic import cubic_orbital_model
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
-    # 4 complex columns per lane: the fastest BSR4 product shape for 32
-    # complex64 columns in a sweep of cpl x workgroups on MI355X.
-    ctx.set_tuning("cpl", args.cpl or 4)
-    if args.groups:
-        ctx.set_tuning("groups", args.groups)
     if args.pcg_poll:
         ctx.set_tuning("pcg_poll", args.pcg_poll)
     t_build = time.perf_counter()
@@ -178,6 +175,7 @@ def nonorthogonal(args):
     blocks = atom_block_factors(model.S, 4)
     metric = HipMetric(ctx, model.S, model.S.diagonal().real, blocks)
     eng = HipEngine(ctx, model.H, metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
+    tuning = apply_tuning(ctx, args, eng.H, nb, torch.complex64)
     bound = CUBIC64_BOUND if args.atoms == 64 and CUBIC64_BOUND else None
     if bound is None:
         import scipy.sparse.linalg as sla
@@ -238,7 +236,8 @@ def nonorthogonal(args):
                                f"{nb} vectors, M={M} moments of S^-1 H, block-Jacobi PCG (FP32 inner to 1e-6, "
                                f"FP64 refinement to 1e-10), one moment chain per step",
                    "n": n, "nnz_H": int(model.H.nnz), "nnz_S": int(model.S.nnz), "vectors": nb, "moments": M,
-                   "precision": "mixed", "parallelism": "single GPU"},
+                   "precision": "mixed", "parallelism": "single GPU", "storage": eng.H.info()["format"],
+                   "launch_shape": tuning},
         "pcg": {"solves": solves, "inner_iterations": its, "iterations_per_solve": its / max(solves, 1),
                 "iterations_per_s": its / (gpu_ms / 1e3),
                 "algorithmic_bytes_per_iteration": pcg_b},
@@ -377,6 +376,53 @@ def mu_errors(ctx, eng, Vd, r, A, Bm, mu32, mub, M, c, a):
     return out
 
 
+def plain_gpu_fp64(H, Bh, c, a, orders, device):
+    """The FP64 plain GPU path for comparison: the reference GPU edition's
+    unfused moment loop (kubo_msd_gpu/src/kubo_msd/solvers/chebyshev.py:60-71:
+    vendor sparse product, separate recursion and dot kernels), here with
+    torch's sparse CSR product and elementwise ops in complex128 and S = I (no
+    solve).  Returns moments/s over `orders` orders and the moments."""
+    import torch
+    Hc = H.tocsr()
+    Ht = torch.sparse_csr_tensor(torch.from_numpy(Hc.indptr.astype(np.int64)),
+                                 torch.from_numpy(Hc.indices.astype(np.int64)),
+                                 torch.from_numpy(Hc.data.astype(np.complex128)), size=Hc.shape).to(device)
+    B = torch.from_numpy(Bh).to(device)
+    nb = B.shape[1]
+    mu = torch.empty((orders, nb), dtype=torch.float64, device=device)
+
+    def run():
+        prev, curr = B, (Ht @ B - c * B) / a
+        mu[0] = (B.conj() * B).sum(0).real
+        mu[1] = (B.conj() * curr).sum(0).real
+        for l in range(2, orders):
+            nxt = 2 * (Ht @ curr - c * curr) / a - prev
+            mu[l] = (B.conj() * nxt).sum(0).real
+            prev, curr = curr, nxt
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    return nb * orders / (ms / 1e3), mu
+
+
+def apply_tuning(ctx, args, matrix, nb, dtype):
+    """Explicit --cpl/--groups, else the library's autotuner (cached)."""
+    if args.cpl or args.groups:
+        if args.cpl:
+            ctx.set_tuning("cpl", args.cpl)
+        if args.groups:
+            ctx.set_tuning("groups", args.groups)
+        return {"cpl": args.cpl or None, "groups": args.groups or None, "source": "command line"}
+    choice = ctx.autotune(matrix, nb, dtype)
+    choice["source"] = "kmsd_autotune"
+    return choice
+
+
 def random_phases(n, seed, ids):
     """exp(2 pi i u), u = PCG64(SeedSequence([seed, id])).random(n) per column
     (reference src/kubo_msd/metric/sampling.py:8-9)."""
@@ -471,10 +517,6 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
-    if args.groups:
-        ctx.set_tuning("groups", args.groups)
-    if args.cpl:
-        ctx.set_tuning("cpl", args.cpl)
     t_build = time.perf_counter()
     model = honeycomb_lattice(args.cells, t=-2.7, W=1.0, a=1.42)
     n, nb, M = model.n, args.vectors, args.moments
@@ -484,6 +526,7 @@ def main():
     metric = HipMetric(ctx, model.S, model.S.diagonal().real)
     eng = HipEngine(ctx, model.H, metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
     dt = torch.complex64 if args.precision == "mixed" else torch.complex128
+    tuning = apply_tuning(ctx, args, eng.H, nb, dt)
     B = torch.from_numpy(Bh).to(dt).to(ctx.torch_device)
     print(f"# setup {time.perf_counter() - t_build:.1f}s: N={n} nnz={model.H.nnz} nb={nb} M={M} "
           f"{args.precision}", file=sys.stderr, flush=True)
@@ -542,6 +585,29 @@ def main():
                    doubling_mu_max_rel=float(np.max(np.abs(mud - mu64)) / np.max(np.abs(mu64))),
                    doubling_dos_max_rel=float(np.max(np.abs(dosd - dos64)) / np.max(np.abs(dos64))))
         del B64
+    # FP64 paths on the same GPU: this library's fused complex128 chain, and the
+    # plain unfused path (vendor sparse product + separate kernels).
+    plain = None
+    if not args.no_plain_baseline:
+        B64 = torch.from_numpy(Bh).to(ctx.torch_device)
+        e4, e5 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        mu64f = eng.moments(B64, 256, c, a)
+        e4.record(stream)
+        mu64f = eng.moments(B64, 256, c, a)
+        e5.record(stream)
+        torch.cuda.synchronize()
+        fused64 = nb * 256 / (e4.elapsed_time(e5) / 1e3)
+        del B64
+        try:
+            plain_rate, mu_plain = plain_gpu_fp64(model.H, Bh, c, a, 256, ctx.torch_device)
+            agree = float((mu_plain - mu64f).abs().max() / mu64f.abs().max())
+            plain = {"moments_per_s": plain_rate, "orders_timed": 256, "max_rel_diff_vs_fused_fp64": agree,
+                     "fused_fp64_moments_per_s": fused64,
+                     "mixed_direct_over_plain": (nb * M / (ms_per_step / 1e3)) / plain_rate,
+                     "mixed_doubling_over_plain": (nb * M / (dbl_ms / 1e3)) / plain_rate,
+                     "path": "torch sparse CSR complex128 product + elementwise recursion + dot per order"}
+        except RuntimeError as exc:  # the vendor sparse product may not take complex128
+            plain = {"error": str(exc)[:200], "fused_fp64_moments_per_s": fused64}
     cpu = None
     if not args.no_cpu_baseline:
         cb_nb = 8
@@ -563,7 +629,8 @@ def main():
         "config": {"workload": f"honeycomb {args.cells}x{args.cells} cells (N={n}), t=-2.7 eV, W=1 eV, "
                                f"{nb} random-phase vectors, M={M} Jackson moments, one moment chain per step",
                    "n": n, "nnz": int(model.H.nnz), "vectors": nb, "moments": M,
-                   "precision": args.precision, "parallelism": "single GPU"},
+                   "precision": args.precision, "parallelism": "single GPU", "storage": eng.H.info()["format"],
+                   "launch_shape": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": per_launch,
@@ -574,6 +641,7 @@ def main():
                             "achieved_GBps": step_bytes(model.H.nnz, n, nb, s_val, s_vec, d=0) * dbl_orders
                             / (dbl_ms / 1e3) / 1e9},
         "err_vs_fp64": err,
+        "fp64_gpu_baselines": plain,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -37,6 +37,7 @@ SIGNATURES = {
     "kmsd_ctx_synchronize": [_P],
     "kmsd_ctx_memory": [_P, _P, _P],
     "kmsd_ctx_set_tuning": [_P, C.c_char_p, _I64],
+    "kmsd_autotune": [_P, _P, _I64, _I, _P],
     "kmsd_last_error": [_P],
     "kmsd_build_info": [_P, _I64],
     "kmsd_matrix_create": [_P, _I, _I64, _I64, _P, _P, _P, _P],
@@ -86,6 +87,9 @@ def library():
         lib.kmsd_last_error.restype = C.c_char_p
         _lib = lib
     return _lib
+
+
+_TUNED = {}
 
 
 def build_info():
@@ -176,6 +180,53 @@ class HipContext:
 
     def set_tuning(self, key, value):
         self.check(self.lib.kmsd_ctx_set_tuning(self.h, key.encode(), int(value)))
+
+    def autotune(self, matrix, nb, dtype, cache_path=None):
+        """Launch shape (columns per lane, workgroups) for products of `matrix`
+        on nb-column blocks of `dtype` (torch dtype), timed on the device by
+        kmsd_autotune and remembered per (device, storage, n, nnz, nb, dtype)
+        in this process and in a JSON file (KUBOMSD_TUNING_CACHE, default
+        ~/.cache/kubomsd_amd/tuning.json), so later runs reuse the same shape
+        and produce the same results.  Applies the shape to this context."""
+        import json
+        import os
+        torch = _torch()
+        code = KMSD_C64 if dtype == torch.complex64 else KMSD_C128
+        info = matrix.info()
+        key = "|".join(str(v) for v in (torch.cuda.get_device_name(self.device), info["format"], info["n"],
+                                         info["nnz"], int(nb), code))
+        path = cache_path or os.environ.get("KUBOMSD_TUNING_CACHE") or \
+            os.path.join(os.path.expanduser("~"), ".cache", "kubomsd_amd", "tuning.json")
+        choice = _TUNED.get(key)
+        if choice is None:
+            try:
+                with open(path) as f:
+                    choice = json.load(f).get(key)
+            except (OSError, ValueError):
+                choice = None
+        if choice is None:
+            self.bind_stream()
+            out = np.zeros(2, np.int64)
+            self.check(self.lib.kmsd_autotune(self.h, matrix.h, int(nb), code, _np_ptr(out)))
+            choice = {"cpl": int(out[0]), "groups": int(out[1])}
+            try:
+                os.makedirs(os.path.dirname(path), exist_ok=True)
+                try:
+                    with open(path) as f:
+                        table = json.load(f)
+                except (OSError, ValueError):
+                    table = {}
+                table[key] = choice
+                tmp = path + f".{os.getpid()}.tmp"
+                with open(tmp, "w") as f:
+                    json.dump(table, f, indent=1, sort_keys=True)
+                os.replace(tmp, path)
+            except OSError:
+                pass  # an unwritable cache only costs a re-tune next run
+        _TUNED[key] = choice
+        self.set_tuning("cpl", choice["cpl"])
+        self.set_tuning("groups", choice["groups"])
+        return dict(choice)
 
     def memory(self):
         used, budget = C.c_uint64(), C.c_uint64()

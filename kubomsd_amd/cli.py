@@ -66,6 +66,7 @@ def parser():
     run.add_argument("--efermi-ev", type=float)
     run.add_argument("--efermi-source")
     run.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    run.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     sub.add_parser("capabilities", help="report the HIP device and library")
     return p
 
@@ -99,6 +100,7 @@ def config_from_args(args):
     d["efermi_eV"] = d.pop("efermi_ev")
     d["output_npz"] = d.pop("npz")
     d["include_vac"] = not d.pop("no_vac")
+    d["autotune"] = not d.pop("no_autotune")
     d["axes"] = tuple(a.strip() for a in d["axes"].split(","))
     if d["axes_vectors"] is not None:
         d["axes_vectors"] = json.loads(d["axes_vectors"].read_text())

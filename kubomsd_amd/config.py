@@ -113,6 +113,7 @@ class RunConfig:
     precision: str = "fp64"
     device: int = 0
     device_memory_limit_gb: float = 0.
+    autotune: bool = True
 
     def __post_init__(self):
         object.__setattr__(self, "energy_eV", grid(self.energy_eV, "energy"))
@@ -179,6 +180,8 @@ class RunConfig:
             raise KuboMSDError("device must be a nonnegative integer")
         if not np.isfinite(self.device_memory_limit_gb) or self.device_memory_limit_gb < 0:
             raise KuboMSDError("device_memory_limit_gb must be nonnegative and finite")
+        if not isinstance(self.autotune, (bool, np.bool_)):
+            raise KuboMSDError("autotune must be a boolean")
         if self.spectral_bounds is not None:
             b = tuple(float(v) for v in self.spectral_bounds)
             if len(b) != 2 or not np.isfinite(b).all() or b[0] >= b[1]:

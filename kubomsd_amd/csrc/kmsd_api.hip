@@ -56,6 +56,7 @@ extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->flag) (void)hipFree(c->flag);
     if (c->mu_part) (void)hipFree(c->mu_part);
+    if (c->capture) (void)hipStreamDestroy(c->capture);
     delete c;
     return KMSD_OK;
 }
@@ -86,13 +87,16 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             if (value < 8 || value % 8) throw Error(KMSD_EINVAL, "groups must be a positive multiple of 8");
             c->groups = value;
         } else if (k == "cpl") {
-            if (value != 1 && value != 2 && value != 4) throw Error(KMSD_EINVAL, "cpl must be 1, 2 or 4");
+            if (value != 0 && value != 1 && value != 2 && value != 4)
+                throw Error(KMSD_EINVAL, "cpl must be 0 (automatic), 1, 2 or 4");
             c->cpl = int(value);
         } else if (k == "moment_chunk") {
             if (value < 1 || value > 4096) throw Error(KMSD_EINVAL, "moment_chunk must be in [1, 4096]");
             c->moment_chunk = int(value);
         } else if (k == "fp32_math") {
             c->fp32_math = value != 0;
+        } else if (k == "graphs") {
+            c->graphs = value != 0;
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
@@ -100,6 +104,87 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
         } else {
             throw Error(KMSD_EINVAL, "unknown tuning key");
         }
+    });
+}
+
+extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice) {
+    if (!c || !A || nb < 1 || nb > (int64_t(1) << 20) || !choice) return KMSD_EINVAL;
+    return guard(c, [&] {
+        if (dtype != KMSD_C64 && dtype != KMSD_C128) throw Error(KMSD_EINVAL, "dtype must be KMSD_C64 or KMSD_C128");
+        const size_t bytes = size_t(A->n) * size_t(nb) * elem_bytes(dtype);
+        void *X = dalloc(c, bytes);
+        void *O = nullptr;
+        const int saved_cpl = c->cpl;
+        const int64_t saved_groups = c->groups;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            O = dalloc(c, bytes);
+            HIPCHECK(hipMemsetAsync(X, 0, bytes, c->stream));
+            HIPCHECK(hipEventCreate(&e0));
+            HIPCHECK(hipEventCreate(&e1));
+            const int def_cpl = dtype == KMSD_C64 ? 2 : 1;
+            const int cpls[3] = {1, 2, 4};
+            const int64_t groups[3] = {1024, 2048, 4096};
+            double best = 0, def_time = 0;
+            int best_cpl = def_cpl;
+            int64_t best_groups = 2048;
+            for (int cp : cpls) {
+                if (dtype == KMSD_C128 && cp == 4) continue;
+                if (nb % cp) continue;
+                for (int64_t g : groups) {
+                    c->cpl = cp;
+                    c->groups = g;
+                    Step s;
+                    s.A = A;
+                    s.X = X;
+                    s.ldx = nb;
+                    s.OUT = O;
+                    s.ldo = nb;
+                    s.n = A->n;
+                    s.w = int32_t(nb);
+                    launch_step(c, dtype, s);  // warm-up
+                    float t[3];
+                    for (float &v : t) {
+                        HIPCHECK(hipEventRecord(e0, c->stream));
+                        launch_step(c, dtype, s);
+                        HIPCHECK(hipEventRecord(e1, c->stream));
+                        HIPCHECK(hipEventSynchronize(e1));
+                        HIPCHECK(hipEventElapsedTime(&v, e0, e1));
+                    }
+                    std::sort(t, t + 3);
+                    const double med = t[1];
+                    if (cp == def_cpl && g == 2048) def_time = med;
+                    if (best == 0 || med < best) {
+                        best = med;
+                        best_cpl = cp;
+                        best_groups = g;
+                    }
+                }
+            }
+            // Keep the default shape unless another is clearly (> 3 %) faster.
+            if (def_time > 0 && best > 0.97 * def_time) {
+                best_cpl = def_cpl;
+                best_groups = 2048;
+            }
+            c->cpl = best_cpl;
+            c->groups = best_groups;
+            choice[0] = best_cpl;
+            choice[1] = best_groups;
+        } catch (...) {
+            c->cpl = saved_cpl;
+            c->groups = saved_groups;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipStreamSynchronize(c->stream);
+            if (O) dfree(c, O, bytes);
+            dfree(c, X, bytes);
+            throw;
+        }
+        HIPCHECK(hipEventDestroy(e0));
+        HIPCHECK(hipEventDestroy(e1));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        dfree(c, O, bytes);
+        dfree(c, X, bytes);
     });
 }
 

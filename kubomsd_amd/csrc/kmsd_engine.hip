@@ -36,6 +36,30 @@ void cheb(kmsd_engine *e, int dtype, Step s) {
     launch_step(c, dtype, s);
 }
 
+// The fused steps of one chunk of a moment chain captured as a HIP graph and
+// replayed for later chunks (the chunk's partial reduction is an ordinary
+// launch after each replay, since only its output row changes).  With an
+// identity metric every step is one kernel with fixed buffers: after the
+// first two orders the three-term recursion alternates two buffers, so an
+// even-length chunk starts from the same buffer assignment every time.
+// Replays run the same kernels with the same arguments as eager launches
+// (bitwise the same results) at a fraction of the per-launch host cost.
+// Capture uses the context's private non-blocking stream (the caller's
+// stream may be the legacy default stream, which cannot be captured).
+struct ChainGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~ChainGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+hipStream_t capture_stream(kmsd_ctx *c) {
+    if (!c->capture) HIPCHECK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
+    return c->capture;
+}
+
 // out (+)= coef * Y where Y = V_axis-action(X) ; helpers for v_cov
 void add_scaled(kmsd_ctx *c, int dtype, const void *Y, int64_t ldy, void *out, int64_t ldo, int64_t n, int32_t w,
                 Cplx coef, bool accumulate) {
@@ -408,7 +432,7 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             cheb(e, dtype, s);
             const void *prev = B;
             void *cur = t1;
-            for (int64_t k = 2; k < L; ++k) {
+            auto step = [&](int64_t) {
                 void *o = (prev == B) ? spare : const_cast<void *>(prev);
                 Step q;
                 q.X = cur;
@@ -431,6 +455,52 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
                 cheb(e, dtype, q);
                 prev = cur;
                 cur = o;
+            };
+            // Graph replay of whole chunks: identity metric (one kernel per
+            // order, no host round trips), even chunk length, chunk-aligned.
+            const bool graphs = c->graphs && e->metric->identity && chunk % 2 == 0 && chunk >= 2;
+            ChainGraph cg;
+            const void *g_prev = nullptr;
+            const void *g_cur = nullptr;
+            int64_t k = 2;
+            while (k < L) {
+                if (graphs && k >= 4 && k + chunk <= L && (slot == chunk || slot == 0)) {
+                    if (slot == chunk) flush();  // eager partials before the graph
+                    if (base != k) throw Error(KMSD_EDEVICE, "internal error: chain chunk misaligned");
+                    if (!cg.exec) {
+                        g_prev = prev;
+                        g_cur = cur;
+                        const uint64_t h0 = e->h_calls;
+                        hipStream_t user = c->stream;
+                        c->stream = capture_stream(c);
+                        HIPCHECK_RESTORE(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), c, user);
+                        try {
+                            for (int64_t j = 0; j < chunk; ++j) step(k + j);  // partial slots 0 .. chunk-1
+                        } catch (...) {
+                            hipGraph_t g = nullptr;
+                            (void)hipStreamEndCapture(c->stream, &g);
+                            if (g) (void)hipGraphDestroy(g);
+                            c->stream = user;
+                            throw;
+                        }
+                        const hipError_t ec = hipStreamEndCapture(c->stream, &cg.graph);
+                        c->stream = user;
+                        HIPCHECK(ec);
+                        HIPCHECK(hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0));
+                        e->h_calls = h0;
+                    }
+                    if (prev != g_prev || cur != g_cur)
+                        throw Error(KMSD_EDEVICE, "internal error: chain buffers do not repeat per chunk");
+                    HIPCHECK(hipGraphLaunch(cg.exec, c->stream));
+                    launch_reduce(c, part, G, w, chunk, mu + base * nb, nb, 0);
+                    e->h_calls += uint64_t(chunk);
+                    base += chunk;
+                    slot = 0;
+                    k += chunk;
+                    continue;
+                }
+                step(k);
+                ++k;
             }
         }
         flush();

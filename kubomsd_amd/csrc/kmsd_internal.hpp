@@ -20,6 +20,8 @@ struct kmsd_ctx {
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
     int pcg_poll = 4;       // PCG iterations between host status polls
+    bool graphs = true;     // replay moment-chain chunks as HIP graphs (identity metric)
+    hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
     size_t mu_part_bytes = 0;
@@ -61,6 +63,16 @@ struct Error : std::runtime_error {
     do {                                                                                                      \
         hipError_t e_ = (x);                                                                                  \
         if (e_ != hipSuccess) throw ::kmsd::Error(KMSD_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// HIPCHECK that first restores the context's stream (used around capture).
+#define HIPCHECK_RESTORE(x, ctx, saved)                                                                     \
+    do {                                                                                                    \
+        hipError_t e_ = (x);                                                                                \
+        if (e_ != hipSuccess) {                                                                             \
+            (ctx)->stream = (saved);                                                                        \
+            throw ::kmsd::Error(KMSD_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_));                \
+        }                                                                                                   \
     } while (0)
 
 extern thread_local std::string g_create_error;

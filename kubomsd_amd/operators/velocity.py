@@ -18,7 +18,7 @@ import scipy.sparse as sp
 
 from ..backends.hip import HipContext, HipEngine, HipMatrix, HipMetric
 from ..contracts import MetricSolveReport
-from ..errors import KuboMSDError, NumericalError
+from ..errors import KuboMSDError, NumericalError, ResourceBlocked
 from ..metric.bounds import spd_bounds
 from ..units import HBAR_EV_FS
 
@@ -118,6 +118,15 @@ class OperatorBundle:
             else:
                 definitions.append((-1, []))
         self.H = HipMatrix(self.ctx, self.host_H)
+        self.tuning = None
+        if config.autotune:
+            # Launch shape for the dominant products (moment chains on one
+            # sample batch); remembered per matrix signature, so reruns and
+            # serving mixes use the same shape.
+            try:
+                self.tuning = self.ctx.autotune(self.H, config.sample_batch, self.dtype)
+            except ResourceBlocked:
+                self.tuning = None
         self.native = HipEngine(self.ctx, self.H, self.metric.device, definitions, config.metric_rtol,
                                 config.metric_atol, config.metric_maxiter, HBAR_EV_FS)
         self.S = self.metric.device.S

@@ -14,8 +14,9 @@ the budget drains, because no new work is admitted past a job that has waited
 longer than ``max_wait_s``).
 
 Every kernel on the path reduces deterministically (fixed-order partial sums,
-launch shapes fixed by the job alone), so a job's result is bitwise the same
-alone and in any mix; tests/test_gpu_serving.py checks it.
+launch shapes fixed by the job alone: jobs run with the default shapes, no
+timing-based tuning), so a job's result is bitwise the same alone and in any
+mix; tests/test_gpu_serving.py checks it.
 
 The reference has no serving layer; its closest analogue is the GPU edition's
 per-run admission (require_bytes, batch halving: kubo_msd_gpu/src/kubo_msd/
@@ -258,7 +259,7 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
         cfg = RunConfig(energy_eV=np.linspace(-7.5, 7.5, 61), time_fs=list(times), solver="stochastic-kpm",
                         kernel="jackson", backend="hip", n_moments=M, axes=("x",), nsamples=nb, sample_batch=nb,
                         statistics_block=max(1, nb // 4), include_vac=False, precision=precision,
-                        metric_rtol=1e-10, seed=20261019 + i)
+                        metric_rtol=1e-10, seed=20261019 + i, autotune=False)
         est = estimate_job_bytes(n_act, 4 * n_act, nb, 1, precision, nonorth)
         jobs.append(TransportJob(i, build, cfg, "stepwise", None, float(arrivals[i]), est,
                                  meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth)))

@@ -1,4 +1,5 @@
 import os
+import tempfile
 import sys
 from pathlib import Path
 
@@ -10,6 +11,9 @@ sys.path.insert(0, str(Path(__file__).resolve().parent))
 
 
 def pytest_configure(config):
+    # launch-shape tuning cache of the test session (not the user's)
+    os.environ.setdefault("KUBOMSD_TUNING_CACHE", os.path.join(tempfile.mkdtemp(prefix="kmsd_tuning_"),
+                                                               "tuning.json"))
     config.addinivalue_line("markers", "gpu: needs a real MI355X (HIP) device; never skipped silently")
 
 

@@ -113,3 +113,26 @@ def test_bsr_sampler_congruence_matches_csr(hip_ctx):
         met = HipMetric(hip_ctx, S, m.S.diagonal().real)
         outs.append(host(met.sample(to_dev(hip_ctx, xi), coeff, 1.0, 0.9)))
     assert rel_err(outs[0], outs[1]) < 1e-13
+
+
+def test_autotune_applies_and_caches_shape(hip_ctx, tmp_path):
+    """kmsd_autotune picks a valid shape, the wrapper stores it per matrix
+    signature, a second call reuses it, and products with the tuned shape
+    still match SciPy."""
+    import json
+    import torch
+    from kubomsd_amd.backends import hip as H
+    m = H.HipMatrix(hip_ctx, cubic().H)
+    path = tmp_path / "tuning.json"
+    H._TUNED.clear()
+    first = hip_ctx.autotune(m, 8, torch.complex64, cache_path=str(path))
+    assert first["cpl"] in (1, 2, 4) and first["groups"] in (1024, 2048, 4096)
+    table = json.loads(path.read_text())
+    assert list(table.values()) == [first]
+    H._TUNED.clear()
+    assert hip_ctx.autotune(m, 8, torch.complex64, cache_path=str(path)) == first  # from the file
+    B = random_block(m.n, 8, 4)
+    got = host(m.matmul(to_dev(hip_ctx, B, "c64")))
+    assert rel_err(got, m.host @ B.astype(np.complex64).astype(np.complex128)) < 2e-6
+    hip_ctx.set_tuning("cpl", 0)  # restore the session defaults for later tests
+    hip_ctx.set_tuning("groups", 2048)
