@@ -16,7 +16,7 @@ import numpy as np
 from ..contracts import Reason, TransportBlock
 from ..errors import NumericalError, ResourceBlocked
 from ..kernels.energy import dos_floor
-from ..metric.bounds import estimate_spectral_bounds
+from ..metric.device_bounds import estimate_spectral_bounds_device
 from ..metric.sampling import sample_metric
 from ..observables.diffusion import msd_gradient
 from ..observables.statistics import estimate_ratio
@@ -144,11 +144,13 @@ def _normalize_slab(raw, dos_blocks, counts, ns, valid, key):
 
 
 def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None):
-    """bounds_report: precomputed bounds (e.g. metric.bounds.certified_bounds)
-    used instead of the generalized Ritz check."""
+    """Spectral bounds come from the device two-start generalized Ritz check
+    (metric/device_bounds.py), which also validates user bounds as the
+    reference does; bounds_report: precomputed bounds (e.g.
+    metric.bounds.certified_bounds) used instead."""
     cfg = config
-    report = bounds_report or estimate_spectral_bounds(ops.host_H, ops.host_S, ops.metric.s_lower,
-                                                       ops.metric.bounds_report["certified"], cfg.spectral_bounds)
+    report = bounds_report or estimate_spectral_bounds_device(ops, cfg.spectral_bounds,
+                                                              ops.metric.bounds_report["certified"])
     cfg = replace(cfg, spectral_bounds=report["bounds"])
     ns = cfg.nsamples if samples is None else samples.shape[1]
     nbk = (ns + cfg.statistics_block - 1) // cfg.statistics_block
