@@ -174,6 +174,14 @@ int kmsd_series(kmsd_engine *e, const void *B, void *out, int64_t width, int dty
  * (cpp/native.cpp:574-589, 706-711).  out: device (L, N, nb) block. */
 int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, int dtype, double center,
                double half);
+/* Reference: the spectral path's cov = V chain(S^-1 V c, K+1)
+ * (src/kubo_msd/solvers/spectral.py:33-36, the v_cov of a stored chain):
+ * out[k] = v_cov(dir, T_k(H~) B), (L, N, nb).  One velocity product covers all
+ * orders (the chain is regrouped to N rows of L*nb columns and back on the
+ * device); two scratch blocks of L*N*nb elements come from the budget and
+ * stay with the engine for later calls (released by kmsd_engine_destroy). */
+int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, void *out, int64_t L, int64_t nb, int dtype,
+                 double center, double half);
 /* Reference: Engine.moments(B, L, center, half) -> f64 (L, nb)
  * (cpp/native.cpp:590-609, 712-717): mu[l][j] = Re <S B_j, T_l(A) B_j>.
  * mu: DEVICE f64 (L, nb). */

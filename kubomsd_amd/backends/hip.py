@@ -59,6 +59,7 @@ SIGNATURES = {
     "kmsd_h_action": [_P, _P, _P, _I64, _I],
     "kmsd_v_cov": [_P, _P, _P, _P, _I64, _I],
     "kmsd_series": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D, _P, _I64],
+    "kmsd_chain_v": [_P, _P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_chain": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments_doubling": [_P, _P, _P, _I64, _I64, _I, _D, _D],
@@ -490,6 +491,19 @@ class HipEngine:
             out = torch.empty((int(length),) + tuple(B.shape), dtype=B.dtype, device=B.device)
         self._check(self.ctx.lib.kmsd_chain(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
                                             int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
+        return out
+
+    def chain_v(self, B, length, center, half, direction, out=None):
+        """(length, N, nb): v_cov(direction, T_k(H~) B) for k < length."""
+        B = self._prep(B)
+        torch = _torch()
+        d = np.ascontiguousarray(direction, np.float64)
+        if d.shape != (3,):
+            raise KuboMSDError("direction must have three components")
+        if out is None:
+            out = torch.empty((int(length),) + tuple(B.shape), dtype=B.dtype, device=B.device)
+        self._check(self.ctx.lib.kmsd_chain_v(self.h, _np_ptr(d), C.c_void_p(B.data_ptr()), C.c_void_p(out.data_ptr()),
+                                              int(length), B.shape[1], _dtype_code(B), float(center), float(half)))
         return out
 
     def moments(self, B, length, center, half, doubling=False):

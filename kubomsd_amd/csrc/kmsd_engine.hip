@@ -319,15 +319,14 @@ extern "C" int kmsd_series(kmsd_engine *e, const void *B, void *out, int64_t wid
     });
 }
 
-extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, int dtype, double center,
-                          double half) {
-    if (!e) return KMSD_EINVAL;
+namespace {
+
+// out[k] = T_k(H~) B for k < L, (L, N, nb) blocks.
+void chain_into(kmsd_engine *e, int dtype, const void *B, void *out, int64_t L, int64_t nb, double center,
+                double half) {
     kmsd_ctx *c = e->ctx;
-    return guard(c, [&] {
-        check_block_args(B, out, nb, dtype);
-        if (L < 1) throw Error(KMSD_EINVAL, "Chebyshev chain must be nonempty");
+    {
         const int64_t n = n_of(e);
-        require_finite(c, dtype, B, nb, n, int32_t(nb));
         const size_t stride = size_t(n) * size_t(nb) * elem_bytes(dtype);
         char *o = static_cast<char *>(out);
         launch_scale(c, dtype, B, nb, o, nb, n, int32_t(nb), Cplx{1, 0});
@@ -361,6 +360,48 @@ extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, i
             s.w = int32_t(nb);
             cheb(e, dtype, s);
         }
+    }
+}
+
+}  // namespace
+
+extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, int dtype, double center,
+                          double half) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        if (L < 1) throw Error(KMSD_EINVAL, "Chebyshev chain must be nonempty");
+        require_finite(c, dtype, B, nb, n_of(e), int32_t(nb));
+        chain_into(e, dtype, B, out, L, nb, center, half);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, void *out, int64_t L, int64_t nb,
+                            int dtype, double center, double half) {
+    if (!e) return KMSD_EINVAL;
+    kmsd_ctx *c = e->ctx;
+    return guard(c, [&] {
+        check_block_args(B, out, nb, dtype);
+        if (!dir || !std::isfinite(dir[0]) || !std::isfinite(dir[1]) || !std::isfinite(dir[2]))
+            throw Error(KMSD_EINVAL, "direction must have three finite components");
+        if (L < 1) throw Error(KMSD_EINVAL, "Chebyshev chain must be nonempty");
+        if (L * nb >= (int64_t(1) << 31)) throw Error(KMSD_EINVAL, "chain too wide for one velocity product");
+        const int64_t n = n_of(e);
+        require_finite(c, dtype, B, nb, n, int32_t(nb));
+        chain_into(e, dtype, B, out, L, nb, center, half);
+        // One velocity product over all orders: (L, N, nb) -> (N, L*nb) rows,
+        // v_cov on L*nb columns, and back.
+        // (grow-only engine scratch: large device allocations cost more than
+        // the regrouping itself)
+        const size_t total = size_t(L) * size_t(n) * size_t(nb) * elem_bytes(dtype);
+        const int64_t words = nb * int64_t(elem_bytes(dtype) / 8);
+        void *flat = e->cv1.ensure(c, total);
+        void *vflat = e->cv2.ensure(c, total);
+        launch_pixel_transpose(c, out, flat, L, n, words);
+        vcov(e, dtype, dir, flat, L * nb, vflat, L * nb, int32_t(L * nb), Cplx{1, 0}, false);
+        launch_pixel_transpose(c, vflat, out, n, L, words);
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
 }

@@ -167,6 +167,8 @@ void launch_reduce(kmsd_ctx *c, const double *part, int64_t groups, int32_t w, i
                    int64_t ldo, int mode);
 void launch_scale(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, void *O, int64_t ldo, int64_t n, int32_t w,
                   Cplx coef);
+// (R, C) grid of pixels of `words` 8-byte words -> (C, R) grid (out != in).
+void launch_pixel_transpose(kmsd_ctx *c, const void *in, void *out, int64_t R, int64_t C, int64_t words);
 bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
 void launch_convert(kmsd_ctx *c, int dtype_in, const void *X, int64_t ldx, int dtype_out, void *O, int64_t ldo,
                     int64_t n, int32_t w);
@@ -225,6 +227,7 @@ struct kmsd_engine {
     int maxiter = 1000;
     int64_t h_calls = 0, v_calls = 0;
     kmsd::DBuf t0, t1, t2, t3, tmp, tmp2, tmp3, part, wbuf;
+    kmsd::DBuf cv1, cv2;  // kmsd_chain_v regrouped blocks (grow-only, like the other scratch)
 };
 
 namespace kmsd {

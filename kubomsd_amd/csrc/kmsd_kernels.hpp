@@ -623,6 +623,47 @@ __global__ __launch_bounds__(256) void bsr4_congruence_kernel(void *valv, const 
     }
 }
 
+// (R, C) grid of pixels of W 8-byte words -> (C, R) grid.  W <= 4: 32 x 32
+// pixel tiles through LDS (coalesced reads and writes); one pixel per thread
+// per pass, block (32, 8).
+template <int W>
+__global__ __launch_bounds__(256) void pixel_transpose_tiled(const uint64_t *in, uint64_t *out, int64_t R,
+                                                             int64_t C) {
+    __shared__ uint64_t tile[32][33 * W];
+    const int64_t tiles_c = (C + 31) / 32;
+    const int64_t c0 = (int64_t(blockIdx.x) % tiles_c) * 32, r0 = (int64_t(blockIdx.x) / tiles_c) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        const int64_t r = r0 + ty + j, c = c0 + tx;
+        if (r < R && c < C) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) tile[ty + j][tx * W + q] = in[(r * C + c) * W + q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        const int64_t c = c0 + ty + j, r = r0 + tx;
+        if (r < R && c < C) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) out[(c * R + r) * W + q] = tile[tx][(ty + j) * W + q];
+        }
+    }
+}
+
+// Same for pixels of any width (W words, W > 4: every pixel is a contiguous
+// segment of >= 40 bytes, so direct copies stay coalesced enough).
+static __global__ __launch_bounds__(256) void pixel_transpose_direct(const uint64_t *in, uint64_t *out, int64_t R,
+                                                                     int64_t C, int64_t W) {
+    const int64_t total = R * C * W;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t pix = e / W, q = e - pix * W;
+        const int64_t r = pix / C, c = pix - r * C;
+        out[(c * R + r) * W + q] = in[e];
+    }
+}
+
 // out_i = z_i (out_i + c0 xi_i), z = diag^-1/2 (final Clenshaw step of the sampler)
 template <class VT>
 __global__ __launch_bounds__(256) void sample_finish_kernel(void *Ov, const void *Xv, int64_t ld, int64_t n, int32_t w,

@@ -544,6 +544,27 @@ kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const dou
     return W;
 }
 
+void launch_pixel_transpose(kmsd_ctx *c, const void *in, void *out, int64_t R, int64_t C, int64_t words) {
+    if (R < 1 || C < 1) return;
+    const uint64_t *I = static_cast<const uint64_t *>(in);
+    uint64_t *O = static_cast<uint64_t *>(out);
+    if (words <= 4) {
+        const int64_t tiles = ((C + 31) / 32) * ((R + 31) / 32);
+        if (tiles >= (int64_t(1) << 31)) throw Error(KMSD_EINVAL, "transpose grid too large");
+        const dim3 grid{unsigned(tiles)};
+        switch (words) {
+            case 1: hipLaunchKernelGGL(pixel_transpose_tiled<1>, grid, dim3(256), 0, c->stream, I, O, R, C); break;
+            case 2: hipLaunchKernelGGL(pixel_transpose_tiled<2>, grid, dim3(256), 0, c->stream, I, O, R, C); break;
+            case 3: hipLaunchKernelGGL(pixel_transpose_tiled<3>, grid, dim3(256), 0, c->stream, I, O, R, C); break;
+            default: hipLaunchKernelGGL(pixel_transpose_tiled<4>, grid, dim3(256), 0, c->stream, I, O, R, C);
+        }
+    } else {
+        const int64_t g = blocks_for(R * C * words);
+        hipLaunchKernelGGL(pixel_transpose_direct, dim3(unsigned(g)), dim3(256), 0, c->stream, I, O, R, C, words);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
 void launch_doubling_finish(kmsd_ctx *c, const double *ab, double *mu, int64_t L, int32_t w) {
     const int64_t g = blocks_for(L * w);
     hipLaunchKernelGGL(doubling_finish_kernel, dim3(unsigned(g)), dim3(256), 0, c->stream, ab, mu, L, w);

@@ -206,6 +206,14 @@ class OperatorBundle:
                                                    config.metric_maxiter)
         return self.native.v_cov(B, direction)
 
+    def apply_v_chain(self, axis, B, length, scale):
+        """(length, N, nb): v_cov(axis, T_k(H~) B), H~ = (S^-1 H - c)/a with
+        scale = (c, a) (kmsd_chain_v)."""
+        direction = self.directions.get(axis, np.eye(3)["xyz".index(axis)] if axis in "xyz" else None)
+        if direction is None:
+            raise KuboMSDError("unknown direction")
+        return self.native.chain_v(B, length, scale[0], scale[1], direction)
+
     def apply_v_action(self, axis, B):
         return self.solve_s(self.apply_v_cov(axis, B))[0]
 

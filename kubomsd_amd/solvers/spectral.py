@@ -14,7 +14,7 @@ import numpy as np
 
 from ..config import resolve_time_grid
 from ..errors import NumericalError
-from .chebyshev import bessel_coefficients, chain, projected_quadratic, reconstruction
+from .chebyshev import bessel_coefficients, chain, projected_quadratic, reconstruction, scaling
 from .stepwise import propagate
 
 
@@ -94,10 +94,8 @@ def spectral_samples(ops, cfg, samples):
             left = chain(c_anchor, ops, cfg, L + K)  # (L+K, n, nb)
             tables = []
             for ai, axis in enumerate(cfg.axes):
-                right = chain(v_anchor[ai], ops, cfg, K + 1)  # (K+1, n, nb)
-                flat = right.permute(1, 0, 2).reshape(n, -1).contiguous()
-                cov = ops.apply_v_cov(axis, flat).reshape(n, K + 1, nb).permute(1, 0, 2).contiguous()
-                del right, flat
+                # cov = V chain(S^-1 V c, K+1): chain and one velocity product over all orders
+                cov = ops.apply_v_chain(axis, v_anchor[ai], K + 1, scaling(cfg))  # (K+1, n, nb)
                 tables.append(_table(ops, left, cov))  # (nb, L+K, K+1)
                 del cov
             del left

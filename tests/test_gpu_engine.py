@@ -135,3 +135,21 @@ def test_moment_doubling_matches_direct_oracle(hip_ctx, honey, dtype, L):
     got = host(hip_engine(hip_ctx, m).moments(to_dev(hip_ctx, B, dtype), L, 0.05, 8.8, doubling=True))
     assert got.shape == want.shape
     assert rel_err(got, want) < (1e-11 if dtype == "c128" else 2e-5)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 5])
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_chain_v_matches_chain_then_velocity(hip_ctx, nb, dtype):
+    """kmsd_chain_v (on-device regrouping around one velocity product) equals
+    chain + per-order v_cov, bit for bit on pixels of 1..5 complex entries
+    (tiled and direct transposes)."""
+    from helpers import hip_engine
+    from kubomsd_amd.synthetic import square_lattice
+    m = square_lattice(12, t=-1.0, W=2.0)
+    eng = hip_engine(hip_ctx, m)
+    B = to_dev(hip_ctx, random_block(m.n, nb, 21), dtype)
+    ch = eng.chain(B, 7, 0.0, 5.1)
+    want = np.stack([host(eng.v_cov(ch[k].contiguous(), [1.0, 0, 0])) for k in range(7)])
+    got = host(eng.chain_v(B, 7, 0.0, 5.1, [1.0, 0, 0]))
+    assert got.shape == (7, m.n, nb)
+    assert rel_err(got, want) < (1e-14 if dtype == "c128" else 1e-6)

@@ -160,6 +160,10 @@ def test_nonorthogonal_engine_matches_oracle(hip_ctx, kind, dtype):
     assert rel_err(host(he.h_action(to_dev(hip_ctx, B, dtype))), oe.h_action(B)) < tol
     assert rel_err(host(he.v_cov(to_dev(hip_ctx, B, dtype), [1.0, 0, 0])), oe.v_cov(B, [1.0, 0, 0])) < tol
     assert rel_err(host(he.chain(to_dev(hip_ctx, B, dtype), 12, 0.0, 2.0)), oe.chain(B, 12, 0.0, 2.0)) < tol * 10
+    # velocity of every chain order (the spectral path's cov), one product over all orders
+    ch = oe.chain(B, 12, 0.0, 2.0)
+    want_v = np.stack([oe.v_cov(np.ascontiguousarray(ch[k]), [1.0, 0, 0]) for k in range(12)])
+    assert rel_err(host(he.chain_v(to_dev(hip_ctx, B, dtype), 12, 0.0, 2.0, [1.0, 0, 0])), want_v) < tol * 10
     assert rel_err(host(he.moments(to_dev(hip_ctx, B, dtype), 20, 0.0, 2.0)), oe.moments(B, 20, 0.0, 2.0)) < tol * 10
     coeff = np.array([.9, .3j, -.05, .01j, .002])
     Ba = np.hstack([B, 0.1 * B])
