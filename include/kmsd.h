@@ -66,20 +66,26 @@ int kmsd_ctx_set_stream(kmsd_ctx *ctx, void *stream);
 int kmsd_ctx_synchronize(kmsd_ctx *ctx);
 /* bytes currently charged to the budget and the budget itself */
 int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
-/* Launch tuning: key "groups" (workgroups per launch, multiple of 8),
- * "cpl" (complex columns per lane: 1, 2 or 4; 0 = 2 for complex64 and 1
- * for complex128 blocks), "moment_chunk" (Chebyshev
- * orders per partial-reduction launch), "fp32_math" (complex64 blocks: FP32
- * values and row sums), "pcg_poll" (PCG iterations between host polls),
- * "graphs" (1: moment chains with an identity metric replay whole chunks of
- * orders as one HIP graph; 0: launch every order; same results either way). */
+/* Launch tuning: key "groups" (workgroups per launch, multiple of 8; 0 =
+ * automatic, the default: each fused step launches one workgroup per
+ * resident slot of its kernel on the device), "cpl" (complex columns per
+ * lane: 1, 2 or 4; 0 = 2 for complex64 and 1 for complex128 blocks),
+ * "moment_chunk" (Chebyshev orders per partial-reduction launch),
+ * "fp32_math" (complex64 blocks: FP32 values and row sums), "pcg_poll" (PCG
+ * iterations between host polls), "graphs" (1: moment chains with an
+ * identity metric replay whole chunks of orders as one HIP graph; 0: launch
+ * every order; same results either way), "ell_kernel" (1, the default:
+ * products whose off-diagonal part is ELL of width 3, 4 or 6 use the kernel
+ * that prefetches the next row's indices; 0: the generic row kernel; same
+ * results either way). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
- * Times the product of A on an nb-column block of dtype for every columns-
- * per-lane (1, 2, 4) and workgroup count (1024, 2048, 4096), keeps the
- * default shape unless another is more than 3 % faster, applies the choice
- * to ctx ("cpl", "groups") and writes it to choice[2].  Uses two scratch
- * blocks of n*nb elements from the budget. */
+ * Times the fused Chebyshev step of a moment chain with product A on an
+ * nb-column block of dtype for every columns-per-lane (1, 2, 4) and
+ * workgroup count (automatic, 1024, 2048, 4096), keeps the default shape
+ * (automatic workgroups) unless another is more than 3 % faster, applies
+ * the choice to ctx ("cpl", "groups") and writes it to choice[2] (groups 0 =
+ * automatic).  Uses three scratch blocks of n*nb elements from the budget. */
 int kmsd_autotune(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice);
 /* Message of the last failure in ctx (or of the last failed create when ctx
  * is NULL).  Valid until the next call on ctx. */

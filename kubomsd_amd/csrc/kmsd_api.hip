@@ -84,8 +84,11 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
     return guard(c, [&] {
         const std::string k(key);
         if (k == "groups") {
-            if (value < 8 || value % 8) throw Error(KMSD_EINVAL, "groups must be a positive multiple of 8");
-            c->groups = value;
+            // 0: automatic (each fused step launches its kernel's resident capacity)
+            if (value != 0 && (value < 8 || value % 8))
+                throw Error(KMSD_EINVAL, "groups must be 0 (automatic) or a positive multiple of 8");
+            c->auto_groups = value == 0;
+            c->groups = value ? value : int64_t(std::max(8, c->cus * 8));
         } else if (k == "cpl") {
             if (value != 0 && value != 1 && value != 2 && value != 4)
                 throw Error(KMSD_EINVAL, "cpl must be 0 (automatic), 1, 2 or 4");
@@ -97,6 +100,8 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             c->fp32_math = value != 0;
         } else if (k == "graphs") {
             c->graphs = value != 0;
+        } else if (k == "ell_kernel") {
+            c->ell_kernel = value != 0;
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
@@ -111,35 +116,65 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
     if (!c || !A || nb < 1 || nb > (int64_t(1) << 20) || !choice) return KMSD_EINVAL;
     return guard(c, [&] {
         if (dtype != KMSD_C64 && dtype != KMSD_C128) throw Error(KMSD_EINVAL, "dtype must be KMSD_C64 or KMSD_C128");
+        // Times the fused Chebyshev step of a moment chain (gathered X,
+        // previous iterate P overwritten in place, projection W, moment
+        // partials) for each candidate shape.
         const size_t bytes = size_t(A->n) * size_t(nb) * elem_bytes(dtype);
-        void *X = dalloc(c, bytes);
-        void *O = nullptr;
+        const int64_t max_groups = std::max<int64_t>(4096, int64_t(c->cus) * 8);
+        const size_t pbytes = size_t(max_groups) * size_t(nb) * 2 * sizeof(double);
+        void *X = nullptr, *P = nullptr, *W = nullptr, *part = nullptr;
         const int saved_cpl = c->cpl;
         const int64_t saved_groups = c->groups;
+        const bool saved_auto = c->auto_groups;
         hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            e0 = e1 = nullptr;
+            (void)hipStreamSynchronize(c->stream);
+            dfree(c, part, pbytes);
+            dfree(c, W, bytes);
+            dfree(c, P, bytes);
+            dfree(c, X, bytes);
+        };
         try {
-            O = dalloc(c, bytes);
+            X = dalloc(c, bytes);
+            P = dalloc(c, bytes);
+            W = dalloc(c, bytes);
+            part = dalloc(c, pbytes);
             HIPCHECK(hipMemsetAsync(X, 0, bytes, c->stream));
+            HIPCHECK(hipMemsetAsync(P, 0, bytes, c->stream));
+            HIPCHECK(hipMemsetAsync(W, 0, bytes, c->stream));
             HIPCHECK(hipEventCreate(&e0));
             HIPCHECK(hipEventCreate(&e1));
             const int def_cpl = dtype == KMSD_C64 ? 2 : 1;
             const int cpls[3] = {1, 2, 4};
-            const int64_t groups[3] = {1024, 2048, 4096};
+            const int64_t groups[4] = {0, 1024, 2048, 4096};  // 0 = resident capacity
             double best = 0, def_time = 0;
             int best_cpl = def_cpl;
-            int64_t best_groups = 2048;
+            int64_t best_groups = 0;
             for (int cp : cpls) {
                 if (dtype == KMSD_C128 && cp == 4) continue;
                 if (nb % cp) continue;
                 for (int64_t g : groups) {
                     c->cpl = cp;
-                    c->groups = g;
+                    c->auto_groups = g == 0;
+                    c->groups = g ? g : int64_t(std::max(8, c->cus * 8));
                     Step s;
                     s.A = A;
                     s.X = X;
                     s.ldx = nb;
-                    s.OUT = O;
+                    s.P = P;
+                    s.ldp = nb;
+                    s.OUT = P;
                     s.ldo = nb;
+                    s.W = W;
+                    s.ldw = nb;
+                    s.part = static_cast<double *>(part);
+                    s.alpha = Cplx{1.0, 0};
+                    s.beta = Cplx{-0.5, 0};
+                    s.gamma = -1.0;
+                    s.use_beta = s.use_p = true;
                     s.n = A->n;
                     s.w = int32_t(nb);
                     launch_step(c, dtype, s);  // warm-up
@@ -153,7 +188,7 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
                     }
                     std::sort(t, t + 3);
                     const double med = t[1];
-                    if (cp == def_cpl && g == 2048) def_time = med;
+                    if (cp == def_cpl && g == 0) def_time = med;
                     if (best == 0 || med < best) {
                         best = med;
                         best_cpl = cp;
@@ -164,27 +199,21 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
             // Keep the default shape unless another is clearly (> 3 %) faster.
             if (def_time > 0 && best > 0.97 * def_time) {
                 best_cpl = def_cpl;
-                best_groups = 2048;
+                best_groups = 0;
             }
             c->cpl = best_cpl;
-            c->groups = best_groups;
+            c->auto_groups = best_groups == 0;
+            c->groups = best_groups ? best_groups : int64_t(std::max(8, c->cus * 8));
             choice[0] = best_cpl;
             choice[1] = best_groups;
         } catch (...) {
             c->cpl = saved_cpl;
             c->groups = saved_groups;
-            if (e0) (void)hipEventDestroy(e0);
-            if (e1) (void)hipEventDestroy(e1);
-            (void)hipStreamSynchronize(c->stream);
-            if (O) dfree(c, O, bytes);
-            dfree(c, X, bytes);
+            c->auto_groups = saved_auto;
+            release();
             throw;
         }
-        HIPCHECK(hipEventDestroy(e0));
-        HIPCHECK(hipEventDestroy(e1));
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        dfree(c, O, bytes);
-        dfree(c, X, bytes);
+        release();
     });
 }
 

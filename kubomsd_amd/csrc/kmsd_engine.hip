@@ -426,7 +426,7 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
         void *t1 = e->t0.ensure(c, bytes);
         void *spare = e->t1.ensure(c, bytes);
         const int cap = cpl_cap_for(dtype, {{B, nb}, {W, nb}, {t1, nb}, {spare, nb}});
-        const int64_t G = step_groups(c, n, w, dtype, cap);
+        const int64_t G = step_groups(c, n, w, dtype, cap, e->metric->identity ? e->H : nullptr);
         const int64_t chunk = std::min<int64_t>(c->moment_chunk, L);
         const size_t slot_doubles = size_t(G) * size_t(w) * 2;
         double *part = static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
@@ -449,7 +449,7 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             s.n = n;
             s.w = w;
             s.cpl_cap = cap;
-            s.expect_groups = G;
+            s.groups = G;
             s.part = next_part();
             launch_step(c, dtype, s);
         }
@@ -468,7 +468,7 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             s.n = n;
             s.w = w;
             s.cpl_cap = cap;
-            s.expect_groups = G;
+            s.groups = G;
             s.part = next_part();
             cheb(e, dtype, s);
             const void *prev = B;
@@ -491,7 +491,7 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
                 q.n = n;
                 q.w = w;
                 q.cpl_cap = cap;
-                q.expect_groups = G;
+                q.groups = G;
                 q.part = next_part();
                 cheb(e, dtype, q);
                 prev = cur;
@@ -565,7 +565,7 @@ extern "C" int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, 
         void *t1 = e->t0.ensure(c, bytes);
         void *spare = e->t1.ensure(c, bytes);
         const int cap = cpl_cap_for(dtype, {{B, nb}, {t1, nb}, {spare, nb}});
-        const int64_t G = step_groups(c, n, w, dtype, cap);
+        const int64_t G = step_groups(c, n, w, dtype, cap, e->H);
         const int64_t steps = (L - 1) / 2 + 1;  // k = 0 .. floor((L-1)/2)
         const int64_t chunk = std::min<int64_t>(c->moment_chunk, steps);
         const size_t slot_doubles = size_t(G) * size_t(w) * 2;
@@ -603,7 +603,7 @@ extern "C" int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, 
             q.n = n;
             q.w = w;
             q.cpl_cap = cap;
-            q.expect_groups = G;
+            q.groups = G;
             q.part = next_part();
             cheb(e, dtype, q);
             prev = cur;

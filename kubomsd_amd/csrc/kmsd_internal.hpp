@@ -16,11 +16,13 @@ struct kmsd_ctx {
     uint64_t budget = 0, used = 0;
     std::string err;
     int64_t groups = 2048;  // workgroups per row-walking launch (multiple of 8)
+    bool auto_groups = true; // fused steps: one workgroup per resident slot of the kernel (else `groups`)
     int cpl = 0;            // 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
     int pcg_poll = 4;       // PCG iterations between host status polls
     bool graphs = true;     // replay moment-chain chunks as HIP graphs (identity metric)
+    bool ell_kernel = true; // fixed-width ELL products (widths 3, 4, 6) use the prefetching kernel
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
@@ -152,14 +154,17 @@ struct Step {
     int cpl_cap = 4;            // columns-per-lane ceiling (keeps the shape fixed across a chain)
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
-    int64_t expect_groups = 0;  // if nonzero, the launch must use exactly this many workgroups
+    int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
 };
 // Launches one fused step; returns the number of workgroups used (the
 // partial-sum row count) so callers can size/reduce partials.
 int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s);
-// Number of workgroups launch_step will use for (n, w, dtype).
-int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap = 4);
+// Number of workgroups launch_step would choose for (n, w, dtype) with
+// product operator A (null: no product).  Chains pass it as Step::groups to
+// every launch that writes partial rows of one reduction.
+int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap, const kmsd_matrix *A,
+                    bool f32_matrix = false);
 // Columns-per-lane ceiling supported by a set of blocks (alignment and leading dimensions).
 int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t>> blocks);
 // out[s*ldo + j] = sum_g part[s][g][j]; mode 0 real part (double), 1 complex.

@@ -325,6 +325,102 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
 }
 
+// step_kernel for an ELL off-diagonal part of compile-time width EW
+// (a.ell == EW, F_SPMM | F_ELL set).  The column indices, values and
+// diagonal of the NEXT row a lane walks are loaded one iteration ahead, so
+// each iteration issues its row-local streams and all EW gathers at once and
+// waits for memory once, instead of an index load -> gather -> index load
+// chain per row.  Products are summed in the same order as step_kernel
+// (entries 0..EW-1, then the diagonal), so both give identical results.
+template <class VT, class MR, bool MC, class AT, int CPL, int EW>
+__global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
+    using V = cx<VT>;
+    using MV = MatVal<MR, MC>;
+    using MT = typename MV::type;
+    using PK = Pack<VT, CPL>;
+    if (a.gate && *a.gate == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lpr = 1 << a.lpr_log2;
+    const int rpw = 64 >> a.lpr_log2;
+    const int rpb = 4 * rpw;
+    const int rloc = wave * rpw + (lane >> a.lpr_log2);
+    const int cgrp = lane & (lpr - 1);
+    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+
+    const V *X = static_cast<const V *>(a.X);
+    const V *P = static_cast<const V *>(a.P);
+    const V *W = static_cast<const V *>(a.W);
+    const MT *val = static_cast<const MT *>(a.val);
+    const MT *dval = static_cast<const MT *>(a.dval);
+    const int32_t *idx = a.idx;
+    const uint32_t fl = a.flags;
+    const bool diag = fl & F_DIAG;
+    const bool need_xi = (fl & F_BETA) || (fl & F_MOM2) || diag;
+
+    cx<double> mom[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) mom[c] = {0.0, 0.0};
+
+    const int64_t nblk = (a.n + rpb - 1) / rpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+
+    int32_t jj[EW];
+    MT vv[EW];
+    MT dd{};
+    auto fetch = [&](int64_t blk, int32_t (&j)[EW], MT (&v)[EW], MT &d) {
+        const int64_t r = blk * rpb + rloc;
+        if (blk < b1 && r < a.n && colok) {
+#pragma unroll
+            for (int e = 0; e < EW; ++e) j[e] = idx[r * EW + e];
+#pragma unroll
+            for (int e = 0; e < EW; ++e) v[e] = val[r * EW + e];
+            if (diag) d = dval[r];
+        }
+    };
+    fetch(b0 + slot, jj, vv, dd);
+
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t row = blk * rpb + rloc;
+        const bool active = row < a.n && colok;
+        PK xi, pv, wv, xg[EW];
+        if (active) {
+            if (need_xi) xi.load(X + row * a.ldx + col);
+            if (fl & F_P) pv.load_nt(P + row * a.ldp + col);
+            if (fl & F_MOM) wv.load_nt(W + row * a.ldw + col);
+#pragma unroll
+            for (int e = 0; e < EW; ++e) xg[e].load(X + int64_t(jj[e]) * a.ldx + col);
+        }
+        int32_t jn[EW] = {};
+        MT vn[EW] = {};
+        MT dn{};
+        fetch(blk + per, jn, vn, dn);
+        if (active) {
+            cx<AT> y[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                y[c] = {AT(0), AT(0)};
+#pragma unroll
+                for (int e = 0; e < EW; ++e) MV::fma(y[c], vv[e], xg[e].template geta<AT>(c));
+                if (diag) MV::fma(y[c], dd, xi.template geta<AT>(c));
+            }
+            step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
+        }
+#pragma unroll
+        for (int e = 0; e < EW; ++e) {
+            jj[e] = jn[e];
+            vv[e] = vn[e];
+        }
+        dd = dn;
+    }
+
+    if (!(fl & (F_MOM | F_MOM2))) return;
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
+}
+
 // Broadcast of lane E of every 16-lane row to the whole row (DPP
 // row_newbcast, one VALU move per 32-bit word).
 template <int E, class T>

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "kmsd_internal.hpp"
 #include "kmsd_kernels.hpp"
@@ -243,7 +245,72 @@ int ilog2_ceil(int64_t v) {
     return l;
 }
 
-Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max = 4) {
+using KernelFn = void (*)(const StepArgs);
+
+// The kernel one fused step runs, from its operand kinds.
+struct KernelKey {
+    int dtype = KMSD_C128;
+    bool f32 = false;    // complex64 blocks with FP32 matrix values (FP64 row sums)
+    int cpl = 1;
+    bool cv = false;     // complex matrix values
+    bool bsr = false;
+    int ell = 0;         // ELL width with the prefetching kernel, 0 = generic kernel
+    int lpr_log2 = 6;
+};
+
+template <class VT, class MR, bool MC, class AT, int CPL>
+KernelFn pick(const KernelKey &k) {
+    if (k.ell == 3) return ell_step_kernel<VT, MR, MC, AT, CPL, 3>;
+    if (k.ell == 4) return ell_step_kernel<VT, MR, MC, AT, CPL, 4>;
+    if (k.ell == 6) return ell_step_kernel<VT, MR, MC, AT, CPL, 6>;
+    if (k.bsr && k.lpr_log2 >= 4) return bsr4_step_kernel<VT, MR, MC, AT, CPL, true>;
+    if (k.bsr) return bsr4_step_kernel<VT, MR, MC, AT, CPL, false>;
+    return step_kernel<VT, MR, MC, AT, CPL>;
+}
+
+template <class VT, class MR, class AT, int CPL>
+KernelFn pick_vk(const KernelKey &k) {
+    return k.cv ? pick<VT, MR, true, AT, CPL>(k) : pick<VT, MR, false, AT, CPL>(k);
+}
+
+KernelFn select_kernel(const kmsd_ctx *c, const KernelKey &k) {
+    if (k.dtype == KMSD_C128) return k.cpl == 2 ? pick_vk<double, double, double, 2>(k)
+                                                : pick_vk<double, double, double, 1>(k);
+    if (c->fp32_math) {
+        if (k.cpl == 4) return pick_vk<float, float, float, 4>(k);
+        return k.cpl == 2 ? pick_vk<float, float, float, 2>(k) : pick_vk<float, float, float, 1>(k);
+    }
+    if (k.f32) {
+        if (k.cpl == 4) return pick_vk<float, float, double, 4>(k);
+        return k.cpl == 2 ? pick_vk<float, float, double, 2>(k) : pick_vk<float, float, double, 1>(k);
+    }
+    if (k.cpl == 4) return pick_vk<float, double, double, 4>(k);
+    return k.cpl == 2 ? pick_vk<float, double, double, 2>(k) : pick_vk<float, double, double, 1>(k);
+}
+
+// Workgroups of `fn` (256 threads) resident per CU, from its register and
+// LDS use (cached per kernel).
+int resident_per_cu(KernelFn fn) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, int> cache;
+    const void *key = reinterpret_cast<const void *>(fn);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int blocks = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, key, 256, 0));
+    blocks = std::max(1, blocks);
+    cache.emplace(key, blocks);
+    return blocks;
+}
+
+// Launch shape and kernel of one fused step with product operator A (null:
+// no product) on n x w blocks.  Workgroups: the explicit `groups` tuning, or
+// (automatic) exactly the kernel's resident capacity, one workgroup per
+// resident slot, so the grid-stride row walk runs as a single wave of
+// workgroups with no partially filled second round.
+Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const kmsd_matrix *A, bool f32,
+                KernelFn *fn_out = nullptr) {
     Shape s;
     const int want = c->cpl ? c->cpl : (dtype == KMSD_C64 ? 2 : 1);
     s.cpl = std::min(want, std::min(cpl_max, dtype == KMSD_C64 ? 4 : 2));
@@ -255,7 +322,24 @@ Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max = 4) {
     const int64_t rpb = 4 * (64 / lpr);
     const int64_t nblk = (n + rpb - 1) / rpb;
     const int64_t want_g = ((nblk + 7) / 8) * 8;
-    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, want_g));
+    KernelKey k;
+    k.dtype = dtype;
+    k.f32 = f32;
+    k.cpl = s.cpl;
+    k.lpr_log2 = s.lpr_log2;
+    if (A) {
+        k.bsr = A->fmt == KMSD_BSR4;
+        k.cv = A->vkind == VK_COMPLEX;
+        if (!k.bsr && c->ell_kernel && (A->ell == 3 || A->ell == 4 || A->ell == 6)) k.ell = A->ell;
+    }
+    const KernelFn fn = select_kernel(c, k);
+    if (fn_out) *fn_out = fn;
+    int64_t g = c->groups;
+    if (c->auto_groups) {
+        const int64_t resident = int64_t(c->cus) * resident_per_cu(fn);
+        g = std::max<int64_t>(8, (resident / s.gy) / 8 * 8);
+    }
+    s.gx = std::max<int64_t>(8, std::min<int64_t>(g, want_g));
     return s;
 }
 
@@ -277,31 +361,16 @@ int cpl_limit(const Step &s, int dtype) {
     return lim;
 }
 
-template <class VT, class MR, bool MC, class AT, int CPL>
-void go(kmsd_ctx *c, const Shape &sh, const StepArgs &a, bool bsr) {
-    const dim3 grid(unsigned(sh.gx), unsigned(sh.gy));
-    if (bsr && a.lpr_log2 >= 4)
-        hipLaunchKernelGGL((bsr4_step_kernel<VT, MR, MC, AT, CPL, true>), grid, dim3(256), 0, c->stream, a);
-    else if (bsr)
-        hipLaunchKernelGGL((bsr4_step_kernel<VT, MR, MC, AT, CPL, false>), grid, dim3(256), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((step_kernel<VT, MR, MC, AT, CPL>), grid, dim3(256), 0, c->stream, a);
-}
-
-template <class VT, class MR, class AT, int CPL>
-void go_vk(kmsd_ctx *c, const Shape &sh, const StepArgs &a, bool complex_values, bool bsr) {
-    if (complex_values)
-        go<VT, MR, true, AT, CPL>(c, sh, a, bsr);
-    else
-        go<VT, MR, false, AT, CPL>(c, sh, a, bsr);
+bool f32_values(const kmsd_ctx *c, int dtype, const kmsd_matrix *A, bool f32_matrix) {
+    return A && dtype == KMSD_C64 && (c->fp32_math || f32_matrix);
 }
 
 int64_t blocks_for(int64_t total) { return std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192)); }
 
 }  // namespace
 
-int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap) {
-    return shape_for(c, n, w, dtype, cpl_cap).gx;
+int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap, const kmsd_matrix *A, bool f32_matrix) {
+    return shape_for(c, n, w, dtype, cpl_cap, A, f32_values(c, dtype, A, f32_matrix)).gx;
 }
 
 int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t>> blocks) {
@@ -316,15 +385,18 @@ int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t
 
 int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     if (s.n < 1 || s.w < 1) return 0;
-    const Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)));
-    if (s.expect_groups && sh.gx != s.expect_groups)
-        throw Error(KMSD_EDEVICE, "internal error: launch shape changed within a reduction chain");
+    const kmsd_matrix *A = s.A;
+    const bool f32 = f32_values(c, dtype, A, s.f32_matrix);
+    KernelFn fn = nullptr;
+    Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)), A, f32, &fn);
+    if (s.groups) {
+        if (s.groups < 8 || s.groups % 8) throw Error(KMSD_EDEVICE, "internal error: pinned workgroup count");
+        sh.gx = s.groups;
+    }
     StepArgs a{};
     uint32_t fl = 0;
     Cplx alpha = s.alpha;
-    const kmsd_matrix *A = s.A;
     const bool bsr = A && A->fmt == KMSD_BSR4;
-    const bool f32 = A && dtype == KMSD_C64 && (c->fp32_math || s.f32_matrix);
     if (A) {
         fl |= F_SPMM;
         a.ptr = A->ptr;
@@ -367,35 +439,9 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     a.lpr_log2 = sh.lpr_log2;
     a.flags = fl;
     a.gate = s.gate;
-    const bool cv = A && A->vkind == VK_COMPLEX;
-    if (dtype == KMSD_C128) {
-        if (sh.cpl == 2)
-            go_vk<double, double, double, 2>(c, sh, a, cv, bsr);
-        else
-            go_vk<double, double, double, 1>(c, sh, a, cv, bsr);
-    } else if (c->fp32_math) {
-        if (sh.cpl == 4)
-            go_vk<float, float, float, 4>(c, sh, a, cv, bsr);
-        else if (sh.cpl == 2)
-            go_vk<float, float, float, 2>(c, sh, a, cv, bsr);
-        else
-            go_vk<float, float, float, 1>(c, sh, a, cv, bsr);
-    } else if (f32) {
-        if (sh.cpl == 4)
-            go_vk<float, float, double, 4>(c, sh, a, cv, bsr);
-        else if (sh.cpl == 2)
-            go_vk<float, float, double, 2>(c, sh, a, cv, bsr);
-        else
-            go_vk<float, float, double, 1>(c, sh, a, cv, bsr);
-    } else {
-        if (sh.cpl == 4)
-            go_vk<float, double, double, 4>(c, sh, a, cv, bsr);
-        else if (sh.cpl == 2)
-            go_vk<float, double, double, 2>(c, sh, a, cv, bsr);
-        else
-            go_vk<float, double, double, 1>(c, sh, a, cv, bsr);
-    }
-    HIPCHECK(hipGetLastError());
+    void *args[] = {&a};
+    HIPCHECK(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(unsigned(sh.gx), unsigned(sh.gy)), dim3(256),
+                             args, 0, c->stream));
     return sh.gx;
 }
 

@@ -248,7 +248,7 @@ extern "C" int kmsd_metric_inner(kmsd_metric *m, const void *A, const void *B, d
         const int64_t n = m->n;
         const int32_t w = int32_t(nb);
         const int cap = cpl_cap_for(dtype, {{A, nb}, {B, nb}});
-        const int64_t G = step_groups(c, n, w, dtype, cap);
+        const int64_t G = step_groups(c, n, w, dtype, cap, m->identity ? nullptr : m->S);
         double *part = static_cast<double *>(m->pinner.ensure(c, sizeof(double) * 2 * size_t(G) * size_t(w)));
         Step s;
         if (m->identity) {
@@ -265,7 +265,7 @@ extern "C" int kmsd_metric_inner(kmsd_metric *m, const void *A, const void *B, d
         s.n = n;
         s.w = w;
         s.cpl_cap = cap;
-        s.expect_groups = G;
+        s.groups = G;
         launch_step(c, dtype, s);
         launch_reduce(c, part, G, w, 1, out, nb, 1);
         HIPCHECK(hipStreamSynchronize(c->stream));
