@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--moments", type=int, default=None, help="default 2048 (honeycomb), 1024 (nonorthogonal)")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     p.add_argument("--no-fp64-check", action="store_true")
+    p.add_argument("--no-doubling", action="store_true",
+                   help="skip the moment-doubling chains (profiles of the direct recursion only)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain-baseline", action="store_true",
                    help="skip the FP64 GPU comparison paths (fused complex128 chain, unfused torch path)")
@@ -434,6 +436,20 @@ def step_bytes(nnz, n, nb, s_val, s_vec, d=1):
     return nnz * (s_val + 4) + 4 * (n + 1) + (3 + d) * n * nb * s_vec
 
 
+def measured_traffic(key, **expect):
+    """HBM bytes per launch of the dominant kernel measured by rocprofv3
+    (profiles/traffic.json, copy-calibrated FETCH_SIZE + WRITE_SIZE) when the
+    run's workload is the profiled one, else None."""
+    try:
+        with open(ROOT / "profiles" / "traffic.json") as f:
+            rec = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    if not rec or any(rec.get(k) != v for k, v in expect.items()):
+        return None
+    return rec["traffic_bytes"]
+
+
 def cpu_threads():
     env = os.environ.get("OMP_NUM_THREADS")
     try:
@@ -554,15 +570,16 @@ def main():
 
     # Same moments from the Chebyshev doubling identities (about half the
     # orders, no projection stream): reported beside the direct recursion.
-    e2, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    mu_d = eng.moments(B, M, c, a, doubling=True)
-    e2.record(stream)
-    for _ in range(args.steps):
+    dbl_ms, dbl_orders, mu_d = None, (M - 1) // 2 + 1, None
+    if not args.no_doubling:
+        e2, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         mu_d = eng.moments(B, M, c, a, doubling=True)
-    e3.record(stream)
-    torch.cuda.synchronize()
-    dbl_ms = e2.elapsed_time(e3) / args.steps
-    dbl_orders = (M - 1) // 2 + 1
+        e2.record(stream)
+        for _ in range(args.steps):
+            mu_d = eng.moments(B, M, c, a, doubling=True)
+        e3.record(stream)
+        torch.cuda.synchronize()
+        dbl_ms = e2.elapsed_time(e3) / args.steps
 
     err = None
     if not args.no_fp64_check and args.precision == "mixed":
@@ -572,12 +589,13 @@ def main():
         from kubomsd_amd.kernels.energy import reconstruction_matrix
         G = reconstruction_matrix(np.linspace(-8.0, 8.0, 161), (lo, hi), M)
         dos64, dos32 = G @ mu64.mean(1), G @ mu32.mean(1)
-        mud = mu_d.cpu().numpy()
-        dosd = G @ mud.mean(1)
         err = dict(mu_max_rel=float(np.max(np.abs(mu32 - mu64)) / np.max(np.abs(mu64))),
-                   dos_max_rel=float(np.max(np.abs(dos32 - dos64)) / np.max(np.abs(dos64))),
-                   doubling_mu_max_rel=float(np.max(np.abs(mud - mu64)) / np.max(np.abs(mu64))),
-                   doubling_dos_max_rel=float(np.max(np.abs(dosd - dos64)) / np.max(np.abs(dos64))))
+                   dos_max_rel=float(np.max(np.abs(dos32 - dos64)) / np.max(np.abs(dos64))))
+        if mu_d is not None:
+            mud = mu_d.cpu().numpy()
+            dosd = G @ mud.mean(1)
+            err.update(doubling_mu_max_rel=float(np.max(np.abs(mud - mu64)) / np.max(np.abs(mu64))),
+                       doubling_dos_max_rel=float(np.max(np.abs(dosd - dos64)) / np.max(np.abs(dos64))))
         del B64
     # FP64 paths on the same GPU: this library's fused complex128 chain, and the
     # plain unfused path (vendor sparse product + separate kernels).
@@ -598,7 +616,7 @@ def main():
             plain = {"moments_per_s": plain_rate, "orders_timed": 256, "max_rel_diff_vs_fused_fp64": agree,
                      "fused_fp64_moments_per_s": fused64,
                      "mixed_direct_over_plain": (nb * M / (ms_per_step / 1e3)) / plain_rate,
-                     "mixed_doubling_over_plain": (nb * M / (dbl_ms / 1e3)) / plain_rate,
+                     "mixed_doubling_over_plain": (nb * M / (dbl_ms / 1e3)) / plain_rate if dbl_ms else None,
                      "path": "torch sparse CSR complex128 product + elementwise recursion + dot per order"}
         except RuntimeError as exc:  # the vendor sparse product may not take complex128
             plain = {"error": str(exc)[:200], "fused_fp64_moments_per_s": fused64}
@@ -626,14 +644,16 @@ def main():
                    "precision": args.precision, "parallelism": "single GPU", "storage": eng.H.info()["format"],
                    "launch_shape": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("ell_step_kernel_honeycomb_n2e6_nb64_c64", n=n, vectors=nb,
+                                                 precision=args.precision),
                      "algorithmic_bytes_per_launch": per_launch,
-                     "kernel": "step_kernel (fused CSR product + 3-term update + moment partials)"},
+                     "kernel": "ell_step_kernel (fused ELL product + 3-term update + moment partials)"
+                     if eng.H.info()["format"] == "csr" else "bsr4_step_kernel"},
         "moments_per_s": nb * M / (ms_per_step / 1e3),
-        "moment_doubling": {"moments_per_s": nb * M / (dbl_ms / 1e3), "ms_per_chain": dbl_ms,
-                            "orders": dbl_orders,
-                            "achieved_GBps": step_bytes(model.H.nnz, n, nb, s_val, s_vec, d=0) * dbl_orders
-                            / (dbl_ms / 1e3) / 1e9},
+        "moment_doubling": None if dbl_ms is None else
+        {"moments_per_s": nb * M / (dbl_ms / 1e3), "ms_per_chain": dbl_ms, "orders": dbl_orders,
+         "achieved_GBps": step_bytes(model.H.nnz, n, nb, s_val, s_vec, d=0) * dbl_orders / (dbl_ms / 1e3) / 1e9},
         "err_vs_fp64": err,
         "fp64_gpu_baselines": plain,
         "cpu_baseline": cpu,
