@@ -214,6 +214,46 @@ int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, 
 int kmsd_mu_table(kmsd_ctx *ctx, const void *left, const void *right, void *table, int64_t n, int64_t m,
                   int64_t k, int64_t nb, int dtype, int mode, int accumulate);
 
+/* ---- statistics of the stochastic estimator ------------------------------ */
+/* Reference: the per-sample accumulation loops of solvers/stochastic.py:60-71
+ * (dos_blocks[block] += dh[local]; accum[key][block] += vh[local], samples in
+ * order; GPU edition solvers/stochastic_gpu.py:88-93).  Device FP64 buffers:
+ * dst[e] += src[l*ld_src + e] for l = 0 .. nloc-1 in that order, e < nelem;
+ * if any src value is not finite, flag_bits are OR-ed into the device int32
+ * *flags (the caller clears and reads it).  Stream-ordered, asynchronous. */
+int kmsd_block_accumulate(kmsd_ctx *ctx, double *dst, const double *src, int64_t nloc, int64_t nelem,
+                          int64_t ld_src, int32_t *flags, int32_t flag_bits);
+/* Reference: msd_gradient (observables/diffusion.py:4-6):
+ * dst = scale * numpy.gradient(src, time, axis=1, edge_order=2 if nt > 2 else 1)
+ * for device FP64 src/dst of shape (outer, nt, inner); time: host, nt >= 2
+ * strictly increasing values.  numpy's formulas and operation order (uniform
+ * spacing when all differences are equal), so results equal numpy's bit for
+ * bit.  Synchronous. */
+int kmsd_time_gradient(kmsd_ctx *ctx, const double *src, double *dst, int64_t outer, int64_t nt, int64_t inner,
+                       const double *time, double scale);
+/* Reference: estimate_ratio (observables/statistics.py:5-23; GPU edition
+ * observables/statistics_gpu.py:6-32), paired block bootstrap of sums.
+ * A: device (nblocks, nelem) block sums, element e paired with column e % ne
+ * of D: device (nblocks, dcols), dcols = ne or 1 (one denominator per block).
+ * value[e] = sum_b A / sum_b D (NaN when that is 0); stderr_out (optional,
+ * needs nblocks >= 2): standard deviation (ddof 1) over the nres resampled
+ * ratios with weights (device, nres x nblocks: the counts of each block in
+ * each resample, drawn on the host as the reference does); NaN when any
+ * ratio is undefined.  valid (optional, device, ne bytes): value and stderr
+ * are NaN where valid[e % ne] == 0.  total (optional): sum_b A.  The block
+ * sums run in block order (value and total equal numpy's bit for bit); the
+ * resampled sums differ from a BLAS product only by rounding.  Asynchronous. */
+int kmsd_bootstrap_ratio(kmsd_ctx *ctx, const double *A, const double *D, int64_t nblocks, int64_t nelem, int64_t ne,
+                         int64_t dcols, const double *weights, int64_t nres, const uint8_t *valid, double *value,
+                         double *stderr_out, double *total);
+/* Reference: the DOS validity masks of _normalize_dos (solvers/stochastic.py:
+ * 78-87): valid = finite(rho) & rho > threshold & !(stderr not finite or
+ * rho - 1.96 stderr <= 0); reason = LOW_DOS (1) | UNCERTAIN_TRACE (2) |
+ * INSUFFICIENT_BLOCKS (32, nblocks < 8).  stderr_in may be NULL (one block).
+ * Device buffers of ne entries.  Asynchronous. */
+int kmsd_dos_validity(kmsd_ctx *ctx, const double *rho, const double *stderr_in, int64_t ne, double threshold,
+                      int64_t nblocks, uint8_t *valid, uint32_t *reason);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,10 @@ SIGNATURES = {
     "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments_doubling": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_mu_table": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I],
+    "kmsd_block_accumulate": [_P, _P, _P, _I64, _I64, _I64, _P, C.c_int32],
+    "kmsd_time_gradient": [_P, _P, _P, _I64, _I64, _I64, _P, _D],
+    "kmsd_bootstrap_ratio": [_P, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P],
+    "kmsd_dos_validity": [_P, _P, _P, _I64, _D, _I64, _P, _P],
 }
 
 

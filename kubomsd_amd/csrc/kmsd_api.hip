@@ -56,6 +56,7 @@ extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->flag) (void)hipFree(c->flag);
     if (c->mu_part) (void)hipFree(c->mu_part);
+    if (c->stats_tab) (void)hipFree(c->stats_tab);
     if (c->capture) (void)hipStreamDestroy(c->capture);
     delete c;
     return KMSD_OK;

@@ -27,6 +27,8 @@ struct kmsd_ctx {
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
     size_t mu_part_bytes = 0;
+    void *stats_tab = nullptr;  // coefficient table of kmsd_time_gradient (grow-only)
+    size_t stats_tab_bytes = 0;
 };
 
 // Value kinds of a stored matrix.

@@ -3,10 +3,13 @@
 batch admission as in the GPU edition's solvers/stochastic_gpu.py:18-145).
 
 Samples are taken in the global order of statistics blocks and batches, so
-results do not depend on the batch width; DOS and numerators are summed per
-block on the host and normalized by the paired block bootstrap.  A batch whose
-device workspace does not fit the HBM budget is halved until it does
-(never below one sample, else RESOURCE_BLOCKED).
+results do not depend on the batch width.  DOS and numerators stay on the
+device: each batch's per-sample arrays are added into device-resident block
+sums in sample order, the MSD time gradient, the paired block bootstrap and
+the validity masks run as device kernels (observables/statistics_device.py),
+and only the finished arrays of each output time slab are copied to the host.
+A batch whose device workspace does not fit the HBM budget is halved until it
+does (never below one sample, else RESOURCE_BLOCKED).
 """
 from dataclasses import replace
 import time
@@ -18,8 +21,7 @@ from ..errors import NumericalError, ResourceBlocked
 from ..kernels.energy import dos_floor
 from ..metric.device_bounds import estimate_spectral_bounds_device
 from ..metric.sampling import sample_metric
-from ..observables.diffusion import msd_gradient
-from ..observables.statistics import estimate_ratio
+from ..observables.statistics_device import FLAG_DOS, FLAG_NUMERATOR, DeviceStatistics
 from .spectral import spectral_samples, spectral_workspace_bytes
 from .stepwise import stepwise_samples
 
@@ -62,7 +64,7 @@ def _host(x):
     return x.detach().cpu().numpy() if hasattr(x, "detach") else np.asarray(x)
 
 
-def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch):
+def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     diagnostics = []
     primary = "commutator" if method == "stepwise" else "vac_integral"
     ns = cfg.nsamples if samples is None else samples.shape[1]
@@ -83,41 +85,32 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch):
                     diag["additional_cost_recorded"] = True
             else:
                 dos, nums, diag = spectral_samples(ops, cfg, B)
-            nums = {k: _host(v) for k, v in nums.items()}
             if len(cfg.time_fs) > 1:
-                nums["diffusion_A2_per_fs/msd_gradient"] = msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
-                                                                        axis=2)
+                nums["diffusion_A2_per_fs/msd_gradient"] = stats.msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
+                                                                              axis=2)
             diagnostics.append(diag)
-            dh = _host(dos)
-            if not np.isfinite(dh).all():
-                raise NumericalError("nonfinite stochastic DOS")
-            for local in range(len(ids)):
-                dos_blocks[block] += dh[local]
-            for key, vh in nums.items():
-                if not np.isfinite(vh).all():
-                    raise NumericalError("nonfinite stochastic numerator")
-                for local in range(len(ids)):
-                    accum[key][block] += vh[local]
+            # Per-sample sums into the block, in sample order (the reference's
+            # loop of += per sample), with the nonfinite checks as flags.
+            stats.clear_flags()
+            stats.accumulate(dos_blocks[block], dos, FLAG_DOS)
+            for key, value in nums.items():
+                stats.accumulate(accum[key][block], value, FLAG_NUMERATOR)
+            stats.check_flags()
     return diagnostics
 
 
-def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics, batch):
+def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics, batch, stats):
     nbk = len(counts)
     primary = "commutator" if method == "stepwise" else "vac_integral"
-    rho, stderr_dos, stat = estimate_ratio(dos_blocks, counts[:, None], counts)
+    import torch
+    dcounts = torch.from_numpy(counts[:, None].astype(float)).to(dos_blocks.device)
+    rho, stderr_dos, stat = stats.estimate_ratio(dos_blocks, dcounts, counts)
     threshold = dos_floor(ops.n, cfg.spectral_bounds[1] - cfg.spectral_bounds[0], cfg)
-    valid = np.isfinite(rho) & (rho > threshold)
-    reason = np.where(valid, 0, int(Reason.LOW_DOS)).astype("uint32")
-    if stderr_dos is not None:
-        uncertain = ~np.isfinite(stderr_dos) | (rho - 1.96 * stderr_dos <= 0)
-        valid &= ~uncertain
-        reason[uncertain] |= int(Reason.UNCERTAIN_TRACE)
-    if nbk < 8:
-        reason |= int(Reason.INSUFFICIENT_BLOCKS)
-    static = {"dos_states_per_eV": rho, "dos_block_sums": dos_blocks, "block_counts": counts,
+    valid, reason = stats.dos_validity(rho, stderr_dos, threshold, nbk)
+    static = {"dos_states_per_eV": _host(rho), "dos_block_sums": _host(dos_blocks), "block_counts": counts,
               "sample_count": np.array(ns), "sample_ids": np.arange(ns)}
     if stderr_dos is not None:
-        static["stderr_dos_states_per_eV"] = stderr_dos
+        static["stderr_dos_states_per_eV"] = _host(stderr_dos)
     metadata = {
         "primary_estimator": primary,
         "primary_diffusion_estimator": "msd_gradient" if method == "stepwise" else "vac_integral",
@@ -127,19 +120,20 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
         "sampling": getattr(getattr(ops, "sampler", None), "report", {"source": "caller supplied samples"}),
         "sample_batch_diagnostics": diagnostics, "admitted_sample_batch": batch,
         "raw_accumulation": "numerator_* are means over nsamples; *_block_sums are sums",
+        "statistics_location": "device-resident block sums and bootstrap",
         "backend": {"name": "hip", "precision": cfg.precision, "memory": ops.ctx.memory(),
                     "metric": ops.metric.diagnostics(), "h_action_calls": ops.h_calls},
     }
-    return valid, reason, static, metadata
+    return valid, _host(reason).astype("uint32"), static, metadata
 
 
-def _normalize_slab(raw, dos_blocks, counts, ns, valid, key):
-    value, se, _ = estimate_ratio(raw, dos_blocks, counts)
-    value[..., ~valid] = np.nan
-    arrays = {key: value, "numerator_" + key: raw.sum(0) / ns, "numerator_block_sums/" + key: raw.copy()}
+def _normalize_slab(raw, dos_blocks, counts, ns, valid, key, stats):
+    raw = raw.contiguous()
+    value, se, _, total = stats.estimate_ratio(raw, dos_blocks, counts, valid=valid, want_total=True)
+    arrays = {key: _host(value), "numerator_" + key: _host(total) / ns,
+              "numerator_block_sums/" + key: _host(raw)}
     if se is not None:
-        se[..., ~valid] = np.nan
-        arrays["stderr_" + key] = se
+        arrays["stderr_" + key] = _host(se)
     return arrays
 
 
@@ -157,26 +151,31 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     counts = np.array([min(cfg.statistics_block, ns - i * cfg.statistics_block) for i in range(nbk)])
     keys = _quantity_keys(cfg, method)
     shape = (nbk, len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV))
-    accum = {key: np.zeros(shape) for key in keys}
-    dos_blocks = np.zeros((nbk, len(cfg.energy_eV)))
+    import torch
+    dev = ops.ctx.torch_device
+    accum = {key: torch.zeros(shape, dtype=torch.float64, device=dev) for key in keys}
+    dos_blocks = torch.zeros((nbk, len(cfg.energy_eV)), dtype=torch.float64, device=dev)
+    stats = DeviceStatistics(ops.ctx)
     batch = admitted_batch(ops, cfg, method)
     t0 = time.perf_counter()
-    diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch)
+    diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats)
     seconds = time.perf_counter() - t0
     valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
-                                                     diagnostics, batch)
+                                                     diagnostics, batch, stats)
     metadata["sample_seconds"] = seconds
+    valid_host = _host(valid)
     for start in range(0, len(cfg.time_fs), cfg.time_block):
         stop = min(start + cfg.time_block, len(cfg.time_fs))
         arrays = {}
         for key in keys:
-            arrays.update(_normalize_slab(accum[key][:, :, start:stop, :], dos_blocks, counts, ns, valid, key))
+            arrays.update(_normalize_slab(accum[key][:, :, start:stop, :], dos_blocks, counts, ns, valid, key,
+                                          stats))
         for estimator in {k.split("/")[-1] for k in keys}:
             r = reason.copy()
             if estimator == "vac_integral":
                 r |= int(Reason.INTEGRATION_UNVERIFIED)
             dims = (len(cfg.axes), stop - start, len(cfg.energy_eV))
-            arrays["valid/" + estimator] = np.broadcast_to(valid, dims).copy()
+            arrays["valid/" + estimator] = np.broadcast_to(valid_host, dims).copy()
             arrays["reason_code/" + estimator] = np.broadcast_to(r, dims).copy()
         yield TransportBlock(start, cfg.time_fs[start:stop], arrays, static if start == 0 else {},
                              metadata if start == 0 else {})
