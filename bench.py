@@ -198,8 +198,9 @@ def nonorthogonal(args):
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for _ in range(args.steps):
-        mu = eng.moments(B, M, c, a)
+    for step in range(args.steps):
+        mu = eng.moments(B, M, c, a)  # returns after the chain's final reduction
+        print(f"# chain {step + 1}/{args.steps} done", file=sys.stderr, flush=True)
     e1.record(stream)
     torch.cuda.synchronize()
     d1 = metric.diagnostics()
