@@ -62,7 +62,9 @@ def parse():
     p.add_argument("--n-max", type=float, default=4e5, help="largest job size (orbitals); reference: 4e6")
     p.add_argument("--slots", type=int, default=6)
     p.add_argument("--check-jobs", type=int, default=6, help="jobs re-run alone in FP64 for the D(E) error")
-    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu"], default="honeycomb",
+    p.add_argument("--tmax", type=float, default=5.0, help="stepwise: last output time (fs); reference: 50")
+    p.add_argument("--dt", type=float, default=1.0, help="stepwise: output time step (fs)")
+    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise"], default="honeycomb",
                    help="honeycomb: orthogonal moment chain (default); nonorthogonal: simple-cubic 64^3 x 4 "
                         "orbitals, S^-1 H chain with batched block-Jacobi PCG; mu: square 2048^2 double-moment "
                         "tables mu_mn on the matrix cores")
@@ -74,7 +76,7 @@ def parse():
     a = p.parse_args()
     nonorth = a.workload == "nonorthogonal"
     a.vectors = a.vectors or (32 if nonorth else 64)
-    a.moments = a.moments or (1024 if a.workload != "honeycomb" else 2048)
+    a.moments = a.moments or (2048 if a.workload in ("honeycomb", "stepwise") else 1024)
     return a
 
 
@@ -340,6 +342,104 @@ def double_moments(args):
     print(json.dumps(line))
 
 
+def stepwise(args):
+    """The 'Batched FP32/FP64-accum' configuration through the whole stepwise
+    estimator (iter_stochastic_blocks, method 'stepwise'): honeycomb 1000 x
+    1000 cells, 64 random-phase vectors in one batch (8 statistics blocks of
+    8), M = 2048 Jackson moments, axes x and y, output times 0 .. tmax fs
+    every dt fs, 161 energies on [-8, 8] eV, bounds (-8.8, 8.8) eV.  Runs
+    once with complex64 vectors (mixed) and once in complex128 (fp64) on the
+    same samples, and reports the estimator's vector-steps/s and the relative
+    error of D(E, t) (diffusion_A2_per_fs/msd_gradient), MSD(E, t) and
+    DOS(E) of the mixed run against the FP64 run."""
+    import torch
+    from kubomsd_amd.backends.hip import HipContext
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.metric.bounds import certified_bounds
+    from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
+    from kubomsd_amd.synthetic import honeycomb_lattice
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    ctx = HipContext(0)
+    t_build = time.perf_counter()
+    model = honeycomb_lattice(args.cells, t=-2.7, W=1.0, a=1.42)
+    n, nb, M = model.n, args.vectors, args.moments
+    E = np.linspace(-8.0, 8.0, 161)
+    T = np.round(np.arange(0.0, args.tmax + 0.5 * args.dt, args.dt), 12)
+    samples = random_phases(n, SEED, range(nb))
+    bundle = ModelBundle(model.H, model.S, velocity=model.velocity)
+    print(f"# setup {time.perf_counter() - t_build:.1f}s: N={n} nb={nb} M={M} times={len(T)} axes=x,y",
+          file=sys.stderr, flush=True)
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method="stepwise",
+                        energy_eV=E, time_fs=T, axes=("x", "y"), spectral_bounds=(-8.8, 8.8), n_moments=M,
+                        nsamples=nb, sample_batch=nb, statistics_block=max(1, nb // 8), include_vac=False,
+                        precision=prec, segment_fs=args.dt, autotune=prec == "mixed")
+        ops = build_operators(bundle, cfg, ctx)
+        # The explicit bounds are checked against Gershgorin discs (a proof of
+        # containment); a Lanczos Ritz check to 1e-10 residuals does not
+        # converge in 512 steps on the dense band edges of a 2M-site lattice.
+        report = certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower, spectral_bounds=cfg.spectral_bounds)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        blocks = list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=samples, bounds_report=report))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cat = lambda key: np.concatenate([b.arrays[key] for b in blocks], axis=1)
+        diag = blocks[0].metadata["sample_batch_diagnostics"][0]
+        runs[prec] = dict(ms=e0.elapsed_time(e1), wall=wall, D=cat("diffusion_A2_per_fs/msd_gradient"),
+                          msd=cat("msd_A2/commutator"), dos=blocks[0].static["dos_states_per_eV"],
+                          valid=blocks[0].arrays["valid/commutator"][0, 0], order=diag["max_propagation_order"],
+                          segments=diag["segments"], tuning=ops.tuning)
+        print(f"# {prec}: {runs[prec]['ms'] / 1e3:.1f} s on HIP events ({wall:.1f} s wall), "
+              f"propagation order {diag['max_propagation_order']}", file=sys.stderr, flush=True)
+        del ops, blocks
+    mx, f64 = runs["mixed"], runs["fp64"]
+
+    def rel(a, b):
+        ok = np.isfinite(a) & np.isfinite(b)
+        return float(np.max(np.abs(a[ok] - b[ok])) / np.max(np.abs(b[ok])))
+
+    na, nt = 2, len(T)
+    # Column-steps: the DOS chain, then per output interval the augmented
+    # series (order K, one segment per interval, width nb(1+na)) and the
+    # moment chain of both axis blocks (width na*nb).
+    chain_steps = (M - 1) * nb + (nt - 1) * (M - 1) * na * nb
+    series_steps = mx["segments"] * mx["order"] * nb * (1 + na)
+    vsteps = chain_steps + series_steps
+    value = vsteps / (mx["ms"] / 1e3)
+    chain_bytes = (M - 1) * step_bytes(model.H.nnz, n, nb, 8, 8) + \
+        (nt - 1) * (M - 1) * step_bytes(model.H.nnz, n, na * nb, 8, 8)
+    achieved = chain_bytes / (mx["ms"] / 1e3) / 1e9
+    line = {
+        "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
+        "value": value, "unit": "vector-steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": mx["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64+f64", "data": "synthetic (seeded honeycomb lattice, NumPy PCG64 random phases)",
+        "config": {"workload": f"stepwise MSD: honeycomb {args.cells}x{args.cells} cells (N={n}), {nb} vectors, "
+                               f"M={M}, axes x,y, t=0..{args.tmax:g} fs every {args.dt:g} fs, 161 energies",
+                   "n": n, "vectors": nb, "moments": M, "times": nt, "precision": "mixed",
+                   "parallelism": "single GPU", "launch_shape": mx["tuning"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "whole estimator: algorithmic bytes of its moment-chain steps over its event time "
+                               "(series steps and statistics counted in the time only)"},
+        "vector_steps": {"moment_chains": chain_steps, "series": series_steps, "propagation_order": mx["order"]},
+        "err_vs_fp64": {"D_E_t_max_rel": rel(mx["D"], f64["D"]), "msd_max_rel": rel(mx["msd"], f64["msd"]),
+                        "dos_max_rel": rel(mx["dos"], f64["dos"]),
+                        "valid_masks_equal": bool(np.array_equal(mx["valid"], f64["valid"]))},
+        "fp64_run": {"ms": f64["ms"], "vector_steps_per_s": vsteps / (f64["ms"] / 1e3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def mu_errors(ctx, eng, Vd, r, A, Bm, mu32, mub, M, c, a):
     """Max-entry relative errors of the FP32 and bf16x3 tables against FP64:
     (1) the complex128 GEMM of the same complex64 chains, over N in chunks;
@@ -516,6 +616,11 @@ def main():
         if args.gpus != 1:
             sys.exit("this build runs on one GPU only")
         nonorthogonal(args)
+        return
+    if args.workload == "stepwise":
+        if args.gpus != 1:
+            sys.exit("this build runs on one GPU only")
+        stepwise(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sample sharding is a design "

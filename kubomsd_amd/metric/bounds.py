@@ -81,11 +81,13 @@ def estimate_spectral_bounds(host_H, host_S, s_lower, s_certified, spectral_boun
             "conservative_bounds": (-b, b), "extrema": [lo, hi]}
 
 
-def certified_bounds(host_H, host_S, s_lower, margin=1e-3):
+def certified_bounds(host_H, host_S, s_lower, margin=1e-3, spectral_bounds=None):
     """Bounds that provably contain the generalized spectrum of (H, S) without
     an eigensolver: Gershgorin discs of H when S = I, otherwise
     +-||H||_inf / lambda_min(S) (the reference's conservative bound,
-    bounds.py:28-32), widened by a relative margin."""
+    bounds.py:28-32), widened by a relative margin.  spectral_bounds: user
+    bounds, returned as given when they enclose those certified bounds
+    (else NumericalError)."""
     H = sp.csr_matrix(host_H)
     S = sp.csr_matrix(host_S)
     n = H.shape[0]
@@ -101,5 +103,12 @@ def certified_bounds(host_H, host_S, s_lower, margin=1e-3):
         source = "||H||_inf / lambda_min(S)"
     w = max(hi - lo, 1e-12)
     bounds = (lo - margin * w, hi + margin * w)
-    return {"bounds": bounds, "source": source, "certified": True, "residuals": [], "conservative_bounds": bounds,
-            "extrema": [lo, hi]}
+    conservative = bounds
+    if spectral_bounds is not None:
+        user = tuple(float(v) for v in spectral_bounds)
+        if not (user[0] < lo and user[1] > hi):
+            raise NumericalError("requested spectral bounds do not enclose the certified spectrum bounds")
+        bounds = user
+        source = "user bounds enclosing " + source
+    return {"bounds": bounds, "source": source, "certified": True, "residuals": [],
+            "conservative_bounds": conservative, "extrema": [lo, hi]}
