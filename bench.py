@@ -361,6 +361,8 @@ def stepwise(args):
     from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
     from kubomsd_amd.synthetic import honeycomb_lattice
     assert torch.cuda.is_available(), "bench needs a HIP device"
+    import logging
+    logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="# %(message)s")
     ctx = HipContext(0)
     t_build = time.perf_counter()
     model = honeycomb_lattice(args.cells, t=-2.7, W=1.0, a=1.42)

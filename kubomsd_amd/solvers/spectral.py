@@ -61,8 +61,10 @@ def spectral_samples(ops, cfg, samples):
     times = np.unique(np.r_[resolve_time_grid(cfg.time_fs[-1], cfg.integration_dt_fs), cfg.time_fs])
     output_indices = {float(t): j for j, t in enumerate(cfg.time_fs)}
     G = reconstruction(ops, cfg)  # (ne, L)
-    c_anchor = samples
-    v_anchor = [ops.apply_v_action(a, samples) for a in cfg.axes]
+    # Anchor states are propagated in complex128 in both precisions; the
+    # chains of each segment run in the precision's storage type.
+    c_anchor = samples.to(torch.complex128)
+    v_anchor = [ops.apply_v_action(a, c_anchor) for a in cfg.axes]
     anchor, next_anchor = 0., -1.
     tables = None
     maxorder = segments = 0
@@ -91,11 +93,11 @@ def spectral_samples(ops, cfg, samples):
             if need > mem["budget_bytes"] - mem["used_bytes"]:
                 raise NumericalError("spectral workspace exceeds memory budget; reduce sample_batch/segment_fs or "
                                      "increase explicit budget")
-            left = chain(c_anchor, ops, cfg, L + K)  # (L+K, n, nb)
+            left = chain(c_anchor.to(samples.dtype), ops, cfg, L + K)  # (L+K, n, nb)
             tables = []
             for ai, axis in enumerate(cfg.axes):
                 # cov = V chain(S^-1 V c, K+1): chain and one velocity product over all orders
-                cov = ops.apply_v_chain(axis, v_anchor[ai], K + 1, scaling(cfg))  # (K+1, n, nb)
+                cov = ops.apply_v_chain(axis, v_anchor[ai].to(samples.dtype), K + 1, scaling(cfg))  # (K+1, n, nb)
                 tables.append(_table(ops, left, cov))  # (nb, L+K, K+1)
                 del cov
             del left
