@@ -1,17 +1,24 @@
 """Continuous-batching transport service on one MI355X.
 
 Requests (transport jobs of different sizes) arrive over time.  The server
-keeps ``slots`` workers, each with its own HIP stream and kmsd context; a job is
-admitted when a slot is free AND its estimated device footprint fits the HBM
-budget left by the jobs already on the device.  Admitted jobs run concurrently:
-their kernels interleave on the GPU through the per-slot streams, so small
-jobs that cannot fill 256 CUs alone overlap with each other and with large
-ones.  Waiting jobs are kept in size classes (by estimated bytes of one
-Chebyshev step); the scheduler takes the oldest waiting job whose footprint
-fits, scanning classes smallest-first when the oldest does not fit, so a large
-job never blocks small ones that fit (and is guaranteed to run once enough of
-the budget drains, because no new work is admitted past a job that has waited
-longer than ``max_wait_s``).
+keeps ``slots`` worker PROCESSES, each with its own HIP context, stream and
+kmsd context on the device; a job is admitted when a slot is free AND its
+estimated device footprint fits the HBM budget left by the jobs already on the
+device.  Admitted jobs run concurrently: their kernels interleave on the GPU
+through the processes' own hardware queues, so small jobs that cannot fill
+256 CUs alone overlap with each other and with large ones, and each slot's
+host-side orchestration runs in its own interpreter (no shared interpreter
+lock between slots).  Waiting jobs are kept in size classes (by estimated
+bytes of one Chebyshev step); the scheduler takes the oldest waiting job
+whose footprint fits, scanning classes smallest-first when the oldest does not
+fit, so a large job never blocks small ones that fit (and is guaranteed to run
+once enough of the budget drains, because no new work is admitted past a job
+that has waited longer than ``max_wait_s``).
+
+The parent process only schedules: one dispatcher thread per slot takes an
+admitted job, sends it to its worker process over a pipe and waits for the
+result; it makes no device calls.  A worker that dies is reported as the
+failure of the job it held and is replaced.
 
 Every kernel on the path reduces deterministically (fixed-order partial sums,
 launch shapes fixed by the job alone: jobs run with the default shapes, no
@@ -78,11 +85,11 @@ def estimate_job_bytes(n, nnz, nb, na, precision, nonorthogonal, n_velocity=1):
 class TransportServer:
     def __init__(self, device=0, slots=4, hbm_budget_bytes=None, class_edges=(1 << 26, 1 << 30),
                  max_wait_s=30.0, keep_blocks=False):
-        import torch
-        self.torch = torch
         self.device = device
         self.slots = slots
+        self.procs = []
         if hbm_budget_bytes is None:
+            import torch
             free, _ = torch.cuda.mem_get_info(device)
             hbm_budget_bytes = int(0.85 * free)
         self.budget = int(hbm_budget_bytes)
@@ -156,35 +163,102 @@ class TransportServer:
         return time.perf_counter() - self.t0
 
     # -- execution -------------------------------------------------------------
-    def _worker(self, slot):
-        torch = self.torch
-        from .backends.hip import HipContext
-        torch.cuda.set_device(self.device)
-        stream = torch.cuda.Stream(device=self.device)  # default priority
-        with torch.cuda.stream(stream):
-            ctx = HipContext(self.device, hbm_budget_bytes=self.budget)
-            while True:
-                job = self._take()
-                if job is None:
-                    return
-                start = self.clock()
-                try:
-                    res = run_job(job, ctx, self.keep_blocks)
-                    err = None
-                except Exception as exc:  # reported per job, the server keeps serving
-                    res, err = {}, f"{type(exc).__name__}: {exc}"
-                finish = self.clock()
-                self._release(job)
-                self.results[job.job_id] = JobResult(job.job_id, job.arrival_s, start, finish, slot,
-                                                     res.get("dos"), res.get("diffusion"), res.get("blocks"), err)
+    MAX_SLOTS = 15  # worker processes on the device, plus this one
+
+    def _launch(self, slot):
+        import multiprocessing as mp
+        ctx = mp.get_context("spawn")  # a fresh interpreter: no device state is inherited
+        parent, child = ctx.Pipe()
+        proc = ctx.Process(target=_worker_main, args=(child, self.device, self.budget, self.keep_blocks),
+                           name=f"kubomsd-slot-{slot}", daemon=True)
+        proc.start()
+        child.close()
+        return proc, parent
+
+    @staticmethod
+    def _ready(slot, proc, conn):
+        try:
+            ok = conn.recv() == "ready"  # the worker's device context exists
+        except EOFError:
+            ok = False
+        if not ok:
+            proc.join(5)
+            raise RuntimeError(f"serving slot {slot} failed to start (exit code {proc.exitcode})")
+
+    def _spawn(self, slot):
+        proc, conn = self._launch(slot)
+        self._ready(slot, proc, conn)
+        return proc, conn
+
+    def start(self):
+        """Starts the worker processes, all at once (once; run() calls it)."""
+        if self.procs:
+            return
+        if not 1 <= self.slots <= self.MAX_SLOTS:
+            raise ValueError(f"slots must be in [1, {self.MAX_SLOTS}]")
+        procs = [self._launch(s) for s in range(self.slots)]
+        try:
+            for s, (proc, conn) in enumerate(procs):
+                self._ready(s, proc, conn)
+        except Exception:
+            self.procs = procs
+            self.close()
+            raise
+        self.procs = procs
+
+    def close(self):
+        """Stops the worker processes."""
+        for proc, conn in self.procs:
+            try:
+                conn.send(None)
+            except (OSError, ValueError):
+                pass
+        for proc, conn in self.procs:
+            proc.join(30)
+            if proc.is_alive():
+                proc.kill()
+                proc.join()
+            conn.close()
+        self.procs = []
+
+    def __enter__(self):
+        self.start()
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _dispatch(self, slot):
+        while True:
+            job = self._take()
+            if job is None:
+                return
+            proc, conn = self.procs[slot]
+            start = self.clock()
+            try:
+                conn.send(job)
+                res, err = conn.recv()
+            except (EOFError, OSError) as exc:
+                proc.join(5)
+                res, err = {}, f"worker process ended ({type(exc).__name__}, exit code {proc.exitcode})"
+                conn.close()
+                self.procs[slot] = self._spawn(slot)
+            finish = self.clock()
+            self._release(job)
+            self.results[job.job_id] = JobResult(job.job_id, job.arrival_s, start, finish, slot,
+                                                 res.get("dos"), res.get("diffusion"), res.get("blocks"), err)
 
     def run(self, jobs, realtime=True):
         """Feeds jobs at their arrival times (seconds from start) and returns
-        {job_id: JobResult} once all have finished."""
+        {job_id: JobResult} once all have finished.  The worker processes
+        are started first (not timed) and kept for later runs until close()."""
+        self.start()
+        with self.cv:
+            self.closed = False
         self.t0 = time.perf_counter()
-        workers = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in range(self.slots)]
-        for w in workers:
-            w.start()
+        threads = [threading.Thread(target=self._dispatch, args=(s,), daemon=True) for s in range(self.slots)]
+        for t in threads:
+            t.start()
         for job in sorted(jobs, key=lambda j: j.arrival_s):
             if realtime:
                 delay = job.arrival_s - self.clock()
@@ -194,8 +268,8 @@ class TransportServer:
         with self.cv:
             self.closed = True
             self.cv.notify_all()
-        for w in workers:
-            w.join()
+        for t in threads:
+            t.join()
         return self.results
 
     @staticmethod
@@ -207,6 +281,40 @@ class TransportServer:
                     p50_latency_s=float(np.percentile(lat, 50)) if len(lat) else None,
                     p99_latency_s=float(np.percentile(lat, 99)) if len(lat) else None,
                     mean_latency_s=float(lat.mean()) if len(lat) else None)
+
+
+def _worker_main(conn, device, budget, keep_blocks):
+    """One serving slot: a process with its own HIP context, running the jobs
+    it receives one at a time (None ends it)."""
+    import torch
+    from .backends.hip import HipContext
+    torch.cuda.set_device(device)
+    ctx = HipContext(device, hbm_budget_bytes=budget)
+    conn.send("ready")
+    while True:
+        job = conn.recv()
+        if job is None:
+            break
+        try:
+            res, err = run_job(job, ctx, keep_blocks), None
+        except Exception as exc:  # reported per job, the slot keeps serving
+            res, err = {}, f"{type(exc).__name__}: {exc}"
+        conn.send((res, err))
+    conn.close()
+
+
+class FlakeModel:
+    """Builds one synthetic serving job's model on demand (a picklable
+    callable, so the job travels to a worker process as a few numbers)."""
+
+    def __init__(self, n_sites, W, nonorthogonal, seed):
+        self.n_sites, self.W, self.nonorthogonal, self.seed = n_sites, W, nonorthogonal, seed
+
+    def __call__(self):
+        from .contracts import ModelBundle
+        from .synthetic import honeycomb_flake
+        m = honeycomb_flake(self.n_sites, W=self.W, s_scale=0.08 if self.nonorthogonal else 0.0, seed=self.seed)
+        return ModelBundle(m.H, m.S, velocity={"x": m.velocity["x"]})
 
 
 def run_job(job, ctx, keep_blocks=False):
@@ -238,8 +346,6 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
     the given sets, a fraction with a nonorthogonal overlap; each job is a
     stepwise transport run (DOS + MSD + D(E) at `times`) on one batch."""
     from .config import RunConfig
-    from .contracts import ModelBundle
-    from .synthetic import honeycomb_flake
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, n_jobs))
     arrivals -= arrivals[0]
@@ -253,9 +359,7 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
         L = max(2, int(round(np.sqrt(n / 2))))
         n_act = 2 * L * L
 
-        def build(L=L, W=W, nonorth=nonorth, i=i):
-            m = honeycomb_flake(2 * L * L, W=W, s_scale=0.08 if nonorth else 0.0, seed=seed + 17 * i)
-            return ModelBundle(m.H, m.S, velocity={"x": m.velocity["x"]})
+        build = FlakeModel(2 * L * L, W, nonorth, seed + 17 * i)
         cfg = RunConfig(energy_eV=np.linspace(-7.5, 7.5, 61), time_fs=list(times), solver="stochastic-kpm",
                         kernel="jackson", backend="hip", n_moments=M, axes=("x",), nsamples=nb, sample_batch=nb,
                         statistics_block=max(1, nb // 4), include_vac=False, precision=precision,

@@ -1,6 +1,7 @@
 """Continuous batching: jobs admitted by slots and HBM budget overlap on the
-device, and every job's result is bitwise identical alone and in the mix
-(deterministic reductions); mixed-precision D(E) stays near the FP64 run."""
+device (one worker process per slot), and every job's result is bitwise
+identical alone and in the mix (deterministic reductions); mixed-precision
+D(E) stays near the FP64 run."""
 from dataclasses import replace
 
 import numpy as np
@@ -16,37 +17,32 @@ def trace(precision="mixed"):
                            nonorthogonal_fraction=0.34, precision=precision, times=(0.0, 0.5, 1.0), seed=11)
 
 
-CONCURRENT_SLOTS_OFF = pytest.mark.skip(
-    reason="multi-slot serving (several worker threads in one process) ended in a host segmentation "
-           "fault in one GPU run; its cause has not been found, so this case stays off the GPU until it is")
-
-
-@CONCURRENT_SLOTS_OFF
 def test_alone_equals_mix_and_overlap():
-    alone = {}
-    for job in trace():
-        srv = TransportServer(slots=1, hbm_budget_bytes=8 << 30)
-        job.arrival_s = 0.0
-        alone.update(srv.run([job], realtime=False))
-    srv = TransportServer(slots=4, hbm_budget_bytes=8 << 30)
+    """Each job's DOS and D(E) are bitwise the same run alone (one slot, jobs
+    one after another) and in a 4-slot mix where jobs overlap on the device;
+    slots are worker processes with their own device contexts."""
     jobs = trace()
     for j in jobs:
         j.arrival_s = 0.0
-    mix = srv.run(jobs, realtime=False)
-    assert srv.peak_running >= 2, "jobs never overlapped"
+    with TransportServer(slots=1, hbm_budget_bytes=8 << 30) as srv:
+        alone = srv.run(jobs, realtime=False)
+        assert srv.peak_running == 1
+    with TransportServer(slots=4, hbm_budget_bytes=8 << 30) as srv:
+        mix = srv.run(trace(), realtime=False)
+        assert srv.peak_running >= 2, "jobs never overlapped"
     assert any(j.meta["nonorthogonal"] for j in jobs)
+    assert len(mix) == len(alone) == len(jobs)
     for jid, r in mix.items():
         assert r.error is None and alone[jid].error is None, (r.error, alone[jid].error)
         np.testing.assert_array_equal(r.dos, alone[jid].dos)
         np.testing.assert_array_equal(r.diffusion, alone[jid].diffusion)
 
 
-@CONCURRENT_SLOTS_OFF
 def test_mixed_precision_diffusion_vs_fp64():
-    srv = TransportServer(slots=3, hbm_budget_bytes=8 << 30)
-    mixed = srv.run(trace("mixed"), realtime=False)
-    srv = TransportServer(slots=3, hbm_budget_bytes=8 << 30)
-    fp64 = srv.run(trace("fp64"), realtime=False)
+    with TransportServer(slots=3, hbm_budget_bytes=8 << 30) as srv:
+        mixed = srv.run(trace("mixed"), realtime=False)
+    with TransportServer(slots=3, hbm_budget_bytes=8 << 30) as srv:
+        fp64 = srv.run(trace("fp64"), realtime=False)
     for jid in mixed:
         a, b = mixed[jid].diffusion, fp64[jid].diffusion
         assert mixed[jid].error is None and fp64[jid].error is None

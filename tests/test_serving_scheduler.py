@@ -60,3 +60,17 @@ def test_size_classes_and_trace_shape():
     assert {j.meta["M"] for j in jobs} <= {256, 512, 1024, 2048, 4096}
     assert estimate_job_bytes(10 ** 6, 4 * 10 ** 6, 64, 1, "mixed", True) > \
         estimate_job_bytes(10 ** 6, 4 * 10 ** 6, 64, 1, "mixed", False)
+
+
+def test_jobs_travel_to_worker_processes():
+    """Serving jobs are sent to slot processes by pickling: the synthetic
+    trace's jobs (model builders, configs) survive a round trip and build the
+    same model."""
+    import pickle
+    jobs = synthetic_trace(3, rate_per_s=8.0, n_range=(1e3, 2e3), nonorthogonal_fraction=0.5, seed=4)
+    for j in jobs:
+        k = pickle.loads(pickle.dumps(j))
+        assert k.job_id == j.job_id and k.cfg.n_moments == j.cfg.n_moments
+        np.testing.assert_array_equal(k.cfg.energy_eV, j.cfg.energy_eV)
+        a, b = j.model(), k.model()
+        assert (a.H_cov != b.H_cov).nnz == 0 and (a.S != b.S).nnz == 0
