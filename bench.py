@@ -105,7 +105,10 @@ def serving(args):
     import torch
     from kubomsd_amd.serving import TransportServer, synthetic_trace
     jobs = synthetic_trace(args.jobs, rate_per_s=args.rate, n_range=(1e4, args.n_max))
-    srv = TransportServer(slots=args.slots)
+    def report(r):
+        print(f"# job {r.job_id} done on slot {r.slot}: latency {r.latency_s:.2f} s"
+              + (f", error {r.error}" if r.error else ""), file=sys.stderr, flush=True)
+    srv = TransportServer(slots=args.slots, on_result=report)
     srv.start()  # worker processes up before the clock starts
     t0 = time.perf_counter()
     res = srv.run(jobs, realtime=True)

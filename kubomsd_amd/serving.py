@@ -84,7 +84,9 @@ def estimate_job_bytes(n, nnz, nb, na, precision, nonorthogonal, n_velocity=1):
 
 class TransportServer:
     def __init__(self, device=0, slots=4, hbm_budget_bytes=None, class_edges=(1 << 26, 1 << 30),
-                 max_wait_s=30.0, keep_blocks=False):
+                 max_wait_s=30.0, keep_blocks=False, on_result=None):
+        """on_result: optional callable(JobResult), called as each job finishes."""
+        self.on_result = on_result
         self.device = device
         self.slots = slots
         self.procs = []
@@ -245,8 +247,11 @@ class TransportServer:
                 self.procs[slot] = self._spawn(slot)
             finish = self.clock()
             self._release(job)
-            self.results[job.job_id] = JobResult(job.job_id, job.arrival_s, start, finish, slot,
-                                                 res.get("dos"), res.get("diffusion"), res.get("blocks"), err)
+            result = JobResult(job.job_id, job.arrival_s, start, finish, slot, res.get("dos"), res.get("diffusion"),
+                               res.get("blocks"), err)
+            self.results[job.job_id] = result
+            if self.on_result is not None:
+                self.on_result(result)
 
     def run(self, jobs, realtime=True):
         """Feeds jobs at their arrival times (seconds from start) and returns
