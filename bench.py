@@ -64,7 +64,10 @@ def parse():
     p.add_argument("--check-jobs", type=int, default=6, help="jobs re-run alone in FP64 for the D(E) error")
     p.add_argument("--tmax", type=float, default=5.0, help="stepwise: last output time (fs); reference: 50")
     p.add_argument("--dt", type=float, default=1.0, help="stepwise: output time step (fs)")
-    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise"], default="honeycomb",
+    p.add_argument("--cpu-vectors", type=int, default=2,
+                   help="square: vectors of the CPU reference run (the full estimator on the first ones)")
+    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise", "square"],
+                   default="honeycomb",
                    help="honeycomb: orthogonal moment chain (default); nonorthogonal: simple-cubic 64^3 x 4 "
                         "orbitals, S^-1 H chain with batched block-Jacobi PCG; mu: square 2048^2 double-moment "
                         "tables mu_mn on the matrix cores")
@@ -75,8 +78,8 @@ def parse():
                    help="only run a plain 1 GiB device copy (torch copy kernel), to scale HBM counters")
     a = p.parse_args()
     nonorth = a.workload == "nonorthogonal"
-    a.vectors = a.vectors or (32 if nonorth else 64)
-    a.moments = a.moments or (2048 if a.workload in ("honeycomb", "stepwise") else 1024)
+    a.vectors = a.vectors or (16 if a.workload == "square" else 32 if nonorth else 64)
+    a.moments = a.moments or {"honeycomb": 2048, "stepwise": 2048, "square": 512}.get(a.workload, 1024)
     return a
 
 
@@ -448,6 +451,123 @@ def stepwise(args):
     print(json.dumps(line))
 
 
+class _RefStepwiseOps:
+    """The reference's own CPU kernel (oracle/_ref Engine) behind the
+    interface the oracle's stepwise orchestration (oracle/kpm.py) drives."""
+
+    def __init__(self, ref, model, threads):
+        import scipy.sparse as sp
+
+        def csr(m):
+            m = sp.csr_matrix(m, dtype=np.complex128)
+            return ref.CSR(m.indptr.astype(np.int64), m.indices.astype(np.int64), m.data, m.shape[0], threads)
+        self.metric = ref.Metric(csr(model.S), model.S.diagonal().real.copy(), [], threads)
+        defs = [(0, [csr(model.velocity["x"])]), (-1, []), (-1, [])]
+        self.eng = ref.Engine(csr(model.H), self.metric, defs, 1e-10, 0.0, 1000, 0.6582119569509067)
+        self.native = self
+
+    def moments(self, B, length, center, half):
+        return self.eng.moments(np.ascontiguousarray(B), int(length), center, half)
+
+    def series(self, B, coeff, center, half, directions=()):
+        return self.eng.series(np.ascontiguousarray(B), np.ascontiguousarray(coeff, np.complex128), center, half,
+                               [[float(v) for v in d] for d in directions])
+
+
+def square(args):
+    """The reference's CPU-runnable configuration in full: square 100 x 100
+    periodic lattice (N = 10,000), t = -1 eV, Anderson U[-1, 1] eV, bond
+    velocity along x, 16 random-phase vectors, M = 512 Jackson moments,
+    161 energies on [-4, 4] eV, stepwise MSD at t = 0 .. 10 fs every 0.5 fs,
+    bounds (-5.1, 5.1) eV, FP64.  The GPU runs the whole estimator
+    (iter_stochastic_blocks) on all vectors; the reference's own CPU kernel
+    (oracle/_ref, all host cores) runs the same stepwise estimator on the first
+    --cpu-vectors of them, and the per-sample MSD numerators of both are
+    compared."""
+    import torch
+    from kubomsd_amd.backends.hip import HipContext
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.metric.bounds import certified_bounds
+    from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.stepwise import stepwise_samples
+    from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
+    from kubomsd_amd.synthetic import square_lattice
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    ctx = HipContext(0)
+    side = args.cells if args.cells != 1000 else 100
+    model = square_lattice(side, t=-1.0, W=2.0)
+    n, nb, M = model.n, args.vectors, args.moments
+    E = np.linspace(-4.0, 4.0, 161)
+    T = np.round(np.arange(0.0, 10.0 + 0.25, 0.5), 12)
+    samples = random_phases(n, SEED, range(nb))
+    cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method="stepwise",
+                    energy_eV=E, time_fs=T, axes=("x",), spectral_bounds=(-5.1, 5.1), n_moments=M, nsamples=nb,
+                    sample_batch=nb, statistics_block=8, include_vac=False, precision="fp64", autotune=False)
+    ops = build_operators(ModelBundle(model.H, model.S, velocity={"x": model.velocity["x"]}), cfg, ctx)
+    report = certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower, spectral_bounds=cfg.spectral_bounds)
+    list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=samples, bounds_report=report))  # warm-up
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(max(1, args.steps)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        blocks = list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=samples, bounds_report=report))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    diag = blocks[0].metadata["sample_batch_diagnostics"][0]
+    K, nt = diag["max_propagation_order"], len(T)
+    # column-steps: DOS chain, then per interval the augmented series (width 2 nb) and the axis chain
+    vsteps = (M - 1) * nb + (nt - 1) * (K * 2 * nb + (M - 1) * nb)
+    print(f"# GPU: whole estimator {ms:.1f} ms (median of {len(times)}), order {K}", file=sys.stderr, flush=True)
+
+    # Reference CPU kernel on the first cpu_vectors samples, and the GPU's
+    # per-sample numerators of the same samples.
+    from oracle import kpm
+    from oracle import native as O
+    cb = min(args.cpu_vectors, nb)
+    threads = cpu_threads()
+    ref = O.load_reference_native()
+    kcfg = kpm.config(energy_eV=E, time_fs=T, spectral_bounds=(-5.1, 5.1), n_moments=M, segment_fs=1.0,
+                      axes_vectors=np.eye(3)[:1])
+    if ref is not None:
+        cops, kind = _RefStepwiseOps(ref, model, threads), "reference"
+    else:
+        cops, kind = kpm.Ops(model.H, model.S, {"x": model.velocity["x"]}, rtol=1e-10, threads=threads), "port"
+    t0 = time.perf_counter()
+    cdos, cnums = kpm.stepwise_samples(cops, kcfg, samples[:, :cb])
+    cpu_s = time.perf_counter() - t0
+    cpu_vsteps = vsteps * cb / nb
+    gdos, gnums, _ = stepwise_samples(ops, cfg, ops.asarray(samples[:, :cb]))
+    g = gnums["msd_A2/commutator"].cpu().numpy()
+    c = cnums["msd_A2/commutator"]
+    parity = {"msd_numerators_max_rel": float(np.max(np.abs(g - c)) / np.max(np.abs(c))),
+              "dos_max_rel": float(np.max(np.abs(gdos.cpu().numpy() - cdos)) / np.max(np.abs(cdos))),
+              "samples_compared": cb}
+    print(f"# CPU ({kind}, {threads} threads): {cpu_s:.1f} s for {cb} vectors", file=sys.stderr, flush=True)
+    value = vsteps / (ms / 1e3)
+    line = {
+        "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
+        "value": value, "unit": "vector-steps/s", "n_gpus": 1, "steps": len(times), "warmup": 1,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded square lattice, NumPy PCG64 random phases)",
+        "config": {"workload": f"square {side}x{side} periodic (N={n}), t=-1 eV, W=2 eV, {nb} vectors, M={M}, "
+                               f"stepwise MSD along x, t=0..10 fs every 0.5 fs, 161 energies, FP64, one whole "
+                               f"estimator run per step", "n": n, "vectors": nb, "moments": M, "times": nt,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                     "traffic": None, "kernel": "launch-bound at this size (N = 10^4: about 2.5 MB per block)"},
+        "parity_vs_reference_cpu": parity,
+        "cpu_baseline": {"value": cpu_vsteps / cpu_s, "unit": "vector-steps/s", "cores": threads, "kind": kind,
+                         "sample": f"the whole stepwise estimator (DOS chain, {nt - 1} series steps and MSD "
+                                   f"chains) on the first {cb} of the {nb} vectors, one run, OMP threads={threads}"},
+    }
+    line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line))
+
+
 def mu_errors(ctx, eng, Vd, r, A, Bm, mu32, mub, M, c, a):
     """Max-entry relative errors of the FP32 and bf16x3 tables against FP64:
     (1) the complex128 GEMM of the same complex64 chains, over N in chunks;
@@ -629,6 +749,11 @@ def main():
         if args.gpus != 1:
             sys.exit("this build runs on one GPU only")
         stepwise(args)
+        return
+    if args.workload == "square":
+        if args.gpus != 1:
+            sys.exit("this build runs on one GPU only")
+        square(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sample sharding is a design "
