@@ -192,6 +192,20 @@ class TransportServer:
         self._ready(slot, proc, conn)
         return proc, conn
 
+    def _replace(self, slot):
+        """Ends the slot's worker and starts a new one; False if that fails."""
+        proc, conn = self.procs[slot]
+        proc.join(5)
+        if proc.is_alive():
+            proc.kill()
+            proc.join()
+        conn.close()
+        try:
+            self.procs[slot] = self._spawn(slot)
+        except Exception:
+            return False
+        return True
+
     def start(self):
         """Starts the worker processes, all at once (once; run() calls it)."""
         if self.procs:
@@ -235,16 +249,28 @@ class TransportServer:
             job = self._take()
             if job is None:
                 return
-            proc, conn = self.procs[slot]
             start = self.clock()
-            try:
-                conn.send(job)
-                res, err = conn.recv()
-            except (EOFError, OSError) as exc:
-                proc.join(5)
-                res, err = {}, f"worker process ended ({type(exc).__name__}, exit code {proc.exitcode})"
-                conn.close()
-                self.procs[slot] = self._spawn(slot)
+            res, err = {}, None
+            lost = False
+            for attempt in (0, 1):
+                proc, conn = self.procs[slot]
+                try:
+                    conn.send(job)
+                except (EOFError, OSError) as exc:  # the worker had ended before this job
+                    err = f"worker process unavailable ({type(exc).__name__})"
+                    lost = not self._replace(slot)
+                    if lost:
+                        break
+                    continue  # one retry on a fresh worker
+                except Exception as exc:  # the job cannot be sent (e.g. not picklable)
+                    err = f"job could not be sent to a worker: {type(exc).__name__}: {exc}"
+                    break
+                try:
+                    res, err = conn.recv()
+                except (EOFError, OSError) as exc:  # the worker died running this job
+                    lost = not self._replace(slot)
+                    res, err = {}, f"worker process ended ({type(exc).__name__}, exit code {proc.exitcode})"
+                break
             finish = self.clock()
             self._release(job)
             result = JobResult(job.job_id, job.arrival_s, start, finish, slot, res.get("dos"), res.get("diffusion"),
@@ -252,6 +278,8 @@ class TransportServer:
             self.results[job.job_id] = result
             if self.on_result is not None:
                 self.on_result(result)
+            if lost:  # no worker could be started for this slot: the other slots keep serving
+                return
 
     def run(self, jobs, realtime=True):
         """Feeds jobs at their arrival times (seconds from start) and returns
@@ -275,6 +303,15 @@ class TransportServer:
             self.cv.notify_all()
         for t in threads:
             t.join()
+        # Jobs still queued had no slot left (every worker failed to restart).
+        with self.cv:
+            for q in self.queues:
+                while q:
+                    job = q.popleft()
+                    self.pending -= 1
+                    now = self.clock()
+                    self.results[job.job_id] = JobResult(job.job_id, job.arrival_s, now, now, -1,
+                                                         error="no serving slot could take the job")
         return self.results
 
     @staticmethod

@@ -74,3 +74,81 @@ def test_jobs_travel_to_worker_processes():
         np.testing.assert_array_equal(k.cfg.energy_eV, j.cfg.energy_eV)
         a, b = j.model(), k.model()
         assert (a.H_cov != b.H_cov).nnz == 0 and (a.S != b.S).nnz == 0
+
+
+class _FakeConn:
+    def __init__(self, script):
+        self.script = list(script)  # per send: "ok", "dead", "unpicklable", "dies"
+        self.action = None
+
+    def send(self, job):
+        self.action = self.script.pop(0)
+        if self.action == "dead":
+            raise BrokenPipeError("worker gone")
+        if self.action == "unpicklable":
+            raise TypeError("cannot pickle")
+
+    def recv(self):
+        if self.action == "dies":
+            raise EOFError
+        return {"dos": "dos-of-job"}, None
+
+    def close(self):
+        pass
+
+
+class _FakeProc:
+    exitcode = -11
+
+    def join(self, timeout=None):
+        pass
+
+    def is_alive(self):
+        return False
+
+    def kill(self):
+        pass
+
+
+class _DispatchSrv(TransportServer):
+    """Real scheduling and dispatch over fake worker pipes (no processes)."""
+
+    def __init__(self, scripts, respawn_ok=True):
+        super().__init__(slots=1, hbm_budget_bytes=10 ** 9)
+        self.scripts = list(scripts)
+        self.respawn_ok = respawn_ok
+        self.spawned = 0
+
+    def _spawn(self, slot):
+        if self.spawned and not self.respawn_ok:
+            raise RuntimeError("cannot start")
+        self.spawned += 1
+        return _FakeProc(), _FakeConn(self.scripts.pop(0))
+
+    def start(self):
+        if not self.procs:
+            self.procs = [self._spawn(0)]
+
+
+def test_dispatch_error_paths_release_budget_and_record_every_job():
+    jobs = [TransportJob(i, None, None, arrival_s=0.0, est_bytes=100) for i in range(4)]
+    # worker 1: job 0 ok, job 1 unpicklable, job 2: pipe dead -> respawn
+    # worker 2: job 2 retried ok, job 3: worker dies while running it -> respawn
+    srv = _DispatchSrv([["ok", "unpicklable", "dead"], ["ok", "dies"], []])
+    res = srv.run(jobs, realtime=False)
+    assert sorted(res) == [0, 1, 2, 3]
+    assert res[0].error is None and res[0].dos == "dos-of-job"
+    assert "could not be sent" in res[1].error
+    assert res[2].error is None and res[2].dos == "dos-of-job"
+    assert "exit code -11" in res[3].error
+    assert srv.used == 0 and srv.spawned == 3
+
+
+def test_lost_slot_reports_remaining_jobs():
+    jobs = [TransportJob(i, None, None, arrival_s=0.0, est_bytes=100) for i in range(3)]
+    srv = _DispatchSrv([["dies"]], respawn_ok=False)
+    res = srv.run(jobs, realtime=False)
+    assert sorted(res) == [0, 1, 2]
+    assert "exit code" in res[0].error
+    assert all("no serving slot" in res[j].error for j in (1, 2))
+    assert srv.used == 100 * 0 and srv.pending == 0
