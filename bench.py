@@ -188,7 +188,8 @@ def nonorthogonal(args):
     blocks = atom_block_factors(model.S, 4)
     metric = HipMetric(ctx, model.S, model.S.diagonal().real, blocks)
     eng = HipEngine(ctx, model.H, metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
-    tuning = apply_tuning(ctx, args, eng.H, nb, torch.complex64)
+    # the chain's dominant products: S in the FP32 inner solves
+    tuning = apply_tuning(ctx, args, metric.S, nb, torch.complex64, f32_values=True)
     bound = CUBIC64_BOUND if args.atoms == 64 and CUBIC64_BOUND else None
     if bound is None:
         import scipy.sparse.linalg as sla
@@ -635,7 +636,7 @@ def plain_gpu_fp64(H, Bh, c, a, orders, device):
     return nb * orders / (ms / 1e3), mu
 
 
-def apply_tuning(ctx, args, matrix, nb, dtype):
+def apply_tuning(ctx, args, matrix, nb, dtype, f32_values=False):
     """Explicit --cpl/--groups, else the library's autotuner (cached)."""
     if args.cpl or args.groups:
         if args.cpl:
@@ -643,7 +644,7 @@ def apply_tuning(ctx, args, matrix, nb, dtype):
         if args.groups:
             ctx.set_tuning("groups", args.groups)
         return {"cpl": args.cpl or None, "groups": args.groups or None, "source": "command line"}
-    choice = ctx.autotune(matrix, nb, dtype)
+    choice = ctx.autotune(matrix, nb, dtype, f32_values=f32_values)
     choice["source"] = "kmsd_autotune"
     return choice
 

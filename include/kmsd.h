@@ -35,6 +35,9 @@ extern "C" {
 
 enum { KMSD_OK = 0, KMSD_EINVAL = 1, KMSD_ENUMERIC = 2, KMSD_ERESOURCE = 3, KMSD_EDEVICE = 4 };
 enum { KMSD_C64 = 1, KMSD_C128 = 2 };
+/* Or-ed into kmsd_autotune's dtype: time the product with FP32 matrix values,
+ * the form the inner solves of a complex64 metric solve use. */
+enum { KMSD_TUNE_F32_VALUES = 0x100 };
 enum { KMSD_CSR = 0, KMSD_BSR4 = 1, KMSD_AUTO = 2 };
 enum { KMSD_PRECOND_DIAGONAL = 0, KMSD_PRECOND_BLOCK_JACOBI = 1 };
 enum { KMSD_MU_FP32 = 0, KMSD_MU_BF16X3 = 1, KMSD_MU_FP64 = 2 };
@@ -69,7 +72,8 @@ int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
 /* Launch tuning: key "groups" (workgroups per launch, multiple of 8; 0 =
  * automatic, the default: each fused step launches one workgroup per
  * resident slot of its kernel on the device), "cpl" (complex columns per
- * lane: 1, 2 or 4; 0 = 2 for complex64 and 1 for complex128 blocks),
+ * lane: 1, 2 or 4; 0 = automatic: 2 for complex64 and 1 for complex128
+ * blocks, 4 and 2 for products with 4x4-block matrices),
  * "moment_chunk" (Chebyshev orders per partial-reduction launch),
  * "fp32_math" (complex64 blocks: FP32 values and row sums), "pcg_poll" (PCG
  * iterations between host polls), "graphs" (1: moment chains with an
@@ -85,7 +89,9 @@ int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
  * workgroup count (automatic, 1024, 2048, 4096), keeps the default shape
  * (automatic workgroups) unless another is more than 3 % faster, applies
  * the choice to ctx ("cpl", "groups") and writes it to choice[2] (groups 0 =
- * automatic).  Uses three scratch blocks of n*nb elements from the budget. */
+ * automatic).  dtype may carry KMSD_TUNE_F32_VALUES (tune for the metric
+ * products of complex64 solves).  Uses three scratch blocks of n*nb elements
+ * from the budget. */
 int kmsd_autotune(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice);
 /* Message of the last failure in ctx (or of the last failed create when ctx
  * is NULL).  Valid until the next call on ctx. */

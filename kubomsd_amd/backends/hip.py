@@ -23,6 +23,7 @@ from ..units import HBAR_EV_FS
 LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libkmsd.so"
 KMSD_OK, KMSD_EINVAL, KMSD_ENUMERIC, KMSD_ERESOURCE, KMSD_EDEVICE = range(5)
 KMSD_C64, KMSD_C128 = 1, 2
+KMSD_TUNE_F32_VALUES = 0x100
 KMSD_CSR, KMSD_BSR4, KMSD_AUTO = 0, 1, 2
 KMSD_MU_FP32, KMSD_MU_BF16X3, KMSD_MU_FP64 = 0, 1, 2
 
@@ -186,17 +187,21 @@ class HipContext:
     def set_tuning(self, key, value):
         self.check(self.lib.kmsd_ctx_set_tuning(self.h, key.encode(), int(value)))
 
-    def autotune(self, matrix, nb, dtype, cache_path=None):
+    def autotune(self, matrix, nb, dtype, cache_path=None, f32_values=False):
         """Launch shape (columns per lane, workgroups) for products of `matrix`
         on nb-column blocks of `dtype` (torch dtype), timed on the device by
         kmsd_autotune and remembered per (device, storage, n, nnz, nb, dtype)
         in this process and in a JSON file (KUBOMSD_TUNING_CACHE, default
         ~/.cache/kubomsd_amd/tuning.json), so later runs reuse the same shape
-        and produce the same results.  Applies the shape to this context."""
+        and produce the same results.  f32_values: tune the product with FP32
+        matrix values (the metric products of complex64 solves, which dominate
+        nonorthogonal chains).  Applies the shape to this context."""
         import json
         import os
         torch = _torch()
         code = KMSD_C64 if dtype == torch.complex64 else KMSD_C128
+        if f32_values and code == KMSD_C64:
+            code |= KMSD_TUNE_F32_VALUES
         info = matrix.info()
         key = "|".join(str(v) for v in (torch.cuda.get_device_name(self.device), info["format"], info["n"],
                                          info["nnz"], int(nb), code))

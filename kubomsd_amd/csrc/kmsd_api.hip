@@ -115,8 +115,11 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
 
 extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice) {
     if (!c || !A || nb < 1 || nb > (int64_t(1) << 20) || !choice) return KMSD_EINVAL;
+    const bool f32_values = (dtype & KMSD_TUNE_F32_VALUES) != 0;
+    dtype &= ~int(KMSD_TUNE_F32_VALUES);
     return guard(c, [&] {
         if (dtype != KMSD_C64 && dtype != KMSD_C128) throw Error(KMSD_EINVAL, "dtype must be KMSD_C64 or KMSD_C128");
+        if (f32_values && dtype != KMSD_C64) throw Error(KMSD_EINVAL, "FP32 matrix values need complex64 blocks");
         // Times the fused Chebyshev step of a moment chain (gathered X,
         // previous iterate P overwritten in place, projection W, moment
         // partials) for each candidate shape.
@@ -148,7 +151,7 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
             HIPCHECK(hipMemsetAsync(W, 0, bytes, c->stream));
             HIPCHECK(hipEventCreate(&e0));
             HIPCHECK(hipEventCreate(&e1));
-            const int def_cpl = dtype == KMSD_C64 ? 2 : 1;
+            const int def_cpl = default_cpl(dtype, A);
             const int cpls[3] = {1, 2, 4};
             const int64_t groups[4] = {0, 1024, 2048, 4096};  // 0 = resident capacity
             double best = 0, def_time = 0;
@@ -169,13 +172,21 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
                     s.ldp = nb;
                     s.OUT = P;
                     s.ldo = nb;
-                    s.W = W;
-                    s.ldw = nb;
                     s.part = static_cast<double *>(part);
                     s.alpha = Cplx{1.0, 0};
-                    s.beta = Cplx{-0.5, 0};
-                    s.gamma = -1.0;
-                    s.use_beta = s.use_p = true;
+                    if (f32_values) {
+                        // the inner solves' product: S p stored, <p, S p> partials
+                        s.W = X;
+                        s.ldw = nb;
+                        s.f32_matrix = true;
+                    } else {
+                        // a moment-chain order: 2 H T_n - T_(n-1) in place, <W, T_(n+1)> partials
+                        s.W = W;
+                        s.ldw = nb;
+                        s.beta = Cplx{-0.5, 0};
+                        s.gamma = -1.0;
+                        s.use_beta = s.use_p = true;
+                    }
                     s.n = A->n;
                     s.w = int32_t(nb);
                     launch_step(c, dtype, s);  // warm-up

@@ -167,6 +167,8 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s);
 // every launch that writes partial rows of one reduction.
 int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap, const kmsd_matrix *A,
                     bool f32_matrix = false);
+// Automatic columns per lane of products with A (null: no product) on dtype blocks.
+int default_cpl(int dtype, const kmsd_matrix *A);
 // Columns-per-lane ceiling supported by a set of blocks (alignment and leading dimensions).
 int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t>> blocks);
 // out[s*ldo + j] = sum_g part[s][g][j]; mode 0 real part (double), 1 complex.

@@ -239,6 +239,18 @@ struct Shape {
     int64_t gx = 8;
 };
 
+}  // namespace
+
+// Automatic columns per lane: 2 (complex64) / 1 (complex128); 4 / 2 for 4x4
+// block products, whose lanes reuse each gathered row for four output rows.
+int default_cpl(int dtype, const kmsd_matrix *A) {
+    const bool bsr = A && A->fmt == KMSD_BSR4;
+    if (dtype == KMSD_C64) return bsr ? 4 : 2;
+    return bsr ? 2 : 1;
+}
+
+namespace {
+
 int ilog2_ceil(int64_t v) {
     int l = 0;
     while ((int64_t(1) << l) < v) ++l;
@@ -312,7 +324,7 @@ int resident_per_cu(KernelFn fn) {
 Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const kmsd_matrix *A, bool f32,
                 KernelFn *fn_out = nullptr) {
     Shape s;
-    const int want = c->cpl ? c->cpl : (dtype == KMSD_C64 ? 2 : 1);
+    const int want = c->cpl ? c->cpl : default_cpl(dtype, A);
     s.cpl = std::min(want, std::min(cpl_max, dtype == KMSD_C64 ? 4 : 2));
     while (s.cpl > 1 && w % s.cpl) s.cpl >>= 1;
     const int64_t lanes = (w + s.cpl - 1) / s.cpl;

@@ -123,8 +123,14 @@ class OperatorBundle:
             # Launch shape for the dominant products (moment chains on one
             # sample batch); remembered per matrix signature, so reruns and
             # serving mixes use the same shape.
+            # With a nonidentity metric the S products of the inner
+            # (complex64: FP32-valued) solves dominate, so those are timed.
             try:
-                self.tuning = self.ctx.autotune(self.H, config.sample_batch, self.dtype)
+                if self.metric.identity:
+                    self.tuning = self.ctx.autotune(self.H, config.sample_batch, self.dtype)
+                else:
+                    self.tuning = self.ctx.autotune(self.metric.device.S, config.sample_batch, self.dtype,
+                                                    f32_values=True)
             except ResourceBlocked:
                 self.tuning = None
         self.native = HipEngine(self.ctx, self.H, self.metric.device, definitions, config.metric_rtol,
