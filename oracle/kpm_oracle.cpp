@@ -79,12 +79,57 @@ struct MetricFailure : std::runtime_error {
     MetricFailure(const char *what, SolveReport r) : std::runtime_error(what), report(std::move(r)) {}
 };
 
-// Contiguous atom block [first, last) with its lower Cholesky factor
-// (block-Jacobi; cpp/native.cpp:145-149, 212-229).
+// Atom block with its lower Cholesky factor: the contiguous rows
+// [first, last) (block-Jacobi; cpp/native.cpp:145-149, 212-229), or, when
+// `rows` is set, the listed rows of one cluster (cluster-Jacobi,
+// cpp/native.cpp:180-197; first = 0, last = rows.size()).
 struct AtomBlock {
     int64_t first = 0, last = 0;
+    std::vector<int64_t> rows;
     cvec L;  // (m, m) row-major
+    int64_t row(int64_t i) const { return rows.empty() ? first + i : rows[size_t(i)]; }
 };
+
+// IC(0) factor of the sparse lower pattern given (cpp/native.cpp:92-121):
+// rows sorted, strictly lower entries first, the diagonal last.  Entry (i,j)
+// subtracts the products of the matching earlier entries of rows i and j
+// (merged by column) and divides by row j's diagonal; the pivot is the
+// diagonal plus the shift minus the squared moduli of the row.
+cvec ic0_factor(int64_t n, const int64_t *ptr, const int64_t *idx, const cplx *a, double shift) {
+    if (!std::isfinite(shift) || shift < 0) throw std::invalid_argument("invalid IC shift");
+    cvec f(size_t(ptr[n]));
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t b = ptr[i], e = ptr[i + 1];
+        if (b == e || idx[e - 1] != i) throw std::invalid_argument("IC pattern requires diagonal at row end");
+        for (int64_t ij = b; ij + 1 < e; ++ij) {
+            const int64_t j = idx[ij];
+            if (j >= i || (ij > b && idx[ij - 1] >= j))
+                throw std::invalid_argument("IC pattern must be sorted strict lower triangle");
+            cplx v = a[ij];
+            int64_t p = b, q = ptr[j];
+            const int64_t qd = ptr[j + 1] - 1;  // row j's diagonal
+            while (p < ij && q < qd) {
+                if (idx[p] < idx[q]) {
+                    ++p;
+                } else if (idx[q] < idx[p]) {
+                    ++q;
+                } else {
+                    v -= f[p] * std::conj(f[q]);
+                    ++p;
+                    ++q;
+                }
+            }
+            f[ij] = v / f[qd];
+        }
+        cplx piv = a[e - 1] + shift;
+        for (int64_t k = b; k + 1 < e; ++k) piv -= std::norm(f[k]);
+        if (!std::isfinite(piv.real()) || !std::isfinite(piv.imag()) || std::abs(piv.imag()) > 1e-12 ||
+            piv.real() <= 1e-14)
+            throw std::runtime_error("IC nonpositive/nonfinite pivot at row " + std::to_string(i));
+        f[e - 1] = std::sqrt(piv.real());
+    }
+    return f;
+}
 
 struct Metric {
     std::shared_ptr<Csr> S;
@@ -96,37 +141,78 @@ struct Metric {
     double gap_ratio = 0.1;
     int64_t calls = 0, iterations = 0, refreshes = 0, residual_checks = 0, restarts = 0;
     double max_relative = 0;
+    // IC(0) preconditioner M = D^-1 P^T L L^H P D^-1 on S (cpp/native.cpp:163-179):
+    // L in CSR with each row's diagonal last, perm[i] = original row of
+    // factor row i, scale = D^-1 per original row.
+    bool has_ic = false;
+    std::vector<int64_t> ic_ptr, ic_idx, ic_perm;
+    cvec ic_val;
+    std::vector<double> ic_scale;
+    // Direct solve with the one full-size factor in blocks[0] plus iterative
+    // refinement on the original equation (cpp/native.cpp:198-205, 331-347).
+    bool direct = false;
+    int refine_maxiter = 3;
+    int64_t refinement_steps = 0;
 
-    // z = M^-1 r: diagonal or block-Jacobi L^-H L^-1 (cpp/native.cpp:237-297).
+    // L^-H L^-1 r on one block, one column (cpp/native.cpp:265-281).
+    void block_solve(const AtomBlock &blk, const Block &r, Block &z, int64_t j) const {
+        const int64_t m = blk.last - blk.first, b = r.b;
+        for (int64_t i = 0; i < m; ++i) {  // forward: L w = r
+            cplx s = r.v[blk.row(i) * b + j];
+            for (int64_t k = 0; k < i; ++k) s -= blk.L[i * m + k] * z.v[blk.row(k) * b + j];
+            z.v[blk.row(i) * b + j] = s / blk.L[i * m + i];
+        }
+        for (int64_t i = m; i-- > 0;) {  // backward: L^H z = w
+            cplx s = z.v[blk.row(i) * b + j];
+            for (int64_t k = i + 1; k < m; ++k) s -= std::conj(blk.L[k * m + i]) * z.v[blk.row(k) * b + j];
+            z.v[blk.row(i) * b + j] = s / std::conj(blk.L[i * m + i]);
+        }
+    }
+
+    // z = M^-1 r: IC(0), diagonal, the direct factor, or block-Jacobi
+    // L^-H L^-1 (cpp/native.cpp:237-297), in that order of precedence.
     Block precondition(const Block &r) const {
         Block z(r.n, r.b);
         const int64_t n = r.n, b = r.b;
+        if (has_ic) {
+            // Per column: forward L w = P D^-1 r in row order, then L^H x = w
+            // column-oriented from the last row (each solved entry is
+            // subtracted from the rows of its factor row at once), z = D^-1 P^T x.
+            Block w(n, b);
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 2048 && threads > 1)
+            for (int64_t j = 0; j < b; ++j) {
+                for (int64_t i = 0; i < n; ++i) {
+                    const int64_t o = ic_perm[i], d = ic_ptr[i + 1] - 1;
+                    cplx v = ic_scale[o] * r.v[o * b + j];
+                    for (int64_t k = ic_ptr[i]; k < d; ++k) v -= ic_val[k] * w.v[ic_idx[k] * b + j];
+                    w.v[i * b + j] = v / ic_val[d];
+                }
+                for (int64_t i = n; i-- > 0;) {
+                    const int64_t d = ic_ptr[i + 1] - 1;
+                    const cplx v = w.v[i * b + j] / std::conj(ic_val[d]);
+                    w.v[i * b + j] = v;
+                    for (int64_t k = ic_ptr[i]; k < d; ++k) w.v[ic_idx[k] * b + j] -= std::conj(ic_val[k]) * v;
+                    const int64_t o = ic_perm[i];
+                    z.v[o * b + j] = ic_scale[o] * v;
+                }
+            }
+            return z;
+        }
         if (blocks.empty()) {
 #pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 4096 && threads > 1)
             for (int64_t i = 0; i < n; ++i)
                 for (int64_t j = 0; j < b; ++j) z.v[i * b + j] = r.v[i * b + j] / diag[i];
             return z;
         }
+        if (direct) {
+#pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 2048 && threads > 1)
+            for (int64_t j = 0; j < b; ++j) block_solve(blocks[0], r, z, j);
+            return z;
+        }
         const int64_t nblk = int64_t(blocks.size());
 #pragma omp parallel for num_threads(threads) schedule(static) if (n * b >= 2048 && threads > 1)
-        for (int64_t q = 0; q < nblk; ++q) {
-            const AtomBlock &blk = blocks[q];
-            const int64_t m = blk.last - blk.first, o = blk.first;
-            for (int64_t j = 0; j < b; ++j) {
-                // forward: L w = r
-                for (int64_t i = 0; i < m; ++i) {
-                    cplx s = r.v[(o + i) * b + j];
-                    for (int64_t k = 0; k < i; ++k) s -= blk.L[i * m + k] * z.v[(o + k) * b + j];
-                    z.v[(o + i) * b + j] = s / blk.L[i * m + i];
-                }
-                // backward: L^H z = w
-                for (int64_t i = m; i-- > 0;) {
-                    cplx s = z.v[(o + i) * b + j];
-                    for (int64_t k = i + 1; k < m; ++k) s -= std::conj(blk.L[k * m + i]) * z.v[(o + k) * b + j];
-                    z.v[(o + i) * b + j] = s / std::conj(blk.L[i * m + i]);
-                }
-            }
-        }
+        for (int64_t q = 0; q < nblk; ++q)
+            for (int64_t j = 0; j < b; ++j) block_solve(blocks[q], r, z, j);
         return z;
     }
 
@@ -170,7 +256,7 @@ struct Metric {
             thr[j] = std::max(rtol * qn[j], atol / safe[j]);
             active[j] = qn[j] > thr[j];
         }
-        Block r = q, z = precondition(r), p = z;
+        Block r = q, z = direct ? Block(n, b) : precondition(r), p = z;
         {
             const auto d = dot(r, z);
             for (int64_t j = 0; j < b; ++j) rz[j] = d[j].real();
@@ -180,7 +266,29 @@ struct Metric {
                 if (!active[j]) p.v[i * b + j] = 0.0;
         report = SolveReport(b);
         bool broke = false;
-        for (int k = 0; k < maxiter; ++k) {
+        if (direct) {
+            // y = M^-1 q, then up to refine_maxiter corrections M^-1 (q - S y)
+            // restricted to the columns whose true residual is above threshold.
+            y = precondition(q);
+            for (int step = 0; step < refine_maxiter; ++step) {
+                Block t = S->apply(y);
+                ++refreshes;
+                ++residual_checks;
+                for (size_t e = 0; e < t.v.size(); ++e) t.v[e] = q.v[e] - t.v[e];
+                const std::vector<double> tn = norms(t);
+                bool needed = false;
+                for (int64_t j = 0; j < b; ++j) {
+                    needed = needed || tn[j] > thr[j];
+                    if (tn[j] <= thr[j])
+                        for (int64_t i = 0; i < n; ++i) t.v[i * b + j] = 0.0;
+                }
+                if (!needed) break;
+                const Block corr = precondition(t);
+                for (size_t e = 0; e < y.v.size(); ++e) y.v[e] += corr.v[e];
+                ++refinement_steps;
+            }
+        }
+        for (int k = 0; k < (direct ? 0 : maxiter); ++k) {
             if (std::none_of(active.begin(), active.end(), [](uint8_t a) { return a != 0; })) break;
             const Block sp = S->apply(p);
             const auto curv = dot(p, sp);
@@ -556,6 +664,82 @@ void *ko_metric_create(void *S, const double *diag, int64_t nblocks, const int64
     return m;
 }
 void ko_metric_destroy(void *m) { delete static_cast<std::shared_ptr<Metric> *>(m); }
+// Cluster blocks: block q holds rows[off[q] .. off[q+1]) with its factor
+// (m x m, complex interleaved) concatenated in `factors`
+// (cpp/native.cpp:180-197).  They replace the contiguous blocks.
+int ko_metric_set_index_blocks(void *m, int64_t nblocks, const int64_t *off, const int64_t *rows,
+                               const double *factors) {
+    return guarded([&] {
+        Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+        const int64_t n = g.S->n;
+        std::vector<AtomBlock> next;
+        std::vector<char> seen(size_t(n), 0);
+        size_t fo = 0;
+        for (int64_t q = 0; q < nblocks; ++q) {
+            AtomBlock blk;
+            const int64_t size = off[q + 1] - off[q];
+            if (size < 1) throw std::invalid_argument("invalid cluster factor");
+            blk.first = 0;
+            blk.last = size;
+            blk.rows.assign(rows + off[q], rows + off[q + 1]);
+            for (int64_t i : blk.rows) {
+                if (i < 0 || i >= n || seen[size_t(i)]) throw std::invalid_argument("invalid cluster coverage");
+                seen[size_t(i)] = 1;
+            }
+            blk.L.resize(size_t(size * size));
+            std::memcpy(blk.L.data(), factors + 2 * fo, sizeof(cplx) * size_t(size * size));
+            fo += size_t(size * size);
+            next.push_back(std::move(blk));
+        }
+        if (std::find(seen.begin(), seen.end(), 0) != seen.end())
+            throw std::invalid_argument("incomplete cluster coverage");
+        g.blocks = std::move(next);
+    });
+}
+// IC(0) preconditioner from the factor L (CSR, n = S's n, diagonal last in
+// each row), perm and scale (cpp/native.cpp:163-179).
+int ko_metric_set_ic0(void *m, const int64_t *ptr, const int64_t *idx, const double *val, const int64_t *perm,
+                      const double *scale) {
+    return guarded([&] {
+        Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+        const int64_t n = g.S->n;
+        std::vector<char> seen(size_t(n), 0);
+        const cplx *v = reinterpret_cast<const cplx *>(val);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t p = perm[i];
+            if (p < 0 || p >= n || seen[size_t(p)] || !(scale[i] > 0) || !std::isfinite(scale[i]))
+                throw std::invalid_argument("invalid IC permutation/scaling");
+            seen[size_t(p)] = 1;
+            if (ptr[i] == ptr[i + 1] || idx[ptr[i + 1] - 1] != i || !(v[ptr[i + 1] - 1].real() > 0))
+                throw std::invalid_argument("invalid IC factor diagonal");
+        }
+        g.ic_ptr.assign(ptr, ptr + n + 1);
+        g.ic_idx.assign(idx, idx + ptr[n]);
+        g.ic_val.assign(v, v + ptr[n]);
+        g.ic_perm.assign(perm, perm + n);
+        g.ic_scale.assign(scale, scale + n);
+        g.has_ic = true;
+    });
+}
+// Direct mode (cpp/native.cpp:198-205): needs exactly one block covering all rows.
+int ko_metric_set_direct(void *m, int enabled, int steps) {
+    return guarded([&] {
+        Metric &g = **static_cast<std::shared_ptr<Metric> *>(m);
+        if (steps < 0 ||
+            (enabled && (g.blocks.size() != 1 || g.blocks[0].first != 0 || g.blocks[0].last != g.S->n)))
+            throw std::invalid_argument("invalid direct metric factor");
+        g.direct = enabled != 0;
+        g.refine_maxiter = steps;
+    });
+}
+int64_t ko_metric_refinement_steps(void *m) { return (**static_cast<std::shared_ptr<Metric> *>(m)).refinement_steps; }
+// Reference: ic0_factor(CSR, shift) (cpp/native.cpp:614-618).
+int ko_ic0_factor(int64_t n, const int64_t *ptr, const int64_t *idx, const double *val, double shift, double *out) {
+    return guarded([&] {
+        const cvec f = ic0_factor(n, ptr, idx, reinterpret_cast<const cplx *>(val), shift);
+        std::memcpy(out, f.data(), sizeof(cplx) * f.size());
+    });
+}
 int ko_metric_set_residual_policy(void *m, int adaptive, int interval, double gap) {
     return guarded([&] {
         if (interval < 1 || !(gap > 0 && gap < 1)) throw std::invalid_argument("invalid residual policy");

@@ -46,6 +46,12 @@ def load():
         lib.ko_metric_solve.argtypes = [P, P, P, I64, I64, D, D, I, P, P, P, P, P]
         lib.ko_metric_sample.argtypes = [P, P, P, I64, I64, P, I64, D, D]
         lib.ko_metric_counters.argtypes = [P, P, P]
+        lib.ko_metric_set_index_blocks.argtypes = [P, I64, P, P, P]
+        lib.ko_metric_set_ic0.argtypes = [P, P, P, P, P, P]
+        lib.ko_metric_set_direct.argtypes = [P, I, I]
+        lib.ko_metric_refinement_steps.argtypes = [P]
+        lib.ko_metric_refinement_steps.restype = I64
+        lib.ko_ic0_factor.argtypes = [I64, P, P, P, D, P]
         lib.ko_engine_create.restype = P
         lib.ko_engine_create.argtypes = [P, P, P, P, D, D, I, D]
         lib.ko_engine_destroy.argtypes = [P]
@@ -162,12 +168,47 @@ class Metric:
         _check(load().ko_metric_sample(self.h, _ptr(xi), _ptr(out), *xi.shape, _ptr(c), len(c), center, half))
         return out
 
+    def set_index_blocks(self, blocks):
+        """Cluster blocks [(rows, L)] replacing the contiguous ones (cpp/native.cpp:180-197)."""
+        rows = np.concatenate([np.asarray(i, np.int64) for i, _ in blocks]) if blocks else np.zeros(1, np.int64)
+        off = np.cumsum([0] + [len(i) for i, _ in blocks]).astype(np.int64)
+        fac = (np.concatenate([np.ascontiguousarray(L, np.complex128).ravel() for _, L in blocks])
+               if blocks else np.zeros(1, np.complex128))
+        self._keep_blocks = (rows, off, fac)
+        _check(load().ko_metric_set_index_blocks(self.h, len(blocks), _ptr(off), _ptr(rows), _ptr(fac)))
+
+    def set_ic0(self, L, permutation, scaling):
+        """IC(0) preconditioner from the factor CSR L (cpp/native.cpp:163-179)."""
+        L = sp.csr_matrix(L, dtype=np.complex128)
+        arrs = (np.ascontiguousarray(L.indptr, np.int64), np.ascontiguousarray(L.indices, np.int64),
+                np.ascontiguousarray(L.data, np.complex128), np.ascontiguousarray(permutation, np.int64),
+                np.ascontiguousarray(scaling, np.float64))
+        if L.shape[0] != self.S.n or len(arrs[3]) != self.S.n or len(arrs[4]) != self.S.n:
+            raise OracleError("invalid IC dimensions")
+        self._keep_ic = arrs
+        _check(load().ko_metric_set_ic0(self.h, *[_ptr(a) for a in arrs]))
+
+    def set_direct(self, enabled, steps):
+        _check(load().ko_metric_set_direct(self.h, int(bool(enabled)), int(steps)))
+
     def diagnostics(self):
         v = np.zeros(5, np.int64)
         mr = C.c_double()
         load().ko_metric_counters(self.h, _ptr(v), C.byref(mr))
         return dict(calls=int(v[0]), max_iterations_sum=int(v[1]), true_residual_refreshes=int(v[2]),
-                    true_residual_checks=int(v[3]), residual_restarts=int(v[4]), max_relative_residual=mr.value)
+                    true_residual_checks=int(v[3]), residual_restarts=int(v[4]), max_relative_residual=mr.value,
+                    refinement_steps=int(load().ko_metric_refinement_steps(self.h)))
+
+
+def ic0_factor(lower, shift):
+    """IC(0) values on the sorted lower pattern of `lower` (cpp/native.cpp:92-121, 614-618)."""
+    L = sp.csr_matrix(lower, dtype=np.complex128)
+    ptr = np.ascontiguousarray(L.indptr, np.int64)
+    idx = np.ascontiguousarray(L.indices, np.int64)
+    val = np.ascontiguousarray(L.data, np.complex128)
+    out = np.empty_like(val)
+    _check(load().ko_ic0_factor(L.shape[0], _ptr(ptr), _ptr(idx), _ptr(val), float(shift), _ptr(out)))
+    return out
 
 
 class Engine:
