@@ -126,9 +126,11 @@ int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int
  * (cpp/native.cpp:212-236, 647-648).  diag: host f64[n] (> 0, finite).
  * nblocks contiguous atom blocks [block_ptr[q], block_ptr[q+1]) with lower
  * Cholesky factors concatenated row-major (complex128 interleaved);
- * nblocks = 0 selects the diagonal preconditioner.  An S that is exactly the
- * identity is detected; its solves return the right-hand side (the
- * reference's PCG converges there in one step with alpha = 1). */
+ * nblocks = 0 selects the diagonal preconditioner.  Blocks may have any size
+ * (one block of n rows is the full Cholesky factor).  An S that is exactly
+ * the identity with the diagonal preconditioner is detected; its solves
+ * return the right-hand side (the reference's PCG converges there in one
+ * step with alpha = 1). */
 int kmsd_metric_create(kmsd_ctx *ctx, kmsd_matrix *S, const double *diag, int64_t nblocks,
                        const int64_t *block_ptr, const double *factors, kmsd_metric **out);
 int kmsd_metric_destroy(kmsd_metric *m);
@@ -153,6 +155,40 @@ int kmsd_metric_inner(kmsd_metric *m, const void *A, const void *B, double *out,
  * out[0] calls, [1] sum of max iterations, [2] refreshes, [3] residual
  * checks, [4] restarts; *max_relative. */
 int kmsd_metric_counters(const kmsd_metric *m, int64_t *out5, double *max_relative);
+/* Reference: Metric.set_index_blocks([(ids, L)]) (cpp/native.cpp:180-197).
+ * Cluster-Jacobi blocks replacing the contiguous ones: block q holds the rows
+ * rows[block_off[q] .. block_off[q+1]) (any order, each row of S exactly
+ * once) and its lower Cholesky factor (m x m row-major complex128, factors
+ * concatenated).  Errors: "invalid cluster factor", "invalid cluster
+ * coverage", "incomplete cluster coverage". */
+int kmsd_metric_set_index_blocks(kmsd_metric *m, int64_t nblocks, const int64_t *block_off, const int64_t *rows,
+                                 const double *factors);
+/* Reference: Metric.set_ic0(L, p, scaling) (cpp/native.cpp:163-179).  IC(0)
+ * preconditioner M^-1 r = D P^T L^-H L^-1 P D r: L in CSR (n rows, host
+ * arrays, complex128 values) with each row's strictly lower entries first
+ * and its real positive diagonal last, perm[i] the original row of factor
+ * row i, scaling D per original row.  Takes precedence over blocks and the
+ * diagonal.  Errors: "invalid IC permutation/scaling", "invalid IC factor
+ * diagonal", and (stricter than the reference, which the level schedules
+ * need) "invalid IC factor pattern" for an entry right of the diagonal. */
+int kmsd_metric_set_ic0(kmsd_metric *m, int64_t nnz, const int64_t *rowptr, const int64_t *colidx,
+                        const double *values, const int64_t *perm, const double *scaling);
+/* Reference: Metric.set_direct(enabled, steps) (cpp/native.cpp:198-205).
+ * Direct solves with the single full-size block factor: y = M^-1 b plus up
+ * to `steps` refinement corrections on the original equation; iterations
+ * are reported as 0.  Enabling needs exactly one block covering all rows
+ * ("invalid direct metric factor"). */
+int kmsd_metric_set_direct(kmsd_metric *m, int enabled, int steps);
+/* Reference: Metric.diagnostics()["refinement_steps"] (cpp/native.cpp:668-672). */
+int kmsd_metric_refinement_steps(const kmsd_metric *m, int64_t *out);
+/* Reference: ic0_factor(CSR, shift) (cpp/native.cpp:92-121, 614-618).  IC(0)
+ * values on the given sorted lower pattern (host arrays in and out,
+ * complex128 interleaved), computed on the device by dependency levels with
+ * the reference's arithmetic (same values to the last bit).  Errors:
+ * KMSD_EINVAL "invalid IC shift" / pattern messages; KMSD_ENUMERIC "IC
+ * nonpositive/nonfinite pivot at row i" for the first failing row. */
+int kmsd_ic0_factor(kmsd_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rowptr, const int64_t *colidx,
+                    const double *values, double shift, double *out);
 
 /* ---- engine ----------------------------------------------------------- */
 /* Reference: Engine(H, Metric, [(kind, [CSR...]) x3], rtol, atol, maxiter,

@@ -52,6 +52,11 @@ SIGNATURES = {
     "kmsd_metric_sample": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D],
     "kmsd_metric_counters": [_P, _P, _P],
     "kmsd_metric_inner": [_P, _P, _P, _P, _I64, _I],
+    "kmsd_metric_set_index_blocks": [_P, _I64, _P, _P, _P],
+    "kmsd_metric_set_ic0": [_P, _I64, _P, _P, _P, _P, _P],
+    "kmsd_metric_set_direct": [_P, _I, _I],
+    "kmsd_metric_refinement_steps": [_P, _P],
+    "kmsd_ic0_factor": [_P, _I64, _I64, _P, _P, _P, _D, _P],
     "kmsd_engine_create": [_P, _P, _P, _P, _P, _D, _D, _I, _D, _P],
     "kmsd_engine_destroy": [_P],
     "kmsd_engine_set_controls": [_P, _D, _D, _I],
@@ -383,13 +388,65 @@ class HipMetric:
                                             C.c_void_p(out.data_ptr()), A.shape[1], _dtype_code(A)))
         return out
 
+    def set_index_blocks(self, blocks):
+        """Cluster-Jacobi blocks [(rows, L)] replacing the contiguous ones
+        (reference Metric.set_index_blocks, cpp/native.cpp:180-197)."""
+        blocks = list(blocks)
+        rows = (np.concatenate([np.asarray(i, np.int64).ravel() for i, _ in blocks]) if blocks
+                else np.zeros(1, np.int64))
+        off = np.cumsum([0] + [np.asarray(i).size for i, _ in blocks]).astype(np.int64)
+        for i, L in blocks:
+            m = np.asarray(i).size
+            if np.asarray(L).shape != (m, m):
+                raise KuboMSDError("invalid cluster factor")
+        fac = (np.concatenate([np.ascontiguousarray(L, np.complex128).ravel() for _, L in blocks]) if blocks
+               else np.zeros(1, np.complex128))
+        self.ctx.check(self.ctx.lib.kmsd_metric_set_index_blocks(self.h, len(blocks), _np_ptr(off), _np_ptr(rows),
+                                                                _np_ptr(fac)))
+
+    def set_ic0(self, L, permutation, scaling):
+        """IC(0) preconditioner from the factor L (CSR, diagonal last in each
+        row), the factor-row-to-original-row permutation and D per original
+        row (reference Metric.set_ic0, cpp/native.cpp:163-179)."""
+        L = sp.csr_matrix(L)
+        p = np.ascontiguousarray(permutation, np.int64)
+        z = np.ascontiguousarray(scaling, np.float64)
+        if L.shape != (self.n, self.n) or p.shape != (self.n,) or z.shape != (self.n,):
+            raise KuboMSDError("invalid IC dimensions")
+        ptr = np.ascontiguousarray(L.indptr, np.int64)
+        idx = np.ascontiguousarray(L.indices, np.int64)
+        val = np.ascontiguousarray(L.data, np.complex128)
+        self.ctx.check(self.ctx.lib.kmsd_metric_set_ic0(self.h, len(val), _np_ptr(ptr), _np_ptr(idx), _np_ptr(val),
+                                                       _np_ptr(p), _np_ptr(z)))
+
+    def set_direct(self, enabled, steps):
+        """Direct solves with the one full-size block factor plus `steps`
+        refinement corrections (reference Metric.set_direct, cpp/native.cpp:198-205)."""
+        self.ctx.check(self.ctx.lib.kmsd_metric_set_direct(self.h, int(bool(enabled)), int(steps)))
+
     def diagnostics(self):
         v = np.zeros(5, np.int64)
         mr = C.c_double()
         self.ctx.lib.kmsd_metric_counters(self.h, _np_ptr(v), C.byref(mr))
+        steps = np.zeros(1, np.int64)
+        self.ctx.lib.kmsd_metric_refinement_steps(self.h, _np_ptr(steps))
         return dict(calls=int(v[0]), max_iterations_sum=int(v[1]), true_residual_refreshes=int(v[2]),
                     true_residual_checks=int(v[3]), residual_restarts=int(v[4]),
-                    max_relative_residual=mr.value)
+                    max_relative_residual=mr.value, refinement_steps=int(steps[0]))
+
+
+def ic0_factor(ctx, lower, shift):
+    """IC(0) values on the sorted lower pattern of `lower`, computed on the
+    device (reference ic0_factor, cpp/native.cpp:92-121, 614-618).  A failing
+    pivot raises NumericalError naming the row."""
+    L = sp.csr_matrix(lower)
+    ptr = np.ascontiguousarray(L.indptr, np.int64)
+    idx = np.ascontiguousarray(L.indices, np.int64)
+    val = np.ascontiguousarray(L.data, np.complex128)
+    out = np.empty_like(val)
+    ctx.check(ctx.lib.kmsd_ic0_factor(ctx.h, L.shape[0], len(val), _np_ptr(ptr), _np_ptr(idx), _np_ptr(val),
+                                      float(shift), _np_ptr(out)))
+    return out
 
 
 class HipEngine:

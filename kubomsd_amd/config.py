@@ -174,6 +174,19 @@ class RunConfig:
             raise KuboMSDError("unknown metric_solver")
         if not _is_int(self.metric_refine_maxiter) or self.metric_refine_maxiter < 0:
             raise KuboMSDError("metric_refine_maxiter must be a nonnegative integer")
+        # config.py:121-129 of the reference: the direct solver takes no PCG
+        # options, and the full dense factor as a preconditioner is a CPU
+        # exact-solver feature.
+        if self.metric_solver == "cholesky":
+            if self.solver != "stochastic-kpm":
+                raise KuboMSDError("metric_solver=cholesky requires stochastic-kpm")
+            if (self.preconditioner != "diagonal" or self.metric_residual_policy != "fixed"
+                    or self.metric_check_interval != 32 or self.metric_gap_ratio != .1
+                    or self.metric_ic_ordering != "rcm" or self.metric_ic_shift_max != 1e-2
+                    or self.metric_gpu_check_batch != 1):
+                raise KuboMSDError("cholesky requires default PCG options; preconditioner is not applicable")
+        if self.preconditioner == "full-cholesky" and (self.solver != "exact" or backend != "cpu"):
+            raise KuboMSDError("full-cholesky is supported only for CPU EXACT")
         if self.precision not in ("fp64", "mixed"):
             raise KuboMSDError("precision must be fp64 or mixed")
         if not _is_int(self.device) or self.device < 0:

@@ -201,14 +201,24 @@ struct kmsd_metric {
     bool identity = false;
     int64_t n = 0;
     double *diag = nullptr;  // device f64[n]
-    // block-Jacobi: contiguous blocks, device ptr + factors
+    // block-Jacobi: contiguous blocks [block_ptr[q], block_ptr[q+1]), or
+    // cluster blocks whose rows are block_rows[block_ptr[q] .. block_ptr[q+1])
     int64_t nblocks = 0;
-    int32_t max_block = 0;
+    int32_t max_block = 0;           // largest block of at most SMALL_BLOCK rows
     int32_t *block_ptr = nullptr;    // device int32[nblocks+1]
     int64_t *factor_off = nullptr;   // device int64[nblocks] offsets into factors (complex entries)
     double *factors = nullptr;       // device cx<double>[sum m^2]
+    int32_t *block_rows = nullptr;   // device int32[n] (null: contiguous blocks)
+    int32_t *large = nullptr;        // device int32[nlarge]: blocks of more than SMALL_BLOCK rows
+    int64_t nlarge = 0;
+    size_t block_bytes = 0;          // device bytes of the block arrays
     std::vector<int64_t> host_block_ptr;
     bool uniform4 = false;           // every block is 4x4 aligned to multiples of 4
+    struct IcData;
+    IcData *ic = nullptr;            // IC(0) preconditioner (takes precedence over blocks)
+    bool direct = false;             // direct solve with the one full-size block plus refinement
+    int refine_maxiter = 3;
+    int64_t refinement_steps = 0;
     size_t bytes = 0;
     bool adaptive = false;
     int check_interval = 32;
@@ -240,6 +250,20 @@ struct kmsd_engine {
 };
 
 namespace kmsd {
+// Blocks of at most this many rows are solved one thread per (block, column)
+// in registers; larger ones one workgroup per (block, column).
+constexpr int32_t SMALL_BLOCK = 64;
+// Replaces the metric's blocks: block q holds rows off[q] .. off[q+1] of the
+// contiguous partition (rows == null) or rows[off[q] .. off[q+1]) (cluster
+// blocks); factors concatenated m x m row-major complex.  Validated by the caller.
+void metric_upload_blocks(kmsd_metric *m, int64_t nblocks, const int64_t *off, const int64_t *rows,
+                          const double *factors);
+void metric_release_blocks(kmsd_metric *m);
+void metric_release_ic(kmsd_metric *m);
+// z = M^-1 r with the IC(0) factor; R, Z (n x w) of dtype, internal leading dimension w.
+void ic_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags);
+// Blocks larger than SMALL_BLOCK rows (the small ones are the PCG's own kernel).
+void large_blocks_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags);
 // Metric solve on device blocks (any dtype); throws Error(KMSD_ENUMERIC) on
 // failure with the report stored in the metric.
 void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb,

@@ -25,15 +25,10 @@ extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *dia
         if (nblocks < 0 || (nblocks && (!block_ptr || !factors)))
             throw Error(KMSD_EINVAL, "invalid block-Jacobi factor partition");
         size_t nfac = 0;
-        bool uniform4 = nblocks > 0;
-        int32_t maxb = 0;
         for (int64_t q = 0; q < nblocks; ++q) {
             const int64_t a = block_ptr[q], b = block_ptr[q + 1];
             if ((q == 0 && a != 0) || b <= a || b > n) throw Error(KMSD_EINVAL, "invalid block-Jacobi factor partition");
-            if (b - a > 64) throw Error(KMSD_EINVAL, "block-Jacobi blocks larger than 64 orbitals are not supported");
             nfac += size_t((b - a) * (b - a));
-            uniform4 = uniform4 && (b - a == 4);
-            maxb = std::max<int32_t>(maxb, int32_t(b - a));
         }
         if (nblocks && block_ptr[nblocks] != n) throw Error(KMSD_EINVAL, "incomplete block-Jacobi partition");
         for (size_t e = 0; e < 2 * nfac; ++e)
@@ -41,32 +36,10 @@ extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *dia
         bool diag_one = true;
         for (int64_t i = 0; i < n && diag_one; ++i) diag_one = diag[i] == 1.0;
         m->identity = S->identity && diag_one && nblocks == 0;
-        m->nblocks = nblocks;
-        m->uniform4 = uniform4;
-        m->max_block = maxb;
         m->diag = static_cast<double *>(dalloc(c, sizeof(double) * n));
         m->bytes = sizeof(double) * n;
         HIPCHECK(hipMemcpyAsync(m->diag, diag, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        if (nblocks) {
-            std::vector<int32_t> bp(nblocks + 1);
-            std::vector<int64_t> off(nblocks);
-            size_t o = 0;
-            for (int64_t q = 0; q <= nblocks; ++q) bp[q] = int32_t(block_ptr[q]);
-            for (int64_t q = 0; q < nblocks; ++q) {
-                off[q] = int64_t(o);
-                o += size_t((block_ptr[q + 1] - block_ptr[q]) * (block_ptr[q + 1] - block_ptr[q]));
-            }
-            m->host_block_ptr.assign(block_ptr, block_ptr + nblocks + 1);
-            m->block_ptr = static_cast<int32_t *>(dalloc(c, sizeof(int32_t) * bp.size()));
-            m->factor_off = static_cast<int64_t *>(dalloc(c, sizeof(int64_t) * off.size()));
-            m->factors = static_cast<double *>(dalloc(c, sizeof(double) * 2 * nfac));
-            m->bytes += sizeof(int32_t) * bp.size() + sizeof(int64_t) * off.size() + sizeof(double) * 2 * nfac;
-            HIPCHECK(hipMemcpyAsync(m->block_ptr, bp.data(), sizeof(int32_t) * bp.size(), hipMemcpyHostToDevice,
-                                    c->stream));
-            HIPCHECK(hipMemcpyAsync(m->factor_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice,
-                                    c->stream));
-            HIPCHECK(hipMemcpyAsync(m->factors, factors, sizeof(double) * 2 * nfac, hipMemcpyHostToDevice, c->stream));
-        }
+        if (nblocks) metric_upload_blocks(m, nblocks, block_ptr, nullptr, factors);
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
     if (rc) {
@@ -87,11 +60,10 @@ extern "C" int kmsd_metric_destroy(kmsd_metric *m) {
         delete m->W;
         m->W = nullptr;
     }
-    const size_t nfac_bytes = m->bytes;  // accounted as a whole
+    metric_release_blocks(m);
+    metric_release_ic(m);
+    const size_t nfac_bytes = m->bytes;  // the diagonal
     if (m->diag) (void)hipFree(m->diag);
-    if (m->block_ptr) (void)hipFree(m->block_ptr);
-    if (m->factor_off) (void)hipFree(m->factor_off);
-    if (m->factors) (void)hipFree(m->factors);
     c->used -= std::min<uint64_t>(c->used, nfac_bytes);
     delete m;
     return KMSD_OK;

@@ -185,29 +185,43 @@ struct Solver {
         g2 = launch_step(c, dtype, s);
     }
 
+    // z = M^-1 r with the reference's precedence (cpp/native.cpp:237-297):
+    // IC(0), else the diagonal when there are no blocks, else the blocks
+    // (small ones in registers, large ones one workgroup each).
     void precondition(bool require_active, const int32_t *gate = nullptr) {
+        if (m->ic) {
+            ic_apply(m, dtype, R, Z, w, fl);
+            row(PS_DOT, nullptr, 0, nullptr, 0, require_active, gate);
+            return;
+        }
         if (m->nblocks == 0) {
             row(PS_DIAG, nullptr, 0, nullptr, 0, require_active, gate);
             return;
         }
-        const int64_t total = m->nblocks * w;
-        const int64_t g = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 16384));
-        if (dtype == KMSD_C128) {
-            if (m->max_block <= 4)
-                hipLaunchKernelGGL((block_precond_kernel<double, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream, R, Z,
-                                   int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
-            else
-                hipLaunchKernelGGL((block_precond_kernel<double, 64>), dim3(unsigned(g)), dim3(256), 0, c->stream, R,
-                                   Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
-        } else {
-            if (m->max_block <= 4)
-                hipLaunchKernelGGL((block_precond_kernel<float, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream, R, Z,
-                                   int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
-            else
-                hipLaunchKernelGGL((block_precond_kernel<float, 64>), dim3(unsigned(g)), dim3(256), 0, c->stream, R,
-                                   Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl);
+        if (m->nlarge < m->nblocks) {
+            const int64_t total = m->nblocks * w;
+            const int64_t g = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 16384));
+            const int32_t *rows = m->block_rows;
+            if (dtype == KMSD_C128) {
+                if (m->max_block <= 4)
+                    hipLaunchKernelGGL((block_precond_kernel<double, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream,
+                                       R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl, rows);
+                else
+                    hipLaunchKernelGGL((block_precond_kernel<double, SMALL_BLOCK>), dim3(unsigned(g)), dim3(256), 0,
+                                       c->stream, R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors,
+                                       m->nblocks, fl, rows);
+            } else {
+                if (m->max_block <= 4)
+                    hipLaunchKernelGGL((block_precond_kernel<float, 4>), dim3(unsigned(g)), dim3(256), 0, c->stream,
+                                       R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks, fl, rows);
+                else
+                    hipLaunchKernelGGL((block_precond_kernel<float, SMALL_BLOCK>), dim3(unsigned(g)), dim3(256), 0,
+                                       c->stream, R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors,
+                                       m->nblocks, fl, rows);
+            }
+            HIPCHECK(hipGetLastError());
         }
-        HIPCHECK(hipGetLastError());
+        large_blocks_apply(m, dtype, R, Z, w, fl);
         row(PS_DOT, nullptr, 0, nullptr, 0, require_active, gate);
     }
 
@@ -282,7 +296,7 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     // Fixed residual policy with a diagonal or uniform 4x4 preconditioner:
     // iterations without a true-residual refresh run fused (Y, R update with
     // z = M^-1 R and both reductions in one pass, P update recomputing z).
-    const bool fusable = !m->adaptive && (m->nblocks == 0 || m->uniform4);
+    const bool fusable = !m->adaptive && !m->ic && (m->nblocks == 0 || m->uniform4);
     for (int k = 0; k < maxiter; ++k) {
         if (k % poll == 0) {
             const PcgFlags h = S.read_flags();
@@ -363,11 +377,131 @@ void account(kmsd_metric *m, const Outcome &o, int64_t extra_checks) {
     m->restarts += o.fl.restarts;
 }
 
+// Per-column |T| from PS_TRUEGAP partials, summed in a fixed order.
+std::vector<double> true_norms(Solver &S) {
+    S.row(PS_TRUEGAP);
+    std::vector<double> h(size_t(S.rs.gx) * size_t(S.w) * 2), out(size_t(S.w), 0.0);
+    HIPCHECK(hipMemcpyAsync(h.data(), S.part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S.c->stream));
+    HIPCHECK(hipStreamSynchronize(S.c->stream));
+    for (int64_t g = 0; g < S.rs.gx; ++g)
+        for (int32_t j = 0; j < S.w; ++j) out[size_t(j)] += h[(size_t(g) * S.w + j) * 2];
+    for (double &v : out) v = std::sqrt(v);
+    return out;
+}
+
+// Direct solve with the one full-size factor (reference Metric::solve with
+// direct set, cpp/native.cpp:331-347): y = M^-1 q, then up to refine_maxiter
+// corrections M^-1 (q - S y) on the columns whose true residual is above
+// threshold (the others get a zero right-hand side), then the same final
+// acceptance as the PCG.  complex128 only; iterations stay 0.
+Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, double rtol, double atol,
+                   int64_t &checks) {
+    kmsd_metric *m = S.m;
+    kmsd_ctx *c = S.c;
+    const int32_t w = S.w;
+    const int64_t n = S.n;
+    S.clear_flag(0, sizeof(PcgFlags));
+    S.row(PS_ABSMAX, B, ldb);
+    S.col(PC_SAFE, S.part, S.rs.gx, rtol, atol);
+    S.row(PS_SCALE, B, ldb);  // Q = B/safe, R = Q, Y = 0
+    S.col(PC_INIT, S.part, S.rs.gx, rtol, atol);
+    auto copy_into_y = [&](bool add) {  // Y = Z or Y += Z
+        Step s;
+        s.Z = S.Z;
+        s.ldz = w;
+        if (add) {
+            s.P = S.Y;
+            s.ldp = w;
+            s.gamma = 1.0;
+            s.use_p = true;
+        }
+        s.OUT = S.Y;
+        s.ldo = w;
+        s.n = n;
+        s.w = w;
+        launch_step(c, KMSD_C128, s);
+    };
+    S.precondition(false);  // Z = M^-1 q
+    copy_into_y(false);
+    std::vector<ColState> cs(static_cast<size_t>(w));
+    HIPCHECK(hipMemcpyAsync(cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int step = 0; step < m->refine_maxiter; ++step) {
+        S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);  // T = q - S y
+        ++checks;
+        const std::vector<double> tn = true_norms(S);
+        bool needed = false;
+        for (int32_t j = 0; j < w; ++j) {
+            cs[size_t(j)].restart = tn[size_t(j)] > cs[size_t(j)].thr;
+            needed = needed || cs[size_t(j)].restart;
+        }
+        if (!needed) break;
+        // R = T on the columns still above threshold, 0 elsewhere
+        HIPCHECK(hipMemcpyAsync(S.cs, cs.data(), sizeof(ColState) * w, hipMemcpyHostToDevice, c->stream));
+        launch_scale(c, KMSD_C128, S.R, w, S.R, w, n, w, Cplx{0, 0});
+        S.row(PS_RESTART);
+        S.precondition(false);
+        copy_into_y(true);
+        ++m->refinement_steps;
+    }
+    for (ColState &x : cs) x.restart = 0;
+    HIPCHECK(hipMemcpyAsync(S.cs, cs.data(), sizeof(ColState) * w, hipMemcpyHostToDevice, c->stream));
+    S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
+    S.row(PS_TRUEGAP);
+    S.col(PC_FINAL, S.part, S.rs.gx, rtol, atol);
+    S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
+    Outcome o;
+    o.cs.resize(w);
+    HIPCHECK(hipMemcpyAsync(o.cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(&o.fl, S.fl, sizeof(PcgFlags), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return o;
+}
+
+void direct_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb,
+                  double rtol, double atol) {
+    kmsd_ctx *c = m->ctx;
+    const int64_t n = m->n;
+    int64_t checks = 0;
+    Outcome o;
+    bool fin_all = true;
+    if (dtype == KMSD_C128) {
+        Solver S = make_solver(m, KMSD_C128, nb);
+        o = direct_run(S, B, ldb, Y, ldy, rtol, atol, checks);
+    } else {
+        // complex64 blocks: the direct solve runs in complex128 throughout.
+        if (!m->work) m->work = new kmsd_metric::Work();
+        kmsd_metric::Work &wk = *m->work;
+        void *bb = wk.b128.ensure(c, size_t(n) * nb * 16);
+        void *yy = wk.y128.ensure(c, size_t(n) * nb * 16);
+        launch_convert(c, KMSD_C64, B, ldb, KMSD_C128, bb, nb, n, nb);
+        Solver S = make_solver(m, KMSD_C128, nb);
+        o = direct_run(S, bb, nb, yy, nb, rtol, atol, checks);
+        launch_convert(c, KMSD_C128, yy, nb, KMSD_C64, Y, ldy, n, nb);
+        fin_all = !launch_nonfinite(c, KMSD_C64, Y, ldy, n, nb);
+        for (ColState &s : o.cs) s.fin = s.fin && fin_all;
+    }
+    store_report(m, o.cs);
+    ++m->calls;
+    m->refreshes += checks;
+    m->residual_checks += checks + 1;
+    bool ok = true, fin = true;
+    for (const ColState &s : o.cs) fin = fin && s.fin;
+    for (size_t j = 0; j < m->rep_conv.size(); ++j) ok = ok && m->rep_conv[j];
+    if (!ok)
+        throw Error(KMSD_ENUMERIC, !fin ? "S solve solution is nonfinite or not representable in complex128 after rescaling"
+                                        : "S solve failed true-residual acceptance");
+}
+
 }  // namespace
 
 void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb, double rtol,
                double atol, int maxiter) {
     kmsd_ctx *c = m->ctx;
+    if (m->direct) {
+        direct_solve(m, dtype, B, ldb, Y, ldy, nb, rtol, atol);
+        return;
+    }
     if (dtype == KMSD_C128) {
         Solver S = make_solver(m, dtype, nb);
         const Outcome o = pcg_run(S, B, ldb, Y, ldy, rtol, atol, maxiter);

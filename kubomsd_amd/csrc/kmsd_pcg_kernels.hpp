@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
 template <class VT, int MAXB>
 __global__ __launch_bounds__(256) void block_precond_kernel(const void *Rv, void *Zv, int64_t w, const int32_t *bptr,
                                                             const int64_t *foff, const double *fac, int64_t nblocks,
-                                                            const PcgFlags *fl) {
+                                                            const PcgFlags *fl, const int32_t *rows) {
     if (fl && fl->broke) return;
     const cx<VT> *R = static_cast<const cx<VT> *>(Rv);
     cx<VT> *Z = static_cast<cx<VT> *>(Zv);
@@ -700,10 +700,13 @@ __global__ __launch_bounds__(256) void block_precond_kernel(const void *Rv, void
     for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
         const int64_t q = e / w, j = e - q * w;
         const int32_t a0 = bptr[q], m = bptr[q + 1] - a0;
+        if (m > MAXB) continue;  // large blocks: one workgroup each (large_block_kernel)
+        const int32_t *rq = rows ? rows + a0 : nullptr;
+        const auto row = [&](int i) -> int64_t { return rq ? int64_t(rq[i]) : int64_t(a0 + i); };
         const cx<double> *Lq = L + foff[q];
         cx<double> v[MAXB];
         for (int i = 0; i < m; ++i) {
-            cx<double> s = to<double>(R[int64_t(a0 + i) * w + j]);
+            cx<double> s = to<double>(R[row(i) * w + j]);
             for (int k = 0; k < i; ++k) {
                 const cx<double> l = Lq[i * m + k];
                 s.re -= l.re * v[k].re - l.im * v[k].im;
@@ -724,7 +727,7 @@ __global__ __launch_bounds__(256) void block_precond_kernel(const void *Rv, void
             const double den = d.re * d.re + d.im * d.im;
             v[i] = {(s.re * d.re - s.im * d.im) / den, (s.im * d.re + s.re * d.im) / den};
         }
-        for (int i = 0; i < m; ++i) Z[int64_t(a0 + i) * w + j] = to<VT>(v[i]);
+        for (int i = 0; i < m; ++i) Z[row(i) * w + j] = to<VT>(v[i]);
     }
 }
 

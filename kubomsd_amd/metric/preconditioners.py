@@ -1,0 +1,96 @@
+"""Set-up of the cluster-Jacobi and IC(0) preconditioners of the metric solve
+(reference src/kubo_msd/metric/preconditioners.py:11-106).  The physical S is
+never modified: both act on fixed Hermitian positive definite approximations.
+
+  cluster-Jacobi: a dense lower Cholesky factor per partition block (rows of
+      S selected by the block's orbitals), factored on the host like every
+      other small block factor, solved on the device;
+  IC(0): W = D S D with D = diag(S)^-1/2, permuted by reverse Cuthill-McKee
+      (or kept in natural order), incomplete Cholesky on W's lower pattern,
+      computed on the device (kmsd_ic0_factor); on a pivot failure the next
+      diagonal shift of 0, 1e-12, 1e-10, ..., 1e-2 up to `shift_max` is tried.
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as la
+import scipy.sparse as sp
+
+from ..errors import KuboMSDError, NumericalError
+
+IC_SHIFTS = (0, 1e-12, 1e-10, 1e-8, 1e-6, 1e-4, 1e-2)
+
+
+@dataclass
+class PreconditionerData:
+    kind: str
+    blocks: list = field(default_factory=list)  # cluster-jacobi: [(rows, L)]
+    factor: object = None                         # ic0: CSR factor of the permuted W
+    permutation: object = None                    # ic0: original row of each factor row
+    scaling: object = None                        # ic0: D per original row
+    report: dict = field(default_factory=dict)
+
+
+def prepare_cluster(S, partition):
+    """Factors S[rows][:, rows] for every partition block."""
+    S = sp.csr_matrix(S)
+    blocks = []
+    for block in partition["blocks"]:
+        rows = np.asarray(block["orbitals"], dtype=np.int64)
+        try:
+            L = la.cholesky(S[rows][:, rows].toarray(), lower=True, check_finite=False)
+        except la.LinAlgError as exc:
+            raise NumericalError("cluster block of the overlap is not positive definite") from exc
+        rows.setflags(write=False)
+        L.setflags(write=False)
+        blocks.append((rows, L))
+    return PreconditionerData("cluster-jacobi", blocks=blocks,
+                              report=dict(block_count=len(blocks), block_sizes=[len(r) for r, _ in blocks],
+                                          factor_bytes=sum(r.nbytes + L.nbytes for r, L in blocks)))
+
+
+def prepare_ic0(ctx, S, *, ordering, shift_max):
+    """IC(0) data for the overlap S; S must be Hermitian positive definite
+    (a shift in the preconditioner never admits an indefinite S)."""
+    from .bounds import spd_bounds
+    S = sp.csr_matrix(S, dtype=np.complex128)
+    spd_bounds(S)
+    return prepare_ic0_validated(ctx, S, ordering=ordering, shift_max=shift_max)
+
+
+def prepare_ic0_validated(ctx, S, *, ordering, shift_max):
+    from ..backends.hip import ic0_factor
+    if ordering not in ("rcm", "natural") or not np.isfinite(shift_max) or shift_max < 0:
+        raise NumericalError("invalid IC ordering/shift limit")
+    start = time.perf_counter()
+    z = 1 / np.sqrt(S.diagonal().real)
+    W = (sp.diags(z) @ S @ sp.diags(z)).tocsr()
+    W.eliminate_zeros()
+    W.sort_indices()
+    p = (sp.csgraph.reverse_cuthill_mckee(W, symmetric_mode=True) if ordering == "rcm"
+         else np.arange(S.shape[0], dtype=np.int64))
+    lower = sp.tril(W[p][:, p], format="csr")
+    lower.sort_indices()
+    attempts = []
+    for shift in IC_SHIFTS:
+        if shift > shift_max:
+            continue
+        try:
+            values = ic0_factor(ctx, lower, shift)
+        except KuboMSDError as exc:
+            attempts.append(dict(shift=shift, status="FAILED", reason=str(exc)))
+            continue
+        attempts.append(dict(shift=shift, status="PASS"))
+        L = sp.csr_matrix((values, lower.indices.copy(), lower.indptr.copy()), shape=S.shape)
+        import scipy
+        report = dict(ordering=ordering, shift=shift, shift_applies_to="M only, normalized coordinates",
+                      attempts=attempts, nnz=L.nnz, factor_bytes=L.data.nbytes + L.indices.nbytes + L.indptr.nbytes,
+                      minimum_pivot=float(np.min(L.diagonal().real ** 2)), seconds=time.perf_counter() - start,
+                      permutation=p.tolist(), scipy_version=scipy.__version__, factor_device="hip")
+        for a in (L.data, L.indices, L.indptr, z, p):
+            a.setflags(write=False)
+        return PreconditionerData("ic0", factor=L, permutation=np.asarray(p, np.int64), scaling=z, report=report)
+    failure = NumericalError("IC(0) failed all allowed shifts; original S was not modified")
+    failure.diagnostics = dict(attempts=attempts, ordering=ordering, shift_max=shift_max)
+    raise failure

@@ -10,6 +10,7 @@ bond/xbar/delta form) once; every action below runs on device tensors:
   solve_s(B)                          ops.native: the fused HipEngine
 Vectors are complex128 (precision='fp64') or complex64 (precision='mixed').
 """
+import time
 from types import SimpleNamespace
 
 import numpy as np
@@ -38,33 +39,73 @@ def hermitian(matrix, name):
 
 
 class DeviceMetric:
-    """Host-side metric set-up plus the device PCG (reference MetricSolver,
-    src/kubo_msd/metric/solve.py:11-174, restricted to the PCG solver with the
-    diagonal and atom-block Jacobi preconditioners)."""
+    """Host-side metric set-up plus the device solves (reference MetricSolver,
+    src/kubo_msd/metric/solve.py:11-174 and metric/setup.py:40-91): batched
+    PCG with the diagonal, atom-block Jacobi, cluster-Jacobi or IC(0)
+    preconditioner, or the direct solver (metric_solver='cholesky': one dense
+    factor of S plus refinement on the original equation)."""
 
-    def __init__(self, ctx, S, config, atom_slices=None):
-        if config.metric_solver != "pcg":
-            raise KuboMSDError("the HIP backend solves the metric by PCG (metric_solver='pcg')")
-        if config.preconditioner not in ("diagonal", "block-jacobi"):
-            raise KuboMSDError("the HIP backend supports the diagonal and block-jacobi preconditioners")
+    def __init__(self, ctx, S, config, atom_slices=None, metric_identity=None):
+        if config.preconditioner == "full-cholesky":
+            raise KuboMSDError("full-cholesky is supported only for CPU EXACT")
         self.config = config
         self.host_S = S
-        self.s_lower, self.s_upper, self.bounds_report = spd_bounds(S)
-        self.diagonal = S.diagonal().real.copy()
+        n = S.shape[0]
+        self.direct = config.metric_solver == "cholesky"
+        self.factor_seconds = 0.0
         blocks = []
+        if self.direct:
+            # metric/cholesky.py:12-20: budget the dense factor before forming it.
+            storage = S.data.nbytes + S.indices.nbytes + S.indptr.nbytes
+            if 2 * storage + 4 * 16 * n * n > config.memory_limit_gb * 1024 ** 3:
+                raise NumericalError("Cholesky factor/workspace exceeds memory budget")
+            self.s_lower, self.s_upper, self.bounds_report = spd_bounds(S, allow_dense=False)
+            t0 = time.perf_counter()
+            try:
+                L = la.cholesky(S.toarray(), lower=True, check_finite=False)
+            except la.LinAlgError as exc:
+                raise NumericalError("overlap is not positive definite (Cholesky factorization)") from exc
+            if not np.isfinite(L).all():
+                raise NumericalError("Cholesky produced nonfinite factor")
+            self.factor_seconds = time.perf_counter() - t0
+            self.factor_bytes = L.nbytes
+            blocks = [(0, n, L)]
+        else:
+            self.s_lower, self.s_upper, self.bounds_report = spd_bounds(S)
+        self.diagonal = S.diagonal().real.copy()
         if config.preconditioner == "block-jacobi":
-            slices = atom_slices or [(j, j + 1) for j in range(S.shape[0])]
-            if [i for a, b in slices for i in range(a, b)] != list(range(S.shape[0])):
+            slices = atom_slices or [(j, j + 1) for j in range(n)]
+            if [i for a, b in slices for i in range(a, b)] != list(range(n)):
                 raise NumericalError("atom_slices must partition orbitals")
             for a, b in slices:
                 blocks.append((a, b, la.cholesky(S[a:b, a:b].toarray(), lower=True)))
         self.blocks = blocks
         self.device = HipMetric(ctx, S, self.diagonal, blocks)
+        self.preconditioner_data = None
+        if self.direct:
+            self.device.set_direct(True, config.metric_refine_maxiter)
+        if config.preconditioner == "cluster-jacobi":
+            from ..metric.partition import load_metric_partition, matrix_fingerprint
+            from ..metric.preconditioners import prepare_cluster
+            if not metric_identity or not all(metric_identity.values()):
+                raise NumericalError("cluster-Jacobi requires explicit model identity")
+            partition = load_metric_partition(config.metric_partition_file, s_fingerprint=matrix_fingerprint(S),
+                                              atom_slices=atom_slices, n_orbitals=n, **metric_identity)
+            self.preconditioner_data = prepare_cluster(S, partition)
+            self.device.set_index_blocks(self.preconditioner_data.blocks)
+        if config.preconditioner == "ic0":
+            from ..metric.preconditioners import prepare_ic0_validated
+            self.preconditioner_data = prepare_ic0_validated(ctx, S, ordering=config.metric_ic_ordering,
+                                                             shift_max=config.metric_ic_shift_max)
+            d = self.preconditioner_data
+            self.device.set_ic0(d.factor, d.permutation, d.scaling)
         self.device.set_residual_policy(config.metric_residual_policy, config.metric_check_interval,
                                         config.metric_gap_ratio)
         ctx.set_tuning("pcg_poll", max(1, int(config.metric_gpu_check_batch)) if
                        config.metric_gpu_check_batch > 1 else 4)
-        self.identity = bool(S.nnz == S.shape[0] and np.all(S.diagonal() == 1))
+        # The device skips solves only for S = I with the diagonal preconditioner.
+        self.identity = bool(S.nnz == n and np.all(S.diagonal() == 1) and config.preconditioner == "diagonal"
+                             and not self.direct)
 
     def solve_s(self, B, config=None):
         cfg = self.config if config is None else config
@@ -78,7 +119,13 @@ class DeviceMetric:
     def diagnostics(self):
         d = self.device.diagnostics()
         d.update(kernel="hip", bounds=self.bounds_report, condition_upper_or_estimate=self.s_upper / self.s_lower,
-                 solver="pcg", resolved_preconditioner=self.config.preconditioner)
+                 solver=self.config.metric_solver,
+                 resolved_preconditioner="not-applicable" if self.direct else self.config.preconditioner)
+        if self.direct:
+            d.update(factorizations=1, factor_bytes=int(self.factor_bytes), factor_seconds=self.factor_seconds,
+                     factor_backend="host LAPACK (solves on the device)")
+        if self.preconditioner_data is not None:
+            d["preconditioner_setup"] = self.preconditioner_data.report
         return d
 
 
@@ -94,7 +141,9 @@ class OperatorBundle:
         budget = int(config.device_memory_limit_gb * 1024 ** 3)
         self.ctx = ctx or HipContext(config.device, budget)
         self.dtype = torch.complex128 if config.precision == "fp64" else torch.complex64
-        self.metric = DeviceMetric(self.ctx, self.host_S, config, model.atom_slices)
+        from ..metric.partition import model_identity
+        self.metric = DeviceMetric(self.ctx, self.host_S, config, model.atom_slices,
+                                   metric_identity=model_identity(model))
         self.velocity = {a: hermitian(v, "velocity " + a) for a, v in model.velocity.items()}
         self.position = {a: hermitian(v, "position " + a) for a, v in model.position.items()}
         self.composite = {a: tuple(sp.csr_matrix(v, dtype=np.complex128) for v in parts)
