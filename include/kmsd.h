@@ -77,8 +77,9 @@ int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
  * "moment_chunk" (Chebyshev orders per partial-reduction launch),
  * "fp32_math" (complex64 blocks: FP32 values and row sums), "pcg_poll" (PCG
  * iterations between host polls), "graphs" (1: moment chains with an
- * identity metric replay whole chunks of orders as one HIP graph; 0: launch
- * every order; same results either way), "ell_kernel" (1, the default:
+ * identity metric replay whole chunks of orders as one HIP graph, and the
+ * level-by-level IC(0) sweeps of a preconditioner application replay as one
+ * graph; 0: launch every kernel; same results either way), "ell_kernel" (1, the default:
  * products whose off-diagonal part is ELL of width 3, 4 or 6 use the kernel
  * that prefetches the next row's indices; 0: the generic row kernel; same
  * results either way). */

@@ -38,9 +38,28 @@ struct kmsd_metric::IcData {
     std::vector<int64_t> fwd_lev, bwd_lev;  // level boundaries into fwd / bwd
     std::vector<std::pair<void *, size_t>> allocs;
     DBuf work;  // n x w complex128 solve workspace
+    // The two sweeps are one launch per level; they are captured once per
+    // argument set as a HIP graph and replayed (same kernels, same
+    // arguments, bitwise the same result as eager launches).
+    struct Graph {
+        int dtype;
+        int32_t w;
+        const void *R, *Z, *W, *fl;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    ~IcData() {
+        for (Graph &g : graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+    }
 };
 
 namespace {
+
+constexpr int IC_CHUNK = 8;  // factor entries whose loads an IC solve thread issues together
 
 #pragma clang fp contract(off)
 
@@ -129,10 +148,22 @@ __global__ __launch_bounds__(256) void ic_fwd_kernel(const void *Rv, int64_t w, 
     const cx<double> x = to<double>(R[o * w + j]);
     const double s = scale[o];
     cx<double> v{s * x.re, s * x.im};
-    for (int64_t k = ptr[r]; k < d; ++k) {
-        const cx<double> l = val[k], u = W[int64_t(idx[k]) * w + j];
-        v.re -= l.re * u.re - l.im * u.im;
-        v.im -= l.re * u.im + l.im * u.re;
+    // Entries in chunks of IC_CHUNK: all loads of a chunk are issued before
+    // its products, which are then subtracted in entry order.
+    for (int64_t k0 = ptr[r]; k0 < d; k0 += IC_CHUNK) {
+        cx<double> l[IC_CHUNK], u[IC_CHUNK];
+#pragma unroll
+        for (int q = 0; q < IC_CHUNK; ++q) {
+            const int64_t k = k0 + q < d ? k0 + q : d;  // past the row: the diagonal, unused
+            l[q] = val[k];
+            u[q] = W[int64_t(k0 + q < d ? idx[k] : 0) * w + j];
+        }
+#pragma unroll
+        for (int q = 0; q < IC_CHUNK; ++q)
+            if (k0 + q < d) {
+                v.re -= l[q].re * u[q].re - l[q].im * u[q].im;
+                v.im -= l[q].re * u[q].im + l[q].im * u[q].re;
+            }
     }
     const double dd = val[d].re;  // the factor's diagonal is real
     W[r * w + j] = cx<double>{v.re / dd, v.im / dd};
@@ -151,10 +182,21 @@ __global__ __launch_bounds__(256) void ic_bwd_kernel(void *Zv, int64_t w, const 
     cx<VT> *Z = static_cast<cx<VT> *>(Zv);
     const int64_t r = rows[t / w], j = t % w;
     cx<double> v = W[r * w + j];
-    for (int64_t k = uptr[r]; k < uptr[r + 1]; ++k) {
-        const cx<double> l = uval[k], x = W[int64_t(urow[k]) * w + j];
-        v.re -= l.re * x.re + l.im * x.im;
-        v.im -= l.re * x.im - l.im * x.re;
+    const int64_t e = uptr[r + 1];
+    for (int64_t k0 = uptr[r]; k0 < e; k0 += IC_CHUNK) {
+        cx<double> l[IC_CHUNK], x[IC_CHUNK];
+#pragma unroll
+        for (int q = 0; q < IC_CHUNK; ++q) {
+            const bool in = k0 + q < e;
+            l[q] = in ? uval[k0 + q] : cx<double>{0.0, 0.0};
+            x[q] = W[int64_t(in ? urow[k0 + q] : r) * w + j];
+        }
+#pragma unroll
+        for (int q = 0; q < IC_CHUNK; ++q)
+            if (k0 + q < e) {
+                v.re -= l[q].re * x[q].re + l[q].im * x[q].im;
+                v.im -= l[q].re * x[q].im - l[q].im * x[q].re;
+            }
     }
     const double dd = val[ptr[r + 1] - 1].re;
     v = cx<double>{v.re / dd, v.im / dd};
@@ -311,11 +353,63 @@ void metric_release_ic(kmsd_metric *m) {
     m->ic = nullptr;
 }
 
+namespace {
+
+void ic_sweeps(kmsd_ctx *c, kmsd_metric::IcData &ic, int dtype, const void *R, void *Z, int32_t w, cx<double> *W,
+               const PcgFlags *fl);
+
+}  // namespace
+
 void ic_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags) {
     kmsd_ctx *c = m->ctx;
     kmsd_metric::IcData &ic = *m->ic;
     const PcgFlags *fl = static_cast<const PcgFlags *>(flags);
     cx<double> *W = static_cast<cx<double> *>(ic.work.ensure(c, size_t(ic.n) * size_t(w) * 16));
+    if (!c->graphs) {
+        ic_sweeps(c, ic, dtype, R, Z, w, W, fl);
+        return;
+    }
+    for (const auto &g : ic.graphs)
+        if (g.dtype == dtype && g.w == w && g.R == R && g.Z == Z && g.W == W && g.fl == fl) {
+            HIPCHECK(hipGraphLaunch(g.exec, c->stream));
+            return;
+        }
+    if (!c->capture) HIPCHECK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
+    hipStream_t user = c->stream;
+    c->stream = c->capture;
+    HIPCHECK_RESTORE(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), c, user);
+    try {
+        ic_sweeps(c, ic, dtype, R, Z, w, W, fl);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c->stream = user;
+        throw;
+    }
+    kmsd_metric::IcData::Graph g{dtype, w, R, Z, W, fl, nullptr, nullptr};
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g.graph);
+    c->stream = user;
+    HIPCHECK(ec);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g.graph);
+        HIPCHECK(ei);
+    }
+    if (ic.graphs.size() >= 8) {  // a few argument sets per metric (dtype, width, workspace)
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(ic.graphs.front().exec);
+        (void)hipGraphDestroy(ic.graphs.front().graph);
+        ic.graphs.erase(ic.graphs.begin());
+    }
+    ic.graphs.push_back(g);
+    HIPCHECK(hipGraphLaunch(g.exec, c->stream));
+}
+
+namespace {
+
+void ic_sweeps(kmsd_ctx *c, kmsd_metric::IcData &ic, int dtype, const void *R, void *Z, int32_t w, cx<double> *W,
+               const PcgFlags *fl) {
     const cx<double> *val = reinterpret_cast<const cx<double> *>(ic.val);
     const cx<double> *uval = reinterpret_cast<const cx<double> *>(ic.uval);
     for (size_t l = 0; l + 1 < ic.fwd_lev.size(); ++l) {
@@ -340,6 +434,8 @@ void ic_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, cons
     }
     HIPCHECK(hipGetLastError());
 }
+
+}  // namespace
 
 void large_blocks_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags) {
     if (!m->nlarge) return;
