@@ -156,24 +156,20 @@ __global__ __launch_bounds__(256, 2) void mu_f32_kernel(const float *L, const fl
     store_partials(part, wk.z, m, k, j0, q0, cre, cim);
 }
 
-__device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
-    const unsigned u = __builtin_bit_cast(unsigned, x);
-    return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float bf16_to_float(unsigned short b) {
-    return __builtin_bit_cast(float, static_cast<unsigned>(b) << 16);
-}
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 b2 __attribute__((ext_vector_type(2)));
 
-// Eight floats -> eight bf16 high parts and eight bf16 residuals, packed.
+// Eight floats -> eight bf16 high parts (round to nearest even) and eight
+// bf16 residuals x - hi (exact in FP32), packed two per word.  Two-value
+// vector conversions map to gfx950's packed v_cvt_pk_bf16_f32.
 __device__ __forceinline__ void split8(f4 a, f4 b, u4 &hi, u4 &lo) {
-    float x[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+    const f2 x[4] = {{a[0], a[1]}, {a[2], a[3]}, {b[0], b[1]}, {b[2], b[3]}};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const unsigned short h0 = bf16_bits_rne(x[2 * e]), h1 = bf16_bits_rne(x[2 * e + 1]);
-        const unsigned short l0 = bf16_bits_rne(x[2 * e] - bf16_to_float(h0));
-        const unsigned short l1 = bf16_bits_rne(x[2 * e + 1] - bf16_to_float(h1));
-        hi[e] = unsigned(h0) | (unsigned(h1) << 16);
-        lo[e] = unsigned(l0) | (unsigned(l1) << 16);
+        const b2 h = __builtin_convertvector(x[e], b2);
+        const b2 l = __builtin_convertvector(x[e] - __builtin_convertvector(h, f2), b2);
+        hi[e] = __builtin_bit_cast(unsigned, h);
+        lo[e] = __builtin_bit_cast(unsigned, l);
     }
 }
 
