@@ -2,8 +2,9 @@
 (reference src/kubo_msd/solvers/stochastic.py:21-228; device residency and
 batch admission as in the GPU edition's solvers/stochastic_gpu.py:18-145).
 
-Samples are taken in the global order of statistics blocks and batches, so
-results do not depend on the batch width.  DOS and numerators stay on the
+Samples are taken in global order, `sample_batch` at a time (a batch may
+span statistics blocks), so results do not depend on the batch width beyond
+rounding.  DOS and numerators stay on the
 device: each batch's per-sample arrays are added into device-resident block
 sums in sample order, the MSD time gradient, the paired block bootstrap and
 the validity masks run as device kernels (observables/statistics_device.py),
@@ -42,8 +43,10 @@ def batch_workspace_bytes(ops, cfg, method, nb):
     es = 16 if cfg.precision == "fp64" else 8
     na = len(cfg.axes)
     w = nb * (1 + na)
-    # total + series out + 2 recursion buffers + projection block + chain buffers + PCG work (7 blocks)
-    vec = ops.n * es * (4 * w + 3 * na * nb + (0 if ops.metric.identity else 7 * w))
+    # The augmented state is complex128 in both precisions: total + series
+    # out + 2 recursion buffers (+ PCG work, 7 blocks, for a nonorthogonal
+    # metric); the moment chains of the projections run in the storage type.
+    vec = ops.n * (16 * (4 * w + (0 if ops.metric.identity else 7 * w)) + es * 3 * na * nb)
     if method == "spectral" or cfg.include_vac:
         vec = max(vec, spectral_workspace_bytes(ops, cfg, nb))
     return int(vec)
@@ -65,37 +68,46 @@ def _host(x):
 
 
 def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
+    """Runs the samples in global order, `batch` at a time, and adds each
+    sample's DOS and numerators to its statistics block in sample order
+    (reference _sample_batches, solvers/stochastic.py:32-71).  The reference
+    cuts batches at statistics-block boundaries; here a batch may span
+    blocks, because a sample's results do not depend on the batch it runs in
+    (tests/unit/test_shared_axes.py:5-16) and the device runs wide blocks of
+    vectors far faster than narrow ones.  Block sums keep the reference's
+    per-sample order."""
     diagnostics = []
     primary = "commutator" if method == "stepwise" else "vac_integral"
     ns = cfg.nsamples if samples is None else samples.shape[1]
-    nbk = dos_blocks.shape[0]
-    for block in range(nbk):
-        first_id = block * cfg.statistics_block
-        end_id = min(ns, first_id + cfg.statistics_block)
-        for first in range(first_id, end_id, batch):
-            last = min(first + batch, end_id)
-            ids = np.arange(first, last)
-            B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else ops.asarray(samples[:, first:last])
-            if method == "stepwise":
-                dos, nums, diag = stepwise_samples(ops, cfg, B)
-                if cfg.include_vac:
-                    _, extra, extra_diag = spectral_samples(ops, cfg, B)
-                    nums.update(extra)
-                    diag["additional_vac_path"] = extra_diag
-                    diag["additional_cost_recorded"] = True
-            else:
-                dos, nums, diag = spectral_samples(ops, cfg, B)
-            if len(cfg.time_fs) > 1:
-                nums["diffusion_A2_per_fs/msd_gradient"] = stats.msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
-                                                                              axis=2)
-            diagnostics.append(diag)
-            # Per-sample sums into the block, in sample order (the reference's
-            # loop of += per sample), with the nonfinite checks as flags.
-            stats.clear_flags()
-            stats.accumulate(dos_blocks[block], dos, FLAG_DOS)
+    sb = cfg.statistics_block
+    for first in range(0, ns, batch):
+        last = min(first + batch, ns)
+        ids = np.arange(first, last)
+        B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else ops.asarray(samples[:, first:last])
+        if method == "stepwise":
+            dos, nums, diag = stepwise_samples(ops, cfg, B)
+            if cfg.include_vac:
+                _, extra, extra_diag = spectral_samples(ops, cfg, B)
+                nums.update(extra)
+                diag["additional_vac_path"] = extra_diag
+                diag["additional_cost_recorded"] = True
+        else:
+            dos, nums, diag = spectral_samples(ops, cfg, B)
+        if len(cfg.time_fs) > 1:
+            nums["diffusion_A2_per_fs/msd_gradient"] = stats.msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
+                                                                          axis=2)
+        diag["sample_ids"] = [int(first), int(last)]
+        diagnostics.append(diag)
+        # Per-sample sums into each block the batch touches, in sample order
+        # (the reference's loop of += per sample), with the nonfinite checks
+        # as device flags.
+        stats.clear_flags()
+        for block in range(first // sb, (last - 1) // sb + 1):
+            lo, hi = max(first, block * sb) - first, min(last, (block + 1) * sb) - first
+            stats.accumulate(dos_blocks[block], dos[lo:hi], FLAG_DOS)
             for key, value in nums.items():
-                stats.accumulate(accum[key][block], value, FLAG_NUMERATOR)
-            stats.check_flags()
+                stats.accumulate(accum[key][block], value[lo:hi], FLAG_NUMERATOR)
+        stats.check_flags()
     return diagnostics
 
 
