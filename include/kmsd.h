@@ -180,6 +180,12 @@ int kmsd_metric_set_ic0(kmsd_metric *m, int64_t nnz, const int64_t *rowptr, cons
  * are reported as 0.  Enabling needs exactly one block covering all rows
  * ("invalid direct metric factor"). */
 int kmsd_metric_set_direct(kmsd_metric *m, int enabled, int steps);
+/* Reference: MetricSolver.precondition(R) (src/kubo_msd/metric/solve.py:59-77;
+ * the compiled Metric applies it only inside solve): Z = M^-1 R with the
+ * metric's preconditioner (IC(0), else the diagonal without blocks, else the
+ * blocks), one application; R and Z are (n, nb) device blocks that must not
+ * overlap. */
+int kmsd_metric_precondition(kmsd_metric *m, const void *R, void *Z, int64_t nb, int dtype);
 /* Reference: Metric.diagnostics()["refinement_steps"] (cpp/native.cpp:668-672). */
 int kmsd_metric_refinement_steps(const kmsd_metric *m, int64_t *out);
 /* Reference: ic0_factor(CSR, shift) (cpp/native.cpp:92-121, 614-618).  IC(0)

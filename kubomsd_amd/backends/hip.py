@@ -56,6 +56,7 @@ SIGNATURES = {
     "kmsd_metric_set_ic0": [_P, _I64, _P, _P, _P, _P, _P],
     "kmsd_metric_set_direct": [_P, _I, _I],
     "kmsd_metric_refinement_steps": [_P, _P],
+    "kmsd_metric_precondition": [_P, _P, _P, _I64, _I],
     "kmsd_ic0_factor": [_P, _I64, _I64, _P, _P, _P, _D, _P],
     "kmsd_engine_create": [_P, _P, _P, _P, _P, _D, _D, _I, _D, _P],
     "kmsd_engine_destroy": [_P],
@@ -418,6 +419,16 @@ class HipMetric:
         val = np.ascontiguousarray(L.data, np.complex128)
         self.ctx.check(self.ctx.lib.kmsd_metric_set_ic0(self.h, len(val), _np_ptr(ptr), _np_ptr(idx), _np_ptr(val),
                                                        _np_ptr(p), _np_ptr(z)))
+
+    def precondition(self, R):
+        """Z = M^-1 R, one application of the metric's preconditioner."""
+        ctx = self.ctx
+        R = ctx.block(R)
+        ctx.bind_stream()
+        out = _torch().empty_like(R)
+        ctx.check(ctx.lib.kmsd_metric_precondition(self.h, C.c_void_p(R.data_ptr()), C.c_void_p(out.data_ptr()),
+                                                  R.shape[1], _dtype_code(R)))
+        return out
 
     def set_direct(self, enabled, steps):
         """Direct solves with the one full-size block factor plus `steps`

@@ -453,6 +453,60 @@ void large_blocks_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32
 
 }  // namespace kmsd
 
+namespace {
+
+template <class VT>
+__global__ __launch_bounds__(256) void diag_precond_kernel(const void *Rv, void *Zv, int64_t n, int64_t w,
+                                                           const double *d) {
+    const cx<VT> *R = static_cast<const cx<VT> *>(Rv);
+    cx<VT> *Z = static_cast<cx<VT> *>(Zv);
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < n * w; e += int64_t(gridDim.x) * 256) {
+        const cx<double> r = to<double>(R[e]);
+        const double di = d[e / w];
+        Z[e] = to<VT>(cx<double>{r.re / di, r.im / di});
+    }
+}
+
+}  // namespace
+
+extern "C" int kmsd_metric_precondition(kmsd_metric *m, const void *R, void *Z, int64_t nb, int dtype) {
+    if (!m) return KMSD_EINVAL;
+    kmsd_ctx *c = m->ctx;
+    return guard(c, [&] {
+        check_block_args(R, Z, nb, dtype);
+        if (R == Z) throw Error(KMSD_EINVAL, "preconditioner input and output must not overlap");
+        const int32_t w = int32_t(nb);
+        const int64_t n = m->n;
+        if (m->ic) {
+            ic_apply(m, dtype, R, Z, w, nullptr);
+        } else if (m->nblocks == 0) {
+            const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((n * w + 255) / 256, 8192)));
+            if (dtype == KMSD_C128)
+                hipLaunchKernelGGL(diag_precond_kernel<double>, dim3(g), dim3(256), 0, c->stream, R, Z, n, int64_t(w),
+                                   m->diag);
+            else
+                hipLaunchKernelGGL(diag_precond_kernel<float>, dim3(g), dim3(256), 0, c->stream, R, Z, n, int64_t(w),
+                                   m->diag);
+        } else {
+            if (m->nlarge < m->nblocks) {
+                const int64_t total = m->nblocks * w;
+                const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 16384)));
+                if (dtype == KMSD_C128)
+                    hipLaunchKernelGGL((block_precond_kernel<double, SMALL_BLOCK>), dim3(g), dim3(256), 0, c->stream,
+                                       R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks,
+                                       static_cast<const PcgFlags *>(nullptr), m->block_rows);
+                else
+                    hipLaunchKernelGGL((block_precond_kernel<float, SMALL_BLOCK>), dim3(g), dim3(256), 0, c->stream,
+                                       R, Z, int64_t(w), m->block_ptr, m->factor_off, m->factors, m->nblocks,
+                                       static_cast<const PcgFlags *>(nullptr), m->block_rows);
+            }
+            large_blocks_apply(m, dtype, R, Z, w, nullptr);
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
 extern "C" int kmsd_metric_set_index_blocks(kmsd_metric *m, int64_t nblocks, const int64_t *block_off,
                                             const int64_t *rows, const double *factors) {
     if (!m) return KMSD_EINVAL;

@@ -25,6 +25,26 @@ def scaled_action(ops, cfg, B):
     return (ops.apply_h_action(B) - c * B) / a
 
 
+def evaluate_series(B, action, coeff, xp=None):
+    """sum_k coeff[k] T_k(A) B for a caller-supplied action A (reference
+    chebyshev.py:32-39), by the three-term recursion on device tensors; the
+    solvers here use the fused kmsd_series instead.  xp is accepted for the
+    reference's signature and unused."""
+    import torch
+    prev = B
+    out = coeff[0] * B
+    if len(coeff) > 1:
+        curr = action(B)
+        out = out + coeff[1] * curr
+        for v in coeff[2:]:
+            nxt = 2 * action(curr) - prev
+            out = out + v * nxt
+            prev, curr = curr, nxt
+    if not bool(torch.isfinite(torch.view_as_real(out) if out.is_complex() else out).all()):
+        raise NumericalError("nonfinite Chebyshev propagation; check bounds and solves")
+    return out
+
+
 def bessel_coefficients(dt, cfg, weighted_tail=False):
     """c_k = (-i)^k J_k(z) (2 for k >= 1) exp(-i c dt/hbar), z = a dt/hbar, with
     the order chosen by the closed-form Bessel tail bound (chebyshev.py:17-30)."""

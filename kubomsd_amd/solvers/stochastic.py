@@ -19,7 +19,7 @@ import numpy as np
 
 from ..contracts import Reason, TransportBlock
 from ..errors import NumericalError, ResourceBlocked
-from ..kernels.energy import dos_floor
+from ..kernels.energy import dos_floor, kernel_resolution
 from ..metric.device_bounds import estimate_spectral_bounds_device
 from ..metric.sampling import sample_metric
 from ..observables.statistics_device import FLAG_DOS, FLAG_NUMERATOR, DeviceStatistics
@@ -126,12 +126,20 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
     metadata = {
         "primary_estimator": primary,
         "primary_diffusion_estimator": "msd_gradient" if method == "stepwise" else "vac_integral",
-        "resolved_config": cfg.as_dict(), "spectral_report": report, "dos_floor": threshold,
+        "resolved_config": cfg.as_dict(), "spectral_report": report,
+        "kernel_resolution": kernel_resolution(cfg), "dos_floor": threshold,
         "statistical_method": stat,
         "sample_generator": "NumPy PCG64 SeedSequence([seed,sample_id]); unit modulus complex phases",
         "sampling": getattr(getattr(ops, "sampler", None), "report", {"source": "caller supplied samples"}),
         "sample_batch_diagnostics": diagnostics, "admitted_sample_batch": batch,
         "raw_accumulation": "numerator_* are means over nsamples; *_block_sums are sums",
+        "scratch_bytes": 0,
+        "parallel": {"distribution": "serial", "world_size": 1, "operator_storage": "one device",
+                     "reduction": "device-resident block sums, normalized once"},
+        "science_status": {
+            "NUMERICS_VERIFIED": "UNVERIFIED for this run",
+            "STATISTICS_VERIFIED": "NOT RUN (no multi-seed convergence study for this run)",
+            "TRUNCATION_VERIFIED": "not applicable"},
         "statistics_location": "device-resident block sums and bootstrap",
         "backend": {"name": "hip", "precision": cfg.precision, "memory": ops.ctx.memory(),
                     "metric": ops.metric.diagnostics(), "h_action_calls": ops.h_calls},

@@ -303,3 +303,52 @@ def test_ic0_graph_replays_equal_eager_launches(hip_ctx, dtype):
         np.testing.assert_array_equal(a, b)
         assert ia == ib
     np.testing.assert_array_equal(out[1][0][0], out[1][2][0])
+
+
+def test_metric_solver_class_as_the_reference_uses_it(hip_ctx):
+    """kubomsd_amd.metric.solve.MetricSolver with the reference's call
+    patterns (tests/unit/test_metric_ic0.py:9-29, test_metric_cholesky.py:15-37,
+    tests/gpu/test_metric_preconditioners.py:53-84)."""
+    from types import SimpleNamespace
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.errors import NumericalError
+    from kubomsd_amd.metric.solve import MetricSolver
+    base = dict(solver="stochastic-kpm", kernel="jackson", backend="hip")
+    for ordering in ("natural", "rcm"):
+        m = MetricSolver(S4, RunConfig([0], [0], **base, preconditioner="ic0", metric_ic_ordering=ordering,
+                                       metric_rtol=1e-13), hip_ctx)
+        B = np.array([[1j, 0], [2, 0], [3j, 0], [1, 0]])
+        Y, r = m.solve_s(B)
+        np.testing.assert_allclose(host(Y), np.linalg.solve(S4.toarray(), B), atol=1e-12)
+        assert np.all(r.converged)
+    S = np.array([[2, .3j], [-.3j, .7]])
+    m = MetricSolver(S, RunConfig([0], [0], **base, metric_solver="cholesky", metric_rtol=1e-12), hip_ctx)
+    for _ in range(3):
+        B = np.array([[1 + 1j, 0, 2e-180j, 1e180], [2j, 0, 1e-180, 2e180j]])
+        Y, r = m.solve_s(B)
+        s = B.max(axis=0, initial=1)
+        np.testing.assert_allclose(host(Y) / s, np.linalg.solve(S, B) / s, rtol=1e-12, atol=1e-12)
+        assert np.all(r.iterations == 0) and np.all(r.relative_residual <= 1e-12)
+    d = m.diagnostics()
+    assert d["factorizations"] == 1 and d["solver"] == "cholesky" and d["calls"] == 3
+    with pytest.raises(NumericalError, match="cannot change prepared"):
+        m.solve_s(B, SimpleNamespace(preconditioner="ic0"))
+    # grouped block-Jacobi application (sizes 2, 3 and 5), and its negation
+    rng = np.random.default_rng(318)
+    sizes = [2, 3, 2, 3, 2, 3, 2, 3, 5]
+    n = sum(sizes)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    S = A @ A.conj().T + np.eye(n)
+    edges = np.r_[0, np.cumsum(sizes)]
+    slices = list(zip(edges[:-1], edges[1:]))
+    m = MetricSolver(S, RunConfig([0], [0], **base, preconditioner="block-jacobi", metric_rtol=1e-13), hip_ctx,
+                     atom_slices=slices)
+    B = rng.normal(size=(n, 8)) + 1j * rng.normal(size=(n, 8))
+    want = np.empty_like(B[:, ::2])
+    for a, b in slices:
+        want[a:b] = la.solve(S[a:b, a:b], B[a:b, ::2], assume_a="pos")
+    dev = to_dev(hip_ctx, B)[:, ::2]
+    np.testing.assert_allclose(host(m.precondition(dev)), want, rtol=3e-13, atol=2e-14)
+    np.testing.assert_allclose(host(m.precondition(-dev)), -want, rtol=3e-13, atol=2e-14)
+    Y, r = m.solve_s(dev)
+    np.testing.assert_allclose(host(Y), la.solve(S, B[:, ::2]), rtol=1e-11, atol=1e-12)
