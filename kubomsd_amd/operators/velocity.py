@@ -99,6 +99,8 @@ class DeviceMetric:
             from ..metric.preconditioners import prepare_cluster
             if not metric_identity or not all(metric_identity.values()):
                 raise NumericalError("cluster-Jacobi requires explicit model identity")
+            if not atom_slices:
+                raise NumericalError("cluster-Jacobi requires the model's atom slices")
             partition = load_metric_partition(config.metric_partition_file, s_fingerprint=matrix_fingerprint(S),
                                               atom_slices=atom_slices, n_orbitals=n, **metric_identity)
             self.preconditioner_data = prepare_cluster(S, partition)
