@@ -52,6 +52,22 @@ def batch_workspace_bytes(ops, cfg, method, nb):
     return int(vec)
 
 
+def check_workspace_budget(cfg, keys, nbk):
+    """The reference's host workspace budget (solvers/stochastic.py:139-157):
+    a batch's per-sample arrays, the merged slabs and, with two or more
+    blocks, the bootstrap's resampled numerators, ratios and scratch must fit
+    memory_limit_gb; the same estimate gives the same rejection here."""
+    na, nt, ne = len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV)
+    tb = min(cfg.time_block, nt)
+    working = min(cfg.sample_batch, cfg.statistics_block) * na * nt * ne * 8 * len(keys) * 2
+    root = nbk * na * tb * ne * 8 * (len(keys) + 6) + nbk * ne * 16
+    if nbk > 1:
+        root += 4 * 256 * na * tb * ne * 8 + 2 * 256 * nbk * 8 + 2 * 256 * ne * 8
+    if max(working, root) > cfg.memory_limit_gb * 1024 ** 3:
+        raise NumericalError("sample/merge/bootstrap workspace exceeds memory budget; reduce "
+                             "sample_batch/time_block/energy count or raise explicit budget")
+
+
 def admitted_batch(ops, cfg, method):
     mem = ops.ctx.memory()
     free = mem["budget_bytes"] - mem["used_bytes"]
@@ -170,6 +186,7 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     nbk = (ns + cfg.statistics_block - 1) // cfg.statistics_block
     counts = np.array([min(cfg.statistics_block, ns - i * cfg.statistics_block) for i in range(nbk)])
     keys = _quantity_keys(cfg, method)
+    check_workspace_budget(cfg, keys, nbk)
     shape = (nbk, len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV))
     import torch
     dev = ops.ctx.torch_device

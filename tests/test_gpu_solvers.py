@@ -133,3 +133,18 @@ def test_sampler_covariance_on_device(hip_ctx):
     sampler = MetricSampler(ops.metric, cfg)
     C = sampler.transform(ops.asarray(np.eye(2, dtype=complex))).cpu().numpy()
     np.testing.assert_allclose(C @ C.conj().T, np.linalg.inv(S), rtol=1e-11, atol=1e-12)
+
+
+def test_bootstrap_workspace_is_included_in_budget(hip_ctx):
+    """tests/unit/test_stochastic_budget.py:9-19: 1000 energies, two blocks and
+    a 5 MB budget are rejected before any sampling, naming the bootstrap."""
+    from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
+    cfg = run_config(energy_eV=np.linspace(-1, 1, 1000), time_fs=[0., .01], axes=("x",), nsamples=2,
+                     statistics_block=1, sample_batch=1, n_moments=4, spectral_bounds=(-2., 2.),
+                     memory_limit_gb=.005)
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.operators.velocity import build_operators
+    ops = build_operators(ModelBundle(np.array([[0., .5], [.5, 0.]]), np.eye(2), position={"x": np.diag([0., 1.])}),
+                          cfg, hip_ctx)
+    with pytest.raises(ValueError, match="bootstrap"):
+        list(iter_stochastic_blocks(ops, cfg, "stepwise"))
