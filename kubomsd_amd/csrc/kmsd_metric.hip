@@ -131,6 +131,7 @@ void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y
         m->rep_res.assign(nb, 0.0);
         m->rep_rel.assign(nb, 0.0);
         ++m->calls;
+        ++m->residual_checks;  // the reference's final true-residual acceptance
         return;
     }
     pcg_solve(m, dtype, B, ldb, Y, ldy, nb, rtol, atol, maxiter);

@@ -352,3 +352,57 @@ def test_metric_solver_class_as_the_reference_uses_it(hip_ctx):
     np.testing.assert_allclose(host(m.precondition(-dev)), -want, rtol=3e-13, atol=2e-14)
     Y, r = m.solve_s(dev)
     np.testing.assert_allclose(host(Y), la.solve(S, B[:, ::2]), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", ["pcg", "cholesky"])
+def test_final_true_residual_acceptance_is_counted(hip_ctx, solver):
+    """tests/unit/test_metric_adaptive.py:90-97: one final check, no refresh."""
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.metric.solve import MetricSolver
+    m = MetricSolver(np.eye(2), RunConfig([0], [0], solver="stochastic-kpm", kernel="jackson", backend="hip",
+                                          metric_solver=solver, metric_refine_maxiter=0), hip_ctx)
+    m.solve_s(np.ones((2, 2)))
+    d = m.diagnostics()
+    assert d["true_residual_checks"] == 1 and d["true_residual_refreshes"] == 0
+
+
+def test_adaptive_extremes_override_and_false_convergence(hip_ctx):
+    """tests/unit/test_metric_adaptive.py:45-86: adaptive policy with column
+    scales 0 / 1 / 1e-180 / 1e180 and a per-solve rtol override; and on an
+    S of condition 1e4 with no periodic refresh, candidate convergence is
+    checked against the true residual."""
+    from types import SimpleNamespace
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.metric.solve import MetricSolver
+    base = dict(solver="stochastic-kpm", kernel="jackson", backend="hip", metric_residual_policy="adaptive")
+    rng = np.random.default_rng(4)
+    U, _ = np.linalg.qr(rng.normal(size=(48, 48)) + 1j * rng.normal(size=(48, 48)))
+    S = (U * np.geomspace(.01, 2, 48)) @ U.conj().T
+    S = (S + S.conj().T) / 2
+    B = (rng.normal(size=(48, 8)) + 1j * rng.normal(size=(48, 8)))[:, ::2] * [0, 1, 1e-180, 1e180]
+    c = RunConfig([0], [0], **base, metric_check_interval=4, metric_rtol=1e-12, metric_maxiter=400)
+    m = MetricSolver(S, c, hip_ctx)
+    y, r = m.solve_s(B, SimpleNamespace(rtol=2e-13))
+    y = host(y)
+    assert np.all(r.relative_residual <= 2e-13)
+    np.testing.assert_array_equal(y[:, 0], 0)
+    for j in (1, 2, 3):
+        s = np.max(abs(B[:, j]))
+        np.testing.assert_allclose(y[:, j] / s, np.linalg.solve(S, B[:, j] / s), rtol=3e-11, atol=3e-11)
+    assert m.diagnostics()["true_residual_checks"] > 0
+    rng = np.random.default_rng(8)
+    U, _ = np.linalg.qr(rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32)))
+    S = (U * np.geomspace(1e-4, 1, 32)) @ U.conj().T
+    S = (S + S.conj().T) / 2
+    B = rng.normal(size=(32, 3)) + 1j * rng.normal(size=(32, 3))
+    B[:, 0] = 0
+    m = MetricSolver(S, RunConfig([0], [0], **base, metric_check_interval=10000, metric_maxiter=1000,
+                                  metric_rtol=1e-12), hip_ctx)
+    Y, r = m.solve_s(B)
+    Y = host(Y)
+    d = m.diagnostics()
+    assert d["true_residual_checks"] > 1 and d["true_residual_refreshes"] > 0  # candidate checks, no periodic one
+    assert np.all(r.converged) and r.iterations[0] == 0
+    np.testing.assert_array_equal(Y[:, 0], 0)
+    for j in (1, 2):
+        assert np.linalg.norm(B[:, j] - S @ Y[:, j]) / np.linalg.norm(B[:, j]) <= 1e-12
