@@ -229,6 +229,9 @@ class OperatorBundle:
                 v = self.apply_v_cov(axis, probe, controls)
             except NumericalError as exc:
                 attempts.append(dict(metric_rtol=rtol, metric_atol=atol, status="S_SOLVE_FAILED", reason=str(exc)))
+                # keep the failed solve's per-column report next to the probe's attempts
+                exc.diagnostics = {**(getattr(exc, "diagnostics", None) or {}),
+                                   "velocity_probe": dict(axis=axis, threshold=1e-10, attempts=attempts)}
                 raise
             V = _gram(probe, v).cpu().numpy()
             error = float(np.linalg.norm(V - V.conj().T) / max(np.linalg.norm(V), 1e-300))
@@ -255,19 +258,28 @@ class OperatorBundle:
     def v_calls(self):
         return self.native.v_calls
 
+    def _dev(self, B):
+        """Device block: torch tensors keep their complex dtype, host arrays
+        (the reference's NumPy calls) become the precision's storage type."""
+        import torch
+        if isinstance(B, torch.Tensor) and B.is_complex():
+            return self.asarray(B, B.dtype)
+        return self.asarray(B)
+
     def solve_s(self, B, config=None):
-        return self.metric.solve_s(self.asarray(B, B.dtype if hasattr(B, "dtype") else None), config)
+        return self.metric.solve_s(self._dev(B), config)
 
     def apply_h_cov(self, B):
-        return self.H.matmul(B)
+        return self.H.matmul(self._dev(B))
 
     def apply_h_action(self, B):
-        return self.native.h_action(B)
+        return self.native.h_action(self._dev(B))
 
     def apply_v_cov(self, axis, B, config=None):
         direction = self.directions.get(axis, np.eye(3)["xyz".index(axis)] if axis in "xyz" else None)
         if direction is None:
             raise KuboMSDError("unknown direction")
+        B = self._dev(B)
         if config is not None:
             return self.native.v_cov_with_controls(B, direction, config.metric_rtol, config.metric_atol,
                                                    config.metric_maxiter)
@@ -286,7 +298,7 @@ class OperatorBundle:
 
     def inner(self, A, B):
         """Per-column <A, S B> on the device (complex128, kmsd_metric_inner)."""
-        return self.metric.device.inner(A, B)
+        return self.metric.device.inner(self._dev(A), self._dev(B))
 
 
 def build_operators(model, config, ctx=None):
