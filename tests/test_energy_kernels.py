@@ -53,3 +53,52 @@ def test_jackson_fwhm_equals_sampled_half_maximum_width():
 def test_energy_weights_reject_levels_outside_bounds():
     with pytest.raises(KuboMSDError, match="spectral bounds"):
         energy_weights([0.0], [9.0], cfg())
+
+
+def test_output_window_keeps_full_spectral_scale():
+    """tests/unit/test_jackson_spectral_scale.py: a narrow output window does
+    not redefine the spectral scale; independent cosine-series evaluation."""
+    energy = np.linspace(-0.5, 0.5, 11)
+    levels = np.array([-8., -2., 0., 2., 15.])
+    c = RunConfig(energy, [0., 0.1], kernel="jackson", n_moments=64, spectral_bounds=(-10., 20.))
+    centre, radius = 5., 15.
+    m = np.arange(64)
+    ang = np.pi / 65
+    damp = ((65 - m) * np.cos(m * ang) + np.sin(m * ang) / np.tan(ang)) / 65
+    x = (energy - centre) / radius
+    y = (levels - centre) / radius
+    want = (np.cos(np.arccos(x)[:, None] * m) * damp * np.where(m == 0, 1, 2)) @ np.cos(m[:, None] * np.arccos(y)[None, :])
+    want /= (np.pi * radius * np.sqrt(1 - x * x))[:, None]
+    got = energy_weights(energy, levels, c)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-13)
+    assert not np.allclose(got[:, 0], got[:, 1])
+
+
+@pytest.mark.parametrize("L", [1, 2, 64, 1024])
+def test_jackson_area_and_term_count(L):
+    """tests/unit/test_energy.py:18-28."""
+    from kubomsd_amd.kernels.energy import jackson_coefficients
+    g = jackson_coefficients(L)
+    assert len(g) == L and g[0] == 1
+    theta = (np.arange(4096) + .5) * np.pi / 4096
+    E = np.cos(theta)[::-1]
+    c = RunConfig(E, [0], kernel="jackson", spectral_bounds=(-1, 1), n_moments=L)
+    w = energy_weights(E, [-.37, .2], c)
+    np.testing.assert_allclose((w * np.sqrt(1 - E[:, None] ** 2)).sum(0) * np.pi / 4096, 1, atol=2e-12)
+
+
+def test_full_spectrum_is_independent_of_local_output_and_no_clipping():
+    """tests/unit/test_energy.py:5-15, 38-44."""
+    from kubomsd_amd.kernels.energy import validate_spectral_window
+    eps = np.array([-9., 0., 19.])
+    assert validate_spectral_window(eps, (-10., 20.)) == (5., 15.)
+    c = RunConfig([-.5, 0, .5], [0, 1], kernel="jackson", spectral_bounds=(-10, 20), n_moments=64)
+    a = energy_weights(c.energy_eV, eps, c)
+    b = energy_weights([-5., 0., 10.], eps, c)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-13)
+    assert not np.allclose(a[:, 0], a[:, 2])
+    with pytest.raises(ValueError):
+        validate_spectral_window([-2, 0], (-1, 1))
+    c = RunConfig([0, 1], [0], kernel="jackson", spectral_bounds=(-1, 1))
+    with pytest.raises(ValueError):
+        energy_weights([0, 1], [0], c)

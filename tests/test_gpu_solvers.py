@@ -148,3 +148,29 @@ def test_bootstrap_workspace_is_included_in_budget(hip_ctx):
                           cfg, hip_ctx)
     with pytest.raises(ValueError, match="bootstrap"):
         list(iter_stochastic_blocks(ops, cfg, "stepwise"))
+
+
+def test_propagation_against_expm_inverse_segments_and_norm(hip_ctx):
+    """tests/unit/test_propagation.py:5-21: propagate = expm(-i S^-1 H t / hbar),
+    backward propagation inverts it, segments compose, the S-norm is kept."""
+    import scipy.linalg as la
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.stepwise import propagate
+    from kubomsd_amd.units import HBAR_EV_FS
+    from helpers import host
+    A = np.array([[1, .1j], [.2, 1.2]])
+    S = A.conj().T @ A
+    H = A.conj().T @ np.array([[.3, .4j], [-.4j, -.2]]) @ A
+    V = A.conj().T @ np.array([[.5, 1j], [-1j, 0.]]) @ A
+    cfg = run_config(energy_eV=[-.1, .1], time_fs=[0, .2, .7], axes=("x",), spectral_bounds=(-1, 1), n_moments=24,
+                     metric_rtol=1e-12, integration_dt_fs=.002, segment_fs=.3)
+    ops = build_operators(ModelBundle(H, S, velocity={"x": V}), cfg, hip_ctx)
+    B = np.array([[1 + 1j, .2], [.4j, 1]])
+    out = host(propagate(B, .7, ops, cfg)[0])
+    want = la.expm(-1j * la.solve(S, H) * .7 / HBAR_EV_FS) @ B
+    np.testing.assert_allclose(out, want, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(host(propagate(out, -.7, ops, cfg)[0]), B, rtol=1e-10, atol=1e-12)
+    mid = propagate(B, .3, ops, cfg)[0]
+    np.testing.assert_allclose(host(propagate(mid, .4, ops, cfg)[0]), out, atol=1e-12)
+    np.testing.assert_allclose(np.sum(B.conj() * (S @ B), axis=0), np.sum(out.conj() * (S @ out), axis=0), atol=1e-12)
