@@ -73,7 +73,8 @@ int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
  * automatic, the default: each fused step launches one workgroup per
  * resident slot of its kernel on the device), "cpl" (complex columns per
  * lane: 1, 2 or 4; 0 = automatic: 2 for complex64 and 1 for complex128
- * blocks, 4 and 2 for products with 4x4-block matrices),
+ * blocks, 4 and 2 for products with 4x4-block matrices; "cpl" sets both
+ * precisions, "cpl_c64" and "cpl_c128" one of them),
  * "moment_chunk" (Chebyshev orders per partial-reduction launch),
  * "fp32_math" (complex64 blocks: FP32 values and row sums), "pcg_poll" (PCG
  * iterations between host polls), "graphs" (1: moment chains with an
@@ -89,8 +90,8 @@ int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
  * nb-column block of dtype for every columns-per-lane (1, 2, 4) and
  * workgroup count (automatic, 1024, 2048, 4096), keeps the default shape
  * (automatic workgroups) unless another is more than 3 % faster, applies
- * the choice to ctx ("cpl", "groups") and writes it to choice[2] (groups 0 =
- * automatic).  dtype may carry KMSD_TUNE_F32_VALUES (tune for the metric
+ * the choice to ctx ("cpl_c64" or "cpl_c128" for the tuned precision, and
+ * "groups") and writes it to choice[2] (groups 0 = automatic).  dtype may carry KMSD_TUNE_F32_VALUES (tune for the metric
  * products of complex64 solves).  Uses three scratch blocks of n*nb elements
  * from the budget. */
 int kmsd_autotune(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice);

@@ -240,7 +240,7 @@ class HipContext:
             except OSError:
                 pass  # an unwritable cache only costs a re-tune next run
         _TUNED[key] = choice
-        self.set_tuning("cpl", choice["cpl"])
+        self.set_tuning("cpl_c64" if dtype == torch.complex64 else "cpl_c128", choice["cpl"])
         self.set_tuning("groups", choice["groups"])
         return dict(choice)
 

@@ -90,10 +90,11 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
                 throw Error(KMSD_EINVAL, "groups must be 0 (automatic) or a positive multiple of 8");
             c->auto_groups = value == 0;
             c->groups = value ? value : int64_t(std::max(8, c->cus * 8));
-        } else if (k == "cpl") {
+        } else if (k == "cpl" || k == "cpl_c64" || k == "cpl_c128") {
             if (value != 0 && value != 1 && value != 2 && value != 4)
                 throw Error(KMSD_EINVAL, "cpl must be 0 (automatic), 1, 2 or 4");
-            c->cpl = int(value);
+            if (k != "cpl_c128") c->cpl[0] = int(value);
+            if (k != "cpl_c64") c->cpl[1] = int(value);
         } else if (k == "moment_chunk") {
             if (value < 1 || value > 4096) throw Error(KMSD_EINVAL, "moment_chunk must be in [1, 4096]");
             c->moment_chunk = int(value);
@@ -127,7 +128,8 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
         const int64_t max_groups = std::max<int64_t>(4096, int64_t(c->cus) * 8);
         const size_t pbytes = size_t(max_groups) * size_t(nb) * 2 * sizeof(double);
         void *X = nullptr, *P = nullptr, *W = nullptr, *part = nullptr;
-        const int saved_cpl = c->cpl;
+        const int slot = dtype == KMSD_C64 ? 0 : 1;  // the tuned precision's columns-per-lane setting
+        const int saved_cpl = c->cpl[slot];
         const int64_t saved_groups = c->groups;
         const bool saved_auto = c->auto_groups;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -161,7 +163,7 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
                 if (dtype == KMSD_C128 && cp == 4) continue;
                 if (nb % cp) continue;
                 for (int64_t g : groups) {
-                    c->cpl = cp;
+                    c->cpl[slot] = cp;
                     c->auto_groups = g == 0;
                     c->groups = g ? g : int64_t(std::max(8, c->cus * 8));
                     Step s;
@@ -213,13 +215,13 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
                 best_cpl = def_cpl;
                 best_groups = 0;
             }
-            c->cpl = best_cpl;
+            c->cpl[slot] = best_cpl;
             c->auto_groups = best_groups == 0;
             c->groups = best_groups ? best_groups : int64_t(std::max(8, c->cus * 8));
             choice[0] = best_cpl;
             choice[1] = best_groups;
         } catch (...) {
-            c->cpl = saved_cpl;
+            c->cpl[slot] = saved_cpl;
             c->groups = saved_groups;
             c->auto_groups = saved_auto;
             release();

@@ -17,7 +17,7 @@ struct kmsd_ctx {
     std::string err;
     int64_t groups = 2048;  // workgroups per row-walking launch (multiple of 8)
     bool auto_groups = true; // fused steps: one workgroup per resident slot of the kernel (else `groups`)
-    int cpl = 0;            // 0 = automatic
+    int cpl[2] = {0, 0};    // columns per lane of complex64 [0] and complex128 [1] blocks; 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
     int pcg_poll = 4;       // PCG iterations between host status polls

@@ -324,7 +324,8 @@ int resident_per_cu(KernelFn fn) {
 Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const kmsd_matrix *A, bool f32,
                 KernelFn *fn_out = nullptr) {
     Shape s;
-    const int want = c->cpl ? c->cpl : default_cpl(dtype, A);
+    const int tuned = c->cpl[dtype == KMSD_C64 ? 0 : 1];
+    const int want = tuned ? tuned : default_cpl(dtype, A);
     s.cpl = std::min(want, std::min(cpl_max, dtype == KMSD_C64 ? 4 : 2));
     while (s.cpl > 1 && w % s.cpl) s.cpl >>= 1;
     const int64_t lanes = (w + s.cpl - 1) / s.cpl;
