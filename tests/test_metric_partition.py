@@ -96,3 +96,22 @@ def test_full_cholesky_is_cpu_exact_only():
     from kubomsd_amd.config import RunConfig
     with pytest.raises(KuboMSDError, match="CPU EXACT"):
         RunConfig([0], [0], solver="stochastic-kpm", kernel="jackson", preconditioner="full-cholesky")
+
+
+def test_prepare_cluster_factors_each_block():
+    """metric/preconditioners.py:37-46 of the reference: a lower Cholesky
+    factor of S[rows][:, rows] per partition block, rows as given."""
+    import scipy.linalg as la
+    from kubomsd_amd.metric.preconditioners import prepare_cluster
+    d = partition()
+    data = prepare_cluster(S3, d)
+    assert data.kind == "cluster-jacobi" and data.report["block_sizes"] == [2, 1]
+    for (rows, L), block in zip(data.blocks, d["blocks"]):
+        np.testing.assert_array_equal(rows, block["orbitals"])
+        sub = S3.toarray()[np.ix_(rows, rows)]
+        np.testing.assert_allclose(L @ L.conj().T, sub, rtol=1e-14, atol=1e-15)
+        np.testing.assert_array_equal(L, np.tril(L))
+    bad = dict(d, blocks=[{"atoms": [0, 1, 2], "orbitals": [0, 1, 2]}])
+    indefinite = sp.csr_matrix(np.array([[1, 2, 0], [2, 1, 0], [0, 0, 1.]]))
+    with pytest.raises(NumericalError, match="positive definite"):
+        prepare_cluster(indefinite, bad)
