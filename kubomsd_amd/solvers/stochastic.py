@@ -127,7 +127,7 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     return diagnostics
 
 
-def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics, batch, stats):
+def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics, batch, stats, resolution):
     nbk = len(counts)
     primary = "commutator" if method == "stepwise" else "vac_integral"
     import torch
@@ -143,7 +143,7 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
         "primary_estimator": primary,
         "primary_diffusion_estimator": "msd_gradient" if method == "stepwise" else "vac_integral",
         "resolved_config": cfg.as_dict(), "spectral_report": report,
-        "kernel_resolution": kernel_resolution(cfg), "dos_floor": threshold,
+        "kernel_resolution": resolution, "dos_floor": threshold,
         "statistical_method": stat,
         "sample_generator": "NumPy PCG64 SeedSequence([seed,sample_id]); unit modulus complex phases",
         "sampling": getattr(getattr(ops, "sampler", None), "report", {"source": "caller supplied samples"}),
@@ -194,11 +194,17 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     dos_blocks = torch.zeros((nbk, len(cfg.energy_eV)), dtype=torch.float64, device=dev)
     stats = DeviceStatistics(ops.ctx)
     batch = admitted_batch(ops, cfg, method)
-    t0 = time.perf_counter()
-    diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats)
-    seconds = time.perf_counter() - t0
-    valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
-                                                     diagnostics, batch, stats)
+    # The reported kernel resolution is host root-finding only; it runs on a
+    # helper thread while the samples' device work (inside library calls
+    # that release the GIL) proceeds.
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        resolution = pool.submit(kernel_resolution, cfg)
+        t0 = time.perf_counter()
+        diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats)
+        seconds = time.perf_counter() - t0
+        valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
+                                                         diagnostics, batch, stats, resolution.result())
     metadata["sample_seconds"] = seconds
     valid_host = _host(valid)
     for start in range(0, len(cfg.time_fs), cfg.time_block):
