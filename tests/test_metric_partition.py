@@ -115,3 +115,27 @@ def test_prepare_cluster_factors_each_block():
     indefinite = sp.csr_matrix(np.array([[1, 2, 0], [2, 1, 0], [0, 0, 1.]]))
     with pytest.raises(NumericalError, match="positive definite"):
         prepare_cluster(indefinite, bad)
+
+
+def test_atom_block_cholesky_matches_per_block_factorization():
+    """The batched set-up equals scipy.linalg.cholesky of every diagonal
+    block (metric/setup.py:73-76 of the reference), for mixed block sizes."""
+    import scipy.linalg as la
+    from kubomsd_amd.operators.velocity import atom_block_cholesky
+    rng = np.random.default_rng(1)
+    n = 40
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    S = A @ A.conj().T + n * np.eye(n)
+    S[np.abs(S) < 3] = 0
+    S = (S + S.conj().T) / 2
+    slices = [(0, 1), (1, 4), (4, 8), (8, 9), (9, 13), (13, 40)]
+    got = atom_block_cholesky(sp.csr_matrix(S), slices)
+    assert [(a, b) for a, b, _ in got] == slices
+    for a, b, L in got:
+        np.testing.assert_allclose(L, la.cholesky(S[a:b, a:b], lower=True), rtol=1e-13, atol=1e-13)
+    bad = S.copy()
+    bad[2, 2] = -50
+    with pytest.raises(NumericalError, match="not positive definite"):
+        atom_block_cholesky(sp.csr_matrix(bad), slices)
+    with pytest.raises(NumericalError, match="partition"):
+        atom_block_cholesky(sp.csr_matrix(S), [(0, 4), (5, 40)])
