@@ -14,7 +14,6 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
-import scipy.linalg as la
 import scipy.sparse as sp
 
 from ..errors import KuboMSDError, NumericalError
@@ -32,16 +31,45 @@ class PreconditionerData:
     report: dict = field(default_factory=dict)
 
 
+def block_cholesky(S, index_blocks, what="block"):
+    """Lower Cholesky factors of S[rows][:, rows] for disjoint row lists
+    covering S, in the given row order: S's entries are gathered into dense
+    blocks at once and factored in one batched LAPACK call per block size."""
+    S = sp.coo_matrix(S)
+    n = S.shape[0]
+    sizes = np.array([len(r) for r in index_blocks], np.int64)
+    rows_all = np.concatenate([np.asarray(r, np.int64) for r in index_blocks]) if len(index_blocks) else \
+        np.zeros(0, np.int64)
+    if sizes.sum() != n or not np.array_equal(np.sort(rows_all), np.arange(n)):
+        raise NumericalError(f"{what}s must partition the orbitals")
+    owner = np.empty(n, np.int64)
+    local = np.empty(n, np.int64)
+    owner[rows_all] = np.repeat(np.arange(len(index_blocks)), sizes)
+    local[rows_all] = np.arange(n) - np.repeat(np.cumsum(sizes) - sizes, sizes)
+    inside = owner[S.row] == owner[S.col]
+    r, c, v = S.row[inside], S.col[inside], S.data[inside]
+    factors = [None] * len(index_blocks)
+    for m in np.unique(sizes):
+        qs = np.flatnonzero(sizes == m)
+        slot = np.full(len(index_blocks), -1, np.int64)
+        slot[qs] = np.arange(len(qs))
+        sel = sizes[owner[r]] == m
+        blk = np.zeros((len(qs), m, m), np.complex128)
+        np.add.at(blk, (slot[owner[r[sel]]], local[r[sel]], local[c[sel]]), v[sel])
+        try:
+            L = np.linalg.cholesky(blk)
+        except np.linalg.LinAlgError as exc:
+            raise NumericalError(f"{what} of the overlap is not positive definite") from exc
+        for k, q in enumerate(qs):
+            factors[q] = L[k]
+    return factors
+
+
 def prepare_cluster(S, partition):
     """Factors S[rows][:, rows] for every partition block."""
-    S = sp.csr_matrix(S)
+    index_blocks = [np.asarray(block["orbitals"], dtype=np.int64) for block in partition["blocks"]]
     blocks = []
-    for block in partition["blocks"]:
-        rows = np.asarray(block["orbitals"], dtype=np.int64)
-        try:
-            L = la.cholesky(S[rows][:, rows].toarray(), lower=True, check_finite=False)
-        except la.LinAlgError as exc:
-            raise NumericalError("cluster block of the overlap is not positive definite") from exc
+    for rows, L in zip(index_blocks, block_cholesky(S, index_blocks, "cluster block")):
         rows.setflags(write=False)
         L.setflags(write=False)
         blocks.append((rows, L))

@@ -40,35 +40,14 @@ def hermitian(matrix, name):
 
 def atom_block_cholesky(S, slices):
     """[(a, b, L)] with L the lower Cholesky factor of S[a:b, a:b] for each
-    contiguous slice (reference metric/setup.py:73-76), gathered from S's
-    entries at once and factored in one batched LAPACK call per block size
-    instead of one sparse slice and factorization per atom."""
-    S = sp.coo_matrix(S)
-    n = S.shape[0]
-    sizes = np.array([b - a for a, b in slices], np.int64)
-    firsts = np.array([a for a, _ in slices], np.int64)
-    if sizes.sum() != n or np.any(firsts != np.concatenate([[0], np.cumsum(sizes)[:-1]])):
+    contiguous slice (reference metric/setup.py:73-76), factored in one
+    batched call per block size instead of one per atom."""
+    from ..metric.preconditioners import block_cholesky
+    bounds = [(int(a), int(b)) for a, b in slices]
+    if [i for a, b in bounds for i in range(a, b)] != list(range(S.shape[0])):
         raise NumericalError("atom_slices must partition orbitals")
-    start = np.repeat(firsts, sizes)  # first row of each row's block
-    owner = np.repeat(np.arange(len(slices)), sizes)
-    inside = owner[S.row] == owner[S.col]
-    rows, cols, vals = S.row[inside], S.col[inside], S.data[inside]
-    out = [None] * len(slices)
-    for m in np.unique(sizes):
-        qs = np.flatnonzero(sizes == m)
-        slot = np.full(len(slices), -1, np.int64)
-        slot[qs] = np.arange(len(qs))
-        sel = sizes[owner[rows]] == m
-        blk = np.zeros((len(qs), m, m), np.complex128)
-        r, c = rows[sel], cols[sel]
-        np.add.at(blk, (slot[owner[r]], r - start[r], c - start[c]), vals[sel])
-        try:
-            L = np.linalg.cholesky(blk)
-        except np.linalg.LinAlgError as exc:
-            raise NumericalError("atom block of the overlap is not positive definite") from exc
-        for k, q in enumerate(qs):
-            out[q] = (int(firsts[q]), int(firsts[q] + m), L[k])
-    return out
+    factors = block_cholesky(S, [np.arange(a, b) for a, b in bounds], "atom block")
+    return [(a, b, L) for (a, b), L in zip(bounds, factors)]
 
 
 def _gram(P, Q, rows=1 << 16):
