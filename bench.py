@@ -55,6 +55,9 @@ def parse():
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
     p.add_argument("--pcg-poll", type=int, default=0, help="override PCG iterations between host polls")
+    p.add_argument("--moment-doubling", action="store_true",
+                   help="serving/stepwise: orthogonal jobs take moments from the product identities "
+                        "(errors are still reported against the FP64 three-term recursion)")
     p.add_argument("--serving", action="store_true",
                    help="run the continuous-batching serving trace instead of the chain benchmark")
     p.add_argument("--jobs", type=int, default=64)
@@ -109,7 +112,8 @@ def serving(args):
     vectors in {4,16,64}, 20% nonorthogonal), admitted by slots and HBM budget."""
     import torch
     from kubomsd_amd.serving import TransportServer, synthetic_trace
-    jobs = synthetic_trace(args.jobs, rate_per_s=args.rate, n_range=(1e4, args.n_max))
+    jobs = synthetic_trace(args.jobs, rate_per_s=args.rate, n_range=(1e4, args.n_max),
+                           moment_doubling=args.moment_doubling)
     def report(r):
         print(f"# job {r.job_id} done on slot {r.slot}: latency {r.latency_s:.2f} s"
               + (f", error {r.error}" if r.error else ""), file=sys.stderr, flush=True)
@@ -137,7 +141,8 @@ def serving(args):
             "warmup": 0, "ms_per_step": 1e3 * wall / len(jobs), "higher_is_better": False, "scaling": "weak",
             "vs_baseline": None, "dtype": "c64+f64", "data": "synthetic (seeded honeycomb flakes, Poisson arrivals)",
             "config": {"workload": f"serving: {len(jobs)} jobs, rate {args.rate}/s, N logU[1e4,{args.n_max:.0e}], "
-                                   f"M in 256..4096, vectors in 4/16/64, 20% nonorthogonal, {args.slots} slots",
+                                   f"M in 256..4096, vectors in 4/16/64, 20% nonorthogonal, {args.slots} slots"
+                                   + (", moments by doubling (S = I jobs)" if args.moment_doubling else ""),
                        "parallelism": "single GPU, one worker process per slot"},
             "serving": {**stats, "peak_concurrent_jobs": srv.peak_running, "peak_budget_bytes": srv.peak_used,
                         "vector_orders_total": work, "diffusion_rel_err_vs_fp64": errs}}
@@ -396,7 +401,8 @@ def stepwise(args):
         cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method="stepwise",
                         energy_eV=E, time_fs=T, axes=("x", "y"), spectral_bounds=(-8.8, 8.8), n_moments=M,
                         nsamples=nb, sample_batch=nb, statistics_block=max(1, nb // 8), include_vac=False,
-                        precision=prec, segment_fs=args.dt, autotune=prec == "mixed")
+                        precision=prec, segment_fs=args.dt, autotune=prec == "mixed",
+                        moment_doubling=bool(args.moment_doubling and prec == "mixed"))
         ops = build_operators(bundle, cfg, ctx)
         # The explicit bounds are checked against Gershgorin discs (a proof of
         # containment); a Lanczos Ritz check to 1e-10 residuals does not
@@ -443,7 +449,9 @@ def stepwise(args):
         "ms_per_step": mx["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64+f64", "data": "synthetic (seeded honeycomb lattice, NumPy PCG64 random phases)",
         "config": {"workload": f"stepwise MSD: honeycomb {args.cells}x{args.cells} cells (N={n}), {nb} vectors, "
-                               f"M={M}, axes x,y, t=0..{args.tmax:g} fs every {args.dt:g} fs, 161 energies",
+                               f"M={M}, axes x,y, t=0..{args.tmax:g} fs every {args.dt:g} fs, 161 energies"
+                               + (" (mixed run: moments by doubling, counted as the M-1 orders they replace)"
+                                  if args.moment_doubling else ""),
                    "n": n, "vectors": nb, "moments": M, "times": nt, "precision": "mixed",
                    "parallelism": "single GPU", "launch_shape": mx["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

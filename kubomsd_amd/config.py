@@ -114,6 +114,7 @@ class RunConfig:
     device: int = 0
     device_memory_limit_gb: float = 0.
     autotune: bool = True
+    moment_doubling: bool = False
 
     def __post_init__(self):
         object.__setattr__(self, "energy_eV", grid(self.energy_eV, "energy"))
@@ -195,6 +196,8 @@ class RunConfig:
             raise KuboMSDError("device_memory_limit_gb must be nonnegative and finite")
         if not isinstance(self.autotune, (bool, np.bool_)):
             raise KuboMSDError("autotune must be a boolean")
+        if not isinstance(self.moment_doubling, (bool, np.bool_)):
+            raise KuboMSDError("moment_doubling must be a boolean")
         if self.spectral_bounds is not None:
             b = tuple(float(v) for v in self.spectral_bounds)
             if len(b) != 2 or not np.isfinite(b).all() or b[0] >= b[1]:

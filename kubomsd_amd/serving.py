@@ -382,7 +382,7 @@ def run_job(job, ctx, keep_blocks=False):
 
 def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256, 512, 1024, 2048, 4096),
                     vectors=(4, 16, 64), nonorthogonal_fraction=0.2, precision="mixed", times=(0.0, 0.5, 1.0),
-                    seed=20261208):
+                    seed=20261208, moment_doubling=False):
     """The serving workload: Poisson arrivals; honeycomb flakes with N
     log-uniform in n_range, W ~ U[0.5, 3] eV, M and the vector count drawn from
     the given sets, a fraction with a nonorthogonal overlap; each job is a
@@ -405,7 +405,7 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
         cfg = RunConfig(energy_eV=np.linspace(-7.5, 7.5, 61), time_fs=list(times), solver="stochastic-kpm",
                         kernel="jackson", backend="hip", n_moments=M, axes=("x",), nsamples=nb, sample_batch=nb,
                         statistics_block=max(1, nb // 4), include_vac=False, precision=precision,
-                        metric_rtol=1e-10, seed=20261019 + i, autotune=False)
+                        metric_rtol=1e-10, seed=20261019 + i, autotune=False, moment_doubling=moment_doubling)
         est = estimate_job_bytes(n_act, 4 * n_act, nb, 1, precision, nonorth)
         jobs.append(TransportJob(i, build, cfg, "stepwise", None, float(arrivals[i]), est,
                                  meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth)))

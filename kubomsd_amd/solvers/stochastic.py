@@ -158,6 +158,8 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
             "TRUNCATION_VERIFIED": "not applicable"},
         "statistics_location": "device-resident block sums and bootstrap",
         "backend": {"name": "hip", "precision": cfg.precision, "memory": ops.ctx.memory(),
+                    "moment_recursion": "product identities (doubling)"
+                    if getattr(cfg, "moment_doubling", False) and ops.metric.identity else "three-term",
                     "metric": ops.metric.diagnostics(), "h_action_calls": ops.h_calls},
     }
     return valid, _host(reason).astype("uint32"), static, metadata

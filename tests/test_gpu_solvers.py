@@ -174,3 +174,27 @@ def test_propagation_against_expm_inverse_segments_and_norm(hip_ctx):
     mid = propagate(B, .3, ops, cfg)[0]
     np.testing.assert_allclose(host(propagate(mid, .4, ops, cfg)[0]), out, atol=1e-12)
     np.testing.assert_allclose(np.sum(B.conj() * (S @ B), axis=0), np.sum(out.conj() * (S @ out), axis=0), atol=1e-12)
+
+
+def test_moment_doubling_option_matches_direct_recursion(hip_ctx):
+    """RunConfig(moment_doubling=True) on an identity metric: the same DOS and
+    MSD as the three-term recursion to rounding (the product identities are
+    exact in arithmetic); a nonorthogonal metric keeps the direct recursion."""
+    from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
+    m = square_lattice(12, seed=9)
+    E = np.linspace(-4, 4, 17)
+    B = phases(m.n, 6, 8)
+    out = {}
+    for doubling in (False, True):
+        cfg = run_config(energy_eV=E, time_fs=[0, .5, 1.0], spectral_bounds=(-5.1, 5.1), n_moments=64,
+                         axes=("x", "y"), nsamples=6, statistics_block=2, sample_batch=6, segment_fs=1.0,
+                         moment_doubling=doubling)
+        ops = bundle(hip_ctx, m.H, m.S, m.velocity, cfg)
+        blk = list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=B))[0]
+        out[doubling] = blk
+        assert blk.metadata["backend"]["moment_recursion"] == ("product identities (doubling)" if doubling
+                                                                else "three-term")
+    a, b = out[False], out[True]
+    close(b.static["dos_block_sums"], a.static["dos_block_sums"], 1e-12)
+    close(b.arrays["numerator_block_sums/msd_A2/commutator"], a.arrays["numerator_block_sums/msd_A2/commutator"],
+          1e-10)
