@@ -3,9 +3,12 @@
 // Workgroup tile 128 (j) x 128 (q) of the complex table, four wavefronts in
 // a 2 x 2 arrangement, each owning 64 x 64 = 2 x 2 blocks of 32 x 32 MFMA
 // outputs for the real and the imaginary part (128 accumulator registers per
-// lane).  K (the N orbitals) is walked in slabs of 16 complex entries staged
-// through LDS; the next slab's global loads are issued before the current
-// slab's MFMAs (register prefetch).  The N range is split across workgroups
+// lane).  K (the N orbitals) is walked in slabs of 16 (FP32) or 32 (bf16x3)
+// complex entries staged through LDS; the next slab's global loads are issued
+// before the current slab's MFMAs (register prefetch).  The bf16x3 products
+// take about a fifth of the matrix-core time of the FP32 ones per entry, so
+// its slabs are twice as wide: half the barriers per product and twice the
+// MFMA work to cover the prefetch latency.  The N range is split across workgroups
 // into FP64 partial tables that a second kernel sums in a fixed order.
 //
 // Operand layout: the sum over K is order-free, so each lane half h takes
@@ -34,9 +37,10 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 128;  // workgroup tile (j and q)
-constexpr int KT = 16;     // complex entries per slab
+constexpr int KT = 16;     // complex entries per FP32 slab
+constexpr int KTB = 32;    // complex entries per bf16x3 slab
 constexpr int LSF = 36;    // FP32 LDS row stride (floats; 16-byte rows)
-constexpr int LSB = 40;    // bf16 LDS row stride (16-bit values; 16-byte rows)
+constexpr int LSB = 72;    // bf16 LDS row stride (16-bit values; 16-byte rows, 36 words as LSF)
 
 struct Work {
     int x, y;
@@ -57,19 +61,20 @@ __device__ __forceinline__ Work decode(int gx, int gy, int64_t total) {
 }
 
 // Slab loads: thread t covers tile row t/2 and complex entries
-// [ib + 8(t%2), ib + 8(t%2) + 8) as four float4 (zeros outside the matrix).
+// [ib + 2Q(t%2), ib + 2Q(t%2) + 2Q) as Q float4 (zeros outside the matrix).
+template <int Q>
 __device__ __forceinline__ void load_slab(const float *M, int64_t n, int rows, int r0, int64_t ib, int64_t iend,
-                                          f4 (&v)[4]) {
+                                          f4 (&v)[Q]) {
     const int t = threadIdx.x;
     const int r = r0 + (t >> 1);
-    const int64_t e0 = ib + 8 * (t & 1);
+    const int64_t e0 = ib + 2 * Q * (t & 1);
     const float *p = M + int64_t(r) * 2 * n + 2 * e0;
-    if (r < rows && e0 + 8 <= iend && (n & 1) == 0) {
+    if (r < rows && e0 + 2 * Q <= iend && (n & 1) == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const f4 *>(p + 4 * q);
+        for (int q = 0; q < Q; ++q) v[q] = *reinterpret_cast<const f4 *>(p + 4 * q);
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int64_t e = e0 + (4 * q + s) / 2;
@@ -197,16 +202,16 @@ __global__ __launch_bounds__(256, 2) void mu_bf16x3_kernel(const float *L, const
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) cre[a][b] = cim[a][b] = f16x{};
-    f4 pl[4], pr[4];
+    f4 pl[8], pr[8];
     if (ib0 < iend) {
         load_slab(L, n, m, j0, ib0, iend, pl);
         load_slab(R, n, k, q0, ib0, iend, pr);
     }
-    const int sr = t >> 1, sc = 16 * (t & 1);
-    for (int64_t ib = ib0; ib < iend; ib += KT) {
+    const int sr = t >> 1, sc = 32 * (t & 1);
+    for (int64_t ib = ib0; ib < iend; ib += KTB) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < 4; ++q) {
             u4 h, l;
             split8(pl[2 * q], pl[2 * q + 1], h, l);
             *reinterpret_cast<u4 *>(&lh[sr][sc + 8 * q]) = h;
@@ -216,12 +221,12 @@ __global__ __launch_bounds__(256, 2) void mu_bf16x3_kernel(const float *L, const
             *reinterpret_cast<u4 *>(&rl[sr][sc + 8 * q]) = l;
         }
         __syncthreads();
-        if (ib + KT < iend) {
-            load_slab(L, n, m, j0, ib + KT, iend, pl);
-            load_slab(R, n, k, q0, ib + KT, iend, pr);
+        if (ib + KTB < iend) {
+            load_slab(L, n, m, j0, ib + KTB, iend, pl);
+            load_slab(R, n, k, q0, ib + KTB, iend, pr);
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < 4; ++u) {
             const int f = 16 * u + 8 * half;
             u4 Ah[2], Al[2], Bh[2], Bl[2];
 #pragma unroll
@@ -281,12 +286,13 @@ void mu_mfma(kmsd_ctx *c, const void *left, const void *right, double *table, in
     // at most 2^16 entries per FP32 accumulation run (the FP32 rounding of the
     // accumulators grows with the run length; partial sums are FP64),
     // partials <= 1 GiB.
-    const int64_t slabs = (n + KT - 1) / KT;
+    const int64_t kt = mode == KMSD_MU_BF16X3 ? KTB : KT;
+    const int64_t slabs = (n + kt - 1) / kt;
     int64_t splits = std::max<int64_t>((2048 + gx * gy - 1) / (gx * gy), (n + (int64_t(1) << 16) - 1) >> 16);
     splits = std::min<int64_t>(splits, slabs);
     splits = std::max<int64_t>(1, std::min<int64_t>(splits, (int64_t(1) << 30) / (16 * count)));
     int64_t rows = (n + splits - 1) / splits;
-    rows = ((rows + KT - 1) / KT) * KT;
+    rows = ((rows + kt - 1) / kt) * kt;
     splits = (n + rows - 1) / rows;
     const size_t bytes = size_t(16) * count * splits;
     if (!c->mu_part || c->mu_part_bytes < bytes) {
