@@ -85,6 +85,7 @@ struct StepArgs {
     int32_t w;            // columns processed (multiple of CPL)
     int32_t lpr_log2;
     int32_t ell;          // ELL width (F_ELL)
+    int32_t uniform;      // ELL rows all full, every off-diagonal value equal to val[0]
     uint32_t flags;
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
@@ -371,13 +372,20 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     int32_t jj[EW];
     MT vv[EW];
     MT dd{};
+    const bool uni = a.uniform;
+    const MT uval = uni ? val[0] : MT{};
     auto fetch = [&](int64_t blk, int32_t (&j)[EW], MT (&v)[EW], MT &d) {
         const int64_t r = blk * rpb + rloc;
         if (blk < b1 && r < a.n && colok) {
 #pragma unroll
             for (int e = 0; e < EW; ++e) j[e] = idx[r * EW + e];
+            if (uni) {
 #pragma unroll
-            for (int e = 0; e < EW; ++e) v[e] = val[r * EW + e];
+                for (int e = 0; e < EW; ++e) v[e] = uval;
+            } else {
+#pragma unroll
+                for (int e = 0; e < EW; ++e) v[e] = val[r * EW + e];
+            }
             if (diag) d = dval[r];
         }
     };

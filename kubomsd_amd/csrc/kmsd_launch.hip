@@ -215,6 +215,14 @@ void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz,
         }
     }
     if (!use_ell) p32[n] = int32_t(pos);
+    // Every row full and one off-diagonal value (a periodic lattice with one
+    // hopping): the prefetching ELL kernel reads that value once instead of
+    // streaming n * ell copies of it.  The array stays for the other kernels.
+    if (use_ell && noff == stored && stored > 0) {
+        bool same = true;
+        for (size_t e = comps; e < v64.size() && same; ++e) same = v64[e] == v64[e % comps];
+        m->uniform = same;
+    }
     std::vector<float> v32(v64.size()), d32(dg.size());
     for (size_t e = 0; e < v64.size(); ++e) v32[e] = float(v64[e]);
     for (size_t e = 0; e < dg.size(); ++e) d32[e] = float(dg[e]);
@@ -419,6 +427,7 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
         if (!bsr && A->has_diag) fl |= F_DIAG;
         a.val = f32 ? A->val32 : A->val64;
         a.dval = f32 ? A->dval32 : A->dval64;
+        a.uniform = !bsr && A->ell && A->uniform;
         if (A->vkind == VK_IMAG) alpha = Cplx{-s.alpha.im, s.alpha.re};  // A = i * stored
     }
     if (s.use_beta) fl |= F_BETA;

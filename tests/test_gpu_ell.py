@@ -1,7 +1,8 @@
 """The prefetching fixed-width ELL product kernel (widths 3, 4, 6) against
 the generic row kernel on the same launch shape: bitwise equal moments,
 chains and actions (same summation order), for real, imaginary and complex
-matrix values in complex64 and complex128 blocks; with the automatic
+matrix values in complex64 and complex128 blocks, with the single value of
+a uniform matrix read once or the value array streamed; with the automatic
 (occupancy-derived) workgroup count, moments stay within rounding of the
 FP64 oracle."""
 import numpy as np
@@ -48,6 +49,51 @@ def _stored_per_row(ctx, H):
     info = HipMatrix(ctx, H).info()
     assert info["format"] == "csr"
     return info["stored"] / info["n"]
+
+
+UNIFORM = {"honeycomb_w3": True, "square_w4": True, "circulant_w6": False}
+
+
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_uniform_value_detection(hip_ctx, name):
+    """Periodic lattices with one hopping store full ELL rows of one value,
+    which the prefetching kernel reads once (the bitwise comparison with the
+    generic kernel, which streams the value array, is the test above)."""
+    from kubomsd_amd.backends.hip import HipMatrix
+    assert HipMatrix(hip_ctx, MODELS[name][0]().H).info()["uniform_values"] is UNIFORM[name]
+
+
+def test_one_changed_bond_is_not_uniform(hip_ctx):
+    """One bond of a periodic square lattice changed: the values are no longer
+    uniform, and the prefetching kernel (streamed values) still equals the
+    generic kernel bitwise and SciPy; an open lattice (padded ELL rows) is
+    never uniform."""
+    from kubomsd_amd.backends.hip import HipMatrix
+    m = square_lattice(30, W=2.0)
+    H = m.H.tolil()
+    H[0, 1] = H[1, 0] = -1.25
+    H = H.tocsr()
+    A = HipMatrix(hip_ctx, H)
+    assert not A.info()["uniform_values"]
+    B = phases(m.n, 4, 9)
+    hip_ctx.set_tuning("groups", 64)
+    try:
+        outs = []
+        for ell in (0, 1):
+            hip_ctx.set_tuning("ell_kernel", ell)
+            outs.append(host(A.matmul(to_dev(hip_ctx, B))))
+    finally:
+        hip_ctx.set_tuning("ell_kernel", 1)
+        hip_ctx.set_tuning("groups", 0)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    assert rel_err(outs[1], H @ B) < 1e-13
+    n = 30 * 30
+    i = np.arange(n)
+    right = i[(i % 30) < 29]
+    open_H = sp.csr_matrix((-np.ones(2 * len(right)), (np.r_[right, right + 1], np.r_[right + 1, right])),
+                           shape=(n, n)) + sp.identity(n, format="csr")
+    info = HipMatrix(hip_ctx, open_H).info()
+    assert info["stored"] > open_H.nnz - n and not info["uniform_values"]
 
 
 @pytest.mark.parametrize("name", sorted(MODELS))
