@@ -83,7 +83,9 @@ int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
  * graph; 0: launch every kernel; same results either way), "ell_kernel" (1, the default:
  * products whose off-diagonal part is ELL of width 3, 4 or 6 use the kernel
  * that prefetches the next row's indices; 0: the generic row kernel; same
- * results either way). */
+ * results either way), "fuse_velocity" (1, the default: kmsd_chain_v forms
+ * V T_k inside the chain's steps where it can; 0: always the regrouping path;
+ * same results either way). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an
@@ -233,10 +235,17 @@ int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, int64_t nb, 
                double half);
 /* Reference: the spectral path's cov = V chain(S^-1 V c, K+1)
  * (src/kubo_msd/solvers/spectral.py:33-36, the v_cov of a stored chain):
- * out[k] = v_cov(dir, T_k(H~) B), (L, N, nb).  One velocity product covers all
- * orders (the chain is regrouped to N rows of L*nb columns and back on the
- * device); two scratch blocks of L*N*nb elements come from the budget and
- * stay with the engine for later calls (released by kmsd_engine_destroy). */
+ * out[k] = v_cov(dir, T_k(H~) B), (L, N, nb).  With an identity metric, H a
+ * real ELL matrix of width 3, 4 or 6 and dir along one axis whose velocity
+ * is one real or imaginary matrix with no diagonal and its entries in H's
+ * pattern, each step that gathers T_k for T_(k+1) also forms V T_k (the
+ * velocity's values in H's ELL slots, n*width doubles, are built on the
+ * first call and kept by the engine; the chain itself uses two N*nb
+ * blocks).  Otherwise one velocity product covers all orders (the chain is
+ * regrouped to N rows of L*nb columns and back on the device; two scratch
+ * blocks of L*N*nb elements come from the budget and stay with the engine).
+ * Both give the same bits; the engine's blocks are released by
+ * kmsd_engine_destroy. */
 int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, void *out, int64_t L, int64_t nb, int dtype,
                  double center, double half);
 /* Reference: Engine.moments(B, L, center, half) -> f64 (L, nb)

@@ -378,6 +378,119 @@ extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, i
     });
 }
 
+namespace {
+
+// V's values in the ELL slots of H (zero where V has no entry), for a
+// velocity matrix whose nonzero off-diagonal entries all lie in H's pattern
+// and come in the same column order in every row; false otherwise.
+bool build_slot_values(kmsd_engine *e, const kmsd_matrix *H, const kmsd_matrix *V, kmsd::DBuf &out) {
+    kmsd_ctx *c = e->ctx;
+    const int64_t n = H->n, w = H->ell;
+    std::vector<int32_t> hidx(size_t(n) * size_t(w)), vptr, vidx(size_t(V->noff));
+    std::vector<double> vval(size_t(V->noff));
+    HIPCHECK(hipMemcpyAsync(hidx.data(), H->idx, hidx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (V->noff) {
+        HIPCHECK(hipMemcpyAsync(vidx.data(), V->idx, vidx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(vval.data(), V->val64, vval.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (!V->ell) {
+        vptr.resize(size_t(n) + 1);
+        HIPCHECK(hipMemcpyAsync(vptr.data(), V->ptr, vptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<double> slots(hidx.size(), 0.0);
+    for (int64_t r = 0; r < n; ++r) {
+        const int64_t k0 = V->ell ? r * V->ell : vptr[r], k1 = V->ell ? (r + 1) * V->ell : vptr[r + 1];
+        int64_t pos = 0;
+        for (int64_t k = k0; k < k1; ++k) {
+            if (vval[k] == 0.0) continue;  // ELL padding or a stored zero: adds nothing to a finite row sum
+            while (pos < w && hidx[r * w + pos] != vidx[k]) ++pos;
+            if (pos == w) return false;
+            slots[r * w + pos++] = vval[k];
+        }
+    }
+    void *d = out.ensure(c, slots.size() * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(d, slots.data(), slots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return true;
+}
+
+// The slot values of the one velocity matrix that makes up direction `dir`
+// when kmsd_chain_v can form V T_k inside the chain's steps, with the scale
+// vcov's product applies (dir times i for imaginary values, as launch_step
+// rotates alpha); null when it cannot: a nonidentity metric (each order is a
+// solve, not one launch), H not a real width-3/4/6 ELL matrix, FP32 matrix
+// arithmetic, several axes, a commutator or composite velocity, a complex or
+// BSR velocity, a nonzero velocity diagonal, or entries outside H's slots.
+const double *fused_velocity(kmsd_engine *e, const double *dir, Cplx *valpha) {
+    const kmsd_ctx *c = e->ctx;
+    const kmsd_matrix *H = e->H;
+    if (!c->fuse_velocity || !c->ell_kernel || c->fp32_math || !e->metric->identity) return nullptr;
+    if (H->fmt != KMSD_CSR || H->vkind == VK_COMPLEX || (H->ell != 3 && H->ell != 4 && H->ell != 6)) return nullptr;
+    int ax = -1;
+    for (int a = 0; a < 3; ++a) {
+        if (dir[a] == 0.0) continue;
+        if (ax >= 0) return nullptr;
+        ax = a;
+    }
+    if (ax < 0 || e->kind[ax] != 0 || !e->mats[ax][0]) return nullptr;
+    const kmsd_matrix *V = e->mats[ax][0];
+    if (V->fmt != KMSD_CSR || V->vkind == VK_COMPLEX || V->has_diag || V->n != H->n) return nullptr;
+    if (e->fv_state[ax] == 0) e->fv_state[ax] = build_slot_values(e, H, V, e->fv[ax]) ? 1 : -1;
+    if (e->fv_state[ax] < 0) return nullptr;
+    const Cplx coef{1.0 * dir[ax], 0.0 * dir[ax]};
+    *valpha = V->vkind == VK_IMAG ? Cplx{-coef.im, coef.re} : coef;
+    return static_cast<const double *>(e->fv[ax].p);
+}
+
+// kmsd_chain_v with V T_(k-1) formed by the step that gathers T_(k-1) to make
+// T_k: the chain lives in two engine buffers (T_k overwrites T_(k-2) in
+// place), `out` receives only V T_k, and the last order's V T_(L-1) is one
+// ordinary velocity product.  Same arithmetic as chain_into + vcov.
+void fused_chain_v(kmsd_engine *e, int dtype, const double *dir, const double *vval, Cplx valpha, const void *B,
+                   void *out, int64_t L, int64_t nb, double center, double half) {
+    kmsd_ctx *c = e->ctx;
+    const int64_t n = n_of(e);
+    const size_t stride = size_t(n) * size_t(nb) * elem_bytes(dtype);
+    char *o = static_cast<char *>(out);
+    const void *prev = nullptr, *cur = B;
+    if (L > 1) check_half(half);
+    void *ta = e->t0.ensure(c, stride);
+    void *tb = e->t1.ensure(c, stride);
+    for (int64_t k = 1; k < L; ++k) {
+        void *nxt = k == 1 ? ta : (prev == B ? tb : const_cast<void *>(prev));
+        Step s;
+        s.X = cur;
+        s.ldx = nb;
+        s.OUT = nxt;
+        s.ldo = nb;
+        if (k == 1) {
+            s.alpha = Cplx{1 / half, 0};
+            s.beta = Cplx{-center / half, 0};
+            s.use_beta = true;
+        } else {
+            s.P = prev;
+            s.ldp = nb;
+            s.alpha = Cplx{2 / half, 0};
+            s.beta = Cplx{-2 * center / half, 0};
+            s.gamma = -1.0;
+            s.use_beta = s.use_p = true;
+        }
+        s.n = n;
+        s.w = int32_t(nb);
+        s.vval = vval;
+        s.VOUT = o + size_t(k - 1) * stride;
+        s.ldv = nb;
+        s.valpha = valpha;
+        cheb(e, dtype, s);
+        prev = cur;
+        cur = nxt;
+    }
+    vcov(e, dtype, dir, cur, nb, o + size_t(L - 1) * stride, nb, int32_t(nb), Cplx{1, 0}, false);
+}
+
+}  // namespace
+
 extern "C" int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, void *out, int64_t L, int64_t nb,
                             int dtype, double center, double half) {
     if (!e) return KMSD_EINVAL;
@@ -390,6 +503,12 @@ extern "C" int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, vo
         if (L * nb >= (int64_t(1) << 31)) throw Error(KMSD_EINVAL, "chain too wide for one velocity product");
         const int64_t n = n_of(e);
         require_finite(c, dtype, B, nb, n, int32_t(nb));
+        Cplx valpha;
+        if (const double *vval = fused_velocity(e, dir, &valpha)) {
+            fused_chain_v(e, dtype, dir, vval, valpha, B, out, L, nb, center, half);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            return;
+        }
         chain_into(e, dtype, B, out, L, nb, center, half);
         // One velocity product over all orders: (L, N, nb) -> (N, L*nb) rows,
         // v_cov on L*nb columns, and back.

@@ -23,6 +23,7 @@ struct kmsd_ctx {
     int pcg_poll = 4;       // PCG iterations between host status polls
     bool graphs = true;     // replay moment-chain chunks as HIP graphs (identity metric)
     bool ell_kernel = true; // fixed-width ELL products (widths 3, 4, 6) use the prefetching kernel
+    bool fuse_velocity = true; // kmsd_chain_v forms V T_k inside the chain's steps where V fits H's ELL slots
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
@@ -159,6 +160,12 @@ struct Step {
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
     int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
+    // Fused second product (real width-3/4/6 ELL A with FP64 values): VOUT =
+    // valpha * V X, V's real values in A's ELL slots.
+    const double *vval = nullptr;
+    void *VOUT = nullptr;
+    int64_t ldv = 0;
+    Cplx valpha{1, 0};
 };
 // Launches one fused step; returns the number of workgroups used (the
 // partial-sum row count) so callers can size/reduce partials.
@@ -248,6 +255,10 @@ struct kmsd_engine {
     int64_t h_calls = 0, v_calls = 0;
     kmsd::DBuf t0, t1, t2, t3, tmp, tmp2, tmp3, part, wbuf;
     kmsd::DBuf cv1, cv2;  // kmsd_chain_v regrouped blocks (grow-only, like the other scratch)
+    // Per axis: the velocity matrix's values in H's ELL slots (fused kmsd_chain_v);
+    // state 0 = not built yet, 1 = built, -1 = the velocity does not fit.
+    kmsd::DBuf fv[3];
+    int fv_state[3] = {0, 0, 0};
 };
 
 namespace kmsd {

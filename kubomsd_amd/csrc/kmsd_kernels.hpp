@@ -64,6 +64,7 @@ enum : uint32_t {
     F_DIAG = 1u << 6,   // the product includes the stored diagonal d_i X_i
     F_ELL = 1u << 7,    // off-diagonal part is ELL (ell entries per row, row-major)
     F_MOM2 = 1u << 8,   // partial.re += |X_i|^2, partial.im += Re conj(X_i) t (moment doubling)
+    F_VOUT = 1u << 9,   // VOUT_i = valpha * sum_k V_ik X_k from the same gathers (ell_step_kernel<..., true>)
 };
 
 struct StepArgs {
@@ -87,6 +88,10 @@ struct StepArgs {
     int32_t ell;          // ELL width (F_ELL)
     int32_t uniform;      // ELL rows all full, every off-diagonal value equal to val[0]
     uint32_t flags;
+    const double *vval;   // F_VOUT: real values of a second operator V in the ELL slots of A (n*ell)
+    void *VOUT;           // F_VOUT: output block of V X
+    int64_t ldv;
+    double valpha_re, valpha_im;
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
 
@@ -333,7 +338,12 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 // waits for memory once, instead of an index load -> gather -> index load
 // chain per row.  Products are summed in the same order as step_kernel
 // (entries 0..EW-1, then the diagonal), so both give identical results.
-template <class VT, class MR, bool MC, class AT, int CPL, int EW>
+// VO: the launch also forms V X for a second operator V whose entries lie in
+// A's slots (real values a.vval, zero where V has none, no diagonal) from the
+// gathers it already makes, summed in slot order and scaled as the product
+// epilogue scales (so equal to a separate product of V when V's entries come
+// in the same column order), and stores it to VOUT.
+template <class VT, class MR, bool MC, class AT, int CPL, int EW, bool VO = false>
 __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     using V = cx<VT>;
     using MV = MatVal<MR, MC>;
@@ -402,6 +412,13 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
 #pragma unroll
             for (int e = 0; e < EW; ++e) xg[e].load(X + int64_t(jj[e]) * a.ldx + col);
         }
+        double vw[VO ? EW : 1] = {};
+        if constexpr (VO) {
+            if (active) {
+#pragma unroll
+                for (int e = 0; e < EW; ++e) vw[e] = a.vval[row * EW + e];
+            }
+        }
         int32_t jn[EW] = {};
         MT vn[EW] = {};
         MT dn{};
@@ -416,6 +433,17 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
                 if (diag) MV::fma(y[c], dd, xi.template geta<AT>(c));
             }
             step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
+            if constexpr (VO) {
+                Pack<VT, CPL> vs;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<AT> yv = {AT(0), AT(0)};
+#pragma unroll
+                    for (int e = 0; e < EW; ++e) MatVal<double, false>::fma(yv, vw[e], xg[e].template geta<AT>(c));
+                    vs.set(c, cmul<double>(a.valpha_re, a.valpha_im, to<double>(yv)));
+                }
+                vs.store_nt(static_cast<V *>(a.VOUT) + row * a.ldv + col);
+            }
         }
 #pragma unroll
         for (int e = 0; e < EW; ++e) {

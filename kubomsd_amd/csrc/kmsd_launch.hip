@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 
 #include "kmsd_internal.hpp"
@@ -276,10 +277,19 @@ struct KernelKey {
     bool bsr = false;
     int ell = 0;         // ELL width with the prefetching kernel, 0 = generic kernel
     int lpr_log2 = 6;
+    bool vout = false;   // the ELL kernel also forms V X (FP64 values and row sums, real A)
 };
 
 template <class VT, class MR, bool MC, class AT, int CPL>
 KernelFn pick(const KernelKey &k) {
+    if (k.vout) {
+        if constexpr (std::is_same<MR, double>::value && std::is_same<AT, double>::value && !MC) {
+            if (k.ell == 3) return ell_step_kernel<VT, MR, MC, AT, CPL, 3, true>;
+            if (k.ell == 4) return ell_step_kernel<VT, MR, MC, AT, CPL, 4, true>;
+            if (k.ell == 6) return ell_step_kernel<VT, MR, MC, AT, CPL, 6, true>;
+        }
+        throw Error(KMSD_EDEVICE, "internal error: no fused velocity kernel for these operands");
+    }
     if (k.ell == 3) return ell_step_kernel<VT, MR, MC, AT, CPL, 3>;
     if (k.ell == 4) return ell_step_kernel<VT, MR, MC, AT, CPL, 4>;
     if (k.ell == 6) return ell_step_kernel<VT, MR, MC, AT, CPL, 6>;
@@ -330,7 +340,7 @@ int resident_per_cu(KernelFn fn) {
 // resident slot, so the grid-stride row walk runs as a single wave of
 // workgroups with no partially filled second round.
 Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const kmsd_matrix *A, bool f32,
-                KernelFn *fn_out = nullptr) {
+                KernelFn *fn_out = nullptr, bool vout = false) {
     Shape s;
     const int tuned = c->cpl[dtype == KMSD_C64 ? 0 : 1];
     const int want = tuned ? tuned : default_cpl(dtype, A);
@@ -348,6 +358,7 @@ Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const
     k.f32 = f32;
     k.cpl = s.cpl;
     k.lpr_log2 = s.lpr_log2;
+    k.vout = vout;
     if (A) {
         k.bsr = A->fmt == KMSD_BSR4;
         k.cv = A->vkind == VK_COMPLEX;
@@ -379,6 +390,7 @@ int cpl_limit(const Step &s, int dtype) {
     fit(s.OUT, s.ldo);
     fit(s.W, s.ldw);
     fit(s.ACC, s.lda);
+    fit(s.VOUT, s.ldv);
     return lim;
 }
 
@@ -409,7 +421,10 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     const kmsd_matrix *A = s.A;
     const bool f32 = f32_values(c, dtype, A, s.f32_matrix);
     KernelFn fn = nullptr;
-    Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)), A, f32, &fn);
+    if (s.VOUT && (!A || A->fmt == KMSD_BSR4 || !c->ell_kernel || f32 || A->vkind == VK_COMPLEX || !s.vval ||
+                   (A->ell != 3 && A->ell != 4 && A->ell != 6)))
+        throw Error(KMSD_EDEVICE, "internal error: fused velocity output needs a real width-3/4/6 ELL operator");
+    Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)), A, f32, &fn, s.VOUT != nullptr);
     if (s.groups) {
         if (s.groups < 8 || s.groups % 8) throw Error(KMSD_EDEVICE, "internal error: pinned workgroup count");
         sh.gx = s.groups;
@@ -428,6 +443,14 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
         a.val = f32 ? A->val32 : A->val64;
         a.dval = f32 ? A->dval32 : A->dval64;
         a.uniform = !bsr && A->ell && A->uniform;
+        if (s.VOUT) {
+            fl |= F_VOUT;
+            a.vval = s.vval;
+            a.VOUT = s.VOUT;
+            a.ldv = s.ldv;
+            a.valpha_re = s.valpha.re;
+            a.valpha_im = s.valpha.im;
+        }
         if (A->vkind == VK_IMAG) alpha = Cplx{-s.alpha.im, s.alpha.re};  // A = i * stored
     }
     if (s.use_beta) fl |= F_BETA;

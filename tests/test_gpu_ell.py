@@ -150,3 +150,49 @@ def test_velocity_bond_matrix_imaginary_values(hip_ctx):
         hip_ctx.set_tuning("groups", 0)
     np.testing.assert_array_equal(outs[0], outs[1])
     assert rel_err(outs[1], V @ B) < 1e-13
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+@pytest.mark.parametrize("nb", [1, 3, 4])
+@pytest.mark.parametrize("direction", [(1.0, 0.0, 0.0), (0.0, -0.5, 0.0)])
+def test_fused_velocity_chain_bitwise_equals_regrouped(hip_ctx, dtype, nb, direction):
+    """kmsd_chain_v forms V T_k inside the chain's steps when the velocity's
+    entries lie in H's ELL slots: bitwise the same table as the regrouping
+    path it replaces (chain, one velocity product over all orders, regrouping
+    back), and it allocates none of that path's two L x N x nb buffers."""
+    m = square_lattice(30, W=2.0)
+    L = 40
+    B = to_dev(hip_ctx, phases(m.n, nb, 31), dtype)
+    eng = hip_engine(hip_ctx, m)
+    before = hip_ctx.memory()["used_bytes"]
+    fused = host(eng.chain_v(B, L, 0.0, 5.1, list(direction)))
+    grown = hip_ctx.memory()["used_bytes"] - before
+    hip_ctx.set_tuning("fuse_velocity", 0)
+    try:
+        before = hip_ctx.memory()["used_bytes"]
+        plain = host(eng.chain_v(B, L, 0.0, 5.1, list(direction)))
+        grown_plain = hip_ctx.memory()["used_bytes"] - before
+    finally:
+        hip_ctx.set_tuning("fuse_velocity", 1)
+    np.testing.assert_array_equal(fused, plain)
+    assert np.abs(fused).max() > 0
+    block = L * m.n * nb * (8 if dtype == "c64" else 16)
+    assert grown < block <= grown_plain / 2
+
+
+def test_velocity_outside_hamiltonian_pattern_is_not_fused(hip_ctx):
+    """A velocity entry where H has none (and two axes at once) keep the
+    regrouping path; the table still equals chain + per-order velocity
+    products."""
+    m = square_lattice(16, W=2.0)
+    V = m.velocity["x"].tolil()
+    V[0, 2] = 0.5j
+    V[2, 0] = -0.5j
+    m.velocity["x"] = V.tocsr()
+    eng = hip_engine(hip_ctx, m)
+    B = to_dev(hip_ctx, phases(m.n, 2, 8))
+    for direction in ([1.0, 0, 0], [0.5, 0.25, 0]):
+        ch = eng.chain(B, 9, 0.0, 5.1)
+        want = np.stack([host(eng.v_cov(ch[k].contiguous(), direction)) for k in range(9)])
+        got = host(eng.chain_v(B, 9, 0.0, 5.1, direction))
+        assert rel_err(got, want) < 1e-14
