@@ -152,15 +152,33 @@ def test_velocity_bond_matrix_imaginary_values(hip_ctx):
     assert rel_err(outs[1], V @ B) < 1e-13
 
 
+def _bondwise_rescaled(m):
+    """The lattice's velocities with every bond scaled by its own factor in
+    [1, 1.1) (the same for (i, j) and (j, i)): about one distinct value per
+    bond instead of +-t a / hbar."""
+    for a in ("x", "y"):
+        V = sp.coo_matrix(m.velocity[a])
+        key = np.minimum(V.row, V.col).astype(np.int64) * m.n + np.maximum(V.row, V.col)
+        f = 1 + 0.1 * ((key * 2654435761) % 1000003) / 1000003.0
+        m.velocity[a] = sp.csr_matrix((V.data * f, (V.row, V.col)), shape=V.shape)
+    return m
+
+
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
 @pytest.mark.parametrize("nb", [1, 3, 4])
 @pytest.mark.parametrize("direction", [(1.0, 0.0, 0.0), (0.0, -0.5, 0.0)])
-def test_fused_velocity_chain_bitwise_equals_regrouped(hip_ctx, dtype, nb, direction):
+@pytest.mark.parametrize("values", ["lattice", "bondwise"])
+def test_fused_velocity_chain_bitwise_equals_regrouped(hip_ctx, dtype, nb, direction, values):
     """kmsd_chain_v forms V T_k inside the chain's steps when the velocity's
     entries lie in H's ELL slots: bitwise the same table as the regrouping
     path it replaces (chain, one velocity product over all orders, regrouping
-    back), and it allocates none of that path's two L x N x nb buffers."""
+    back), and it allocates none of that path's two L x N x nb buffers; for
+    the lattice velocity (three distinct values) and a bond-wise rescaled one
+    (one value per bond)."""
     m = square_lattice(30, W=2.0)
+    if values == "bondwise":
+        m = _bondwise_rescaled(m)
+        assert len(np.unique(m.velocity["x"].data)) > 256
     L = 40
     B = to_dev(hip_ctx, phases(m.n, nb, 31), dtype)
     eng = hip_engine(hip_ctx, m)
