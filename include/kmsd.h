@@ -120,8 +120,9 @@ int kmsd_matrix_destroy(kmsd_matrix *m);
 /* No reference counterpart (the reference's CSR exposes no layout): out[7] =
  * {storage format (KMSD_CSR/KMSD_BSR4), n, nnz as given, stored entries
  * (off-diagonal or block entries), device bytes, value kind (0 real,
- * 1 imaginary, 2 complex), uniform values (1: ELL rows all full and every
- * off-diagonal value equal, read once by the width-3/4/6 ELL kernel)}. */
+ * 1 imaginary, 2 complex), uniform values (1: ELL storage whose stored
+ * off-diagonal values, padding aside, are all equal: the width-3/4/6 ELL
+ * kernel reads the value once)}. */
 int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out);
 /* Reference: CSR.matmul(B) (cpp/native.cpp:75-89, 642-646): out = A B. */
 int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int dtype);

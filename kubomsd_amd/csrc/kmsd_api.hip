@@ -280,7 +280,7 @@ extern "C" int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out) {
     out[3] = m->noff;
     out[4] = int64_t(m->bytes);
     out[5] = m->vkind;
-    out[6] = m->uniform ? 1 : 0;
+    out[6] = m->uniform_slot >= 0 ? 1 : 0;
     return KMSD_OK;
 }
 

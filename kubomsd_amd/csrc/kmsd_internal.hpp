@@ -45,7 +45,7 @@ struct kmsd_matrix {
     // nonzero 4x4 blocks, val* 16 row-major entries per block, no split diagonal.
     int64_t noff = 0;        // stored off-diagonal entries (ELL: n*ell incl. padding; BSR4: 16 * blocks)
     int32_t ell = 0;         // ELL width, 0 = CSR
-    bool uniform = false;    // ELL without padding whose off-diagonal values are all equal
+    int64_t uniform_slot = -1;  // ELL whose off-diagonal values are all equal: a slot holding it (-1: not)
     int32_t *ptr = nullptr;  // CSR row pointers of the off-diagonal part (CSR only)
     int32_t *idx = nullptr;
     int vkind = VK_COMPLEX;

@@ -86,7 +86,7 @@ struct StepArgs {
     int32_t w;            // columns processed (multiple of CPL)
     int32_t lpr_log2;
     int32_t ell;          // ELL width (F_ELL)
-    int32_t uniform;      // ELL rows all full, every off-diagonal value equal to val[0]
+    int64_t uniform_slot; // >= 0: every ELL off-diagonal value equals val[uniform_slot] (padding: zero)
     uint32_t flags;
     const double *vval;   // F_VOUT: real values of a second operator V in the ELL slots of A (n*ell)
     void *VOUT;           // F_VOUT: output block of V X
@@ -382,8 +382,8 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     int32_t jj[EW];
     MT vv[EW];
     MT dd{};
-    const bool uni = a.uniform;
-    const MT uval = uni ? val[0] : MT{};
+    const bool uni = a.uniform_slot >= 0;
+    const MT uval = uni ? val[a.uniform_slot] : MT{};
     auto fetch = [&](int64_t blk, int32_t (&j)[EW], MT (&v)[EW], MT &d) {
         const int64_t r = blk * rpb + rloc;
         if (blk < b1 && r < a.n && colok) {
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
             for (int e = 0; e < EW; ++e) j[e] = idx[r * EW + e];
             if (uni) {
 #pragma unroll
-                for (int e = 0; e < EW; ++e) v[e] = uval;
+                for (int e = 0; e < EW; ++e) v[e] = int64_t(j[e]) == r ? MT{} : uval;  // padding slots hold zero
             } else {
 #pragma unroll
                 for (int e = 0; e < EW; ++e) v[e] = val[r * EW + e];

@@ -216,13 +216,19 @@ void matrix_upload(kmsd_ctx *c, kmsd_matrix *m, int fmt, int64_t n, int64_t nnz,
         }
     }
     if (!use_ell) p32[n] = int32_t(pos);
-    // Every row full and one off-diagonal value (a periodic lattice with one
-    // hopping): the prefetching ELL kernel reads that value once instead of
-    // streaming n * ell copies of it.  The array stays for the other kernels.
-    if (use_ell && noff == stored && stored > 0) {
+    // One off-diagonal value (a lattice with one hopping): the prefetching
+    // ELL kernel reads that value once instead of streaming n * ell copies of
+    // it, and uses zero in padding slots (column = own row, which no stored
+    // off-diagonal entry has).  The array stays for the other kernels.
+    if (use_ell && noff > 0) {
+        int64_t first = -1;
         bool same = true;
-        for (size_t e = comps; e < v64.size() && same; ++e) same = v64[e] == v64[e % comps];
-        m->uniform = same;
+        for (int64_t q = 0; q < stored && same; ++q) {
+            if (i32[q] == q / maxlen) continue;  // padding
+            if (first < 0) first = q;
+            for (int t = 0; t < comps; ++t) same = same && v64[size_t(q) * comps + t] == v64[size_t(first) * comps + t];
+        }
+        m->uniform_slot = same ? first : -1;
     }
     std::vector<float> v32(v64.size()), d32(dg.size());
     for (size_t e = 0; e < v64.size(); ++e) v32[e] = float(v64[e]);
@@ -442,7 +448,7 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
         if (!bsr && A->has_diag) fl |= F_DIAG;
         a.val = f32 ? A->val32 : A->val64;
         a.dval = f32 ? A->dval32 : A->dval64;
-        a.uniform = !bsr && A->ell && A->uniform;
+        a.uniform_slot = (!bsr && A->ell) ? A->uniform_slot : -1;
         if (s.VOUT) {
             fl |= F_VOUT;
             a.vval = s.vval;
@@ -570,6 +576,7 @@ void spmm(kmsd_ctx *c, int dtype, const kmsd_matrix *A, const void *X, int64_t l
 kmsd_matrix *matrix_congruence_copy(kmsd_ctx *c, const kmsd_matrix *S, const double *diag) {
     kmsd_matrix *W = new kmsd_matrix(*S);
     W->bytes = 0;
+    W->uniform_slot = -1;  // the scaled values are no longer one value
     W->ptr = W->idx = nullptr;
     W->val64 = W->val32 = W->dval64 = W->dval32 = nullptr;
     try {

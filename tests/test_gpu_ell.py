@@ -39,6 +39,7 @@ def circulant6(n=3000, seed=41):
 
 MODELS = {
     "honeycomb_w3": (lambda: honeycomb_lattice(24, t=-2.7, W=1.0), 3, 8.9),
+    "honeycomb_open_w3": (lambda: honeycomb_lattice(24, t=-2.7, W=1.0, periodic=False), 3, 8.9),
     "square_w4": (lambda: square_lattice(30, W=2.0), 4, 5.1),
     "circulant_w6": (circulant6, 6, 4.0),
 }
@@ -51,14 +52,15 @@ def _stored_per_row(ctx, H):
     return info["stored"] / info["n"]
 
 
-UNIFORM = {"honeycomb_w3": True, "square_w4": True, "circulant_w6": False}
+UNIFORM = {"honeycomb_w3": True, "honeycomb_open_w3": True, "square_w4": True, "circulant_w6": False}
 
 
 @pytest.mark.parametrize("name", sorted(MODELS))
 def test_uniform_value_detection(hip_ctx, name):
-    """Periodic lattices with one hopping store full ELL rows of one value,
-    which the prefetching kernel reads once (the bitwise comparison with the
-    generic kernel, which streams the value array, is the test above)."""
+    """Lattices with one hopping store ELL rows of one value (an open
+    lattice's short rows are padded), which the prefetching kernel reads once,
+    with zero in padding slots (the bitwise comparison with the generic
+    kernel, which streams the value array, is the test above)."""
     from kubomsd_amd.backends.hip import HipMatrix
     assert HipMatrix(hip_ctx, MODELS[name][0]().H).info()["uniform_values"] is UNIFORM[name]
 
@@ -66,8 +68,8 @@ def test_uniform_value_detection(hip_ctx, name):
 def test_one_changed_bond_is_not_uniform(hip_ctx):
     """One bond of a periodic square lattice changed: the values are no longer
     uniform, and the prefetching kernel (streamed values) still equals the
-    generic kernel bitwise and SciPy; an open lattice (padded ELL rows) is
-    never uniform."""
+    generic kernel bitwise and SciPy; the padding of an open lattice's short
+    rows does not count as a second value."""
     from kubomsd_amd.backends.hip import HipMatrix
     m = square_lattice(30, W=2.0)
     H = m.H.tolil()
@@ -93,7 +95,27 @@ def test_one_changed_bond_is_not_uniform(hip_ctx):
     open_H = sp.csr_matrix((-np.ones(2 * len(right)), (np.r_[right, right + 1], np.r_[right + 1, right])),
                            shape=(n, n)) + sp.identity(n, format="csr")
     info = HipMatrix(hip_ctx, open_H).info()
-    assert info["stored"] > open_H.nnz - n and not info["uniform_values"]
+    assert info["stored"] > open_H.nnz - n and info["uniform_values"]
+
+
+def test_sampler_scaled_copy_of_one_value_overlap(hip_ctx):
+    """An overlap stored as ELL with one off-diagonal value and a varying
+    diagonal: the sampler's scaled copy Z S Z (Z = diag(S)^-1/2) has a value
+    per bond, must not keep the one-value storage of S, and its samples equal
+    the oracle's."""
+    from helpers import O
+    from kubomsd_amd.backends.hip import HipMatrix, HipMetric
+    m = square_lattice(20, W=0.0)
+    A = (m.H - sp.diags(m.H.diagonal())).tocsr()
+    rng = np.random.default_rng(12)
+    S = (sp.diags(1.0 + 0.2 * rng.random(m.n)) + 0.05 * A).astype(np.complex128).tocsr()
+    S.sort_indices()
+    assert HipMatrix(hip_ctx, S).info()["uniform_values"]
+    xi = phases(m.n, 3, 4)
+    coeff = np.array([1.1, -.2, .05, -.01])
+    got = host(HipMetric(hip_ctx, S, S.diagonal().real).sample(to_dev(hip_ctx, xi), coeff, 1.0, 0.9))
+    want = O.Metric(O.Csr(S, 1), S.diagonal().real, (), 1).sample(xi, coeff, 1.0, 0.9)
+    assert rel_err(got, want) < 1e-13
 
 
 @pytest.mark.parametrize("name", sorted(MODELS))
