@@ -189,15 +189,15 @@ def _bondwise_rescaled(m):
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
 @pytest.mark.parametrize("nb", [1, 3, 4])
 @pytest.mark.parametrize("direction", [(1.0, 0.0, 0.0), (0.0, -0.5, 0.0)])
-@pytest.mark.parametrize("values", ["lattice", "bondwise"])
+@pytest.mark.parametrize("values", ["lattice", "bondwise", "honeycomb"])
 def test_fused_velocity_chain_bitwise_equals_regrouped(hip_ctx, dtype, nb, direction, values):
     """kmsd_chain_v forms V T_k inside the chain's steps when the velocity's
     entries lie in H's ELL slots: bitwise the same table as the regrouping
     path it replaces (chain, one velocity product over all orders, regrouping
-    back), and it allocates none of that path's two L x N x nb buffers; for
-    the lattice velocity (three distinct values) and a bond-wise rescaled one
-    (one value per bond)."""
-    m = square_lattice(30, W=2.0)
+    back), and it allocates none of that path's two L x N x nb buffers; on
+    the square and honeycomb lattices (ELL widths 4 and 3) and with a
+    bond-wise rescaled velocity (a value per bond)."""
+    m = honeycomb_lattice(16, t=-2.7, W=1.0) if values == "honeycomb" else square_lattice(30, W=2.0)
     if values == "bondwise":
         m = _bondwise_rescaled(m)
         assert len(np.unique(m.velocity["x"].data)) > 256
