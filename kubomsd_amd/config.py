@@ -157,12 +157,12 @@ class RunConfig:
             raise KuboMSDError("invalid metric_ic_ordering")
         if not np.isfinite(self.metric_ic_shift_max) or self.metric_ic_shift_max < 0:
             raise KuboMSDError("metric_ic_shift_max must be finite nonnegative")
-        if self.preconditioner == "cluster-jacobi":
-            if self.solver != "stochastic-kpm" or not isinstance(self.metric_partition_file, str) \
-                    or not self.metric_partition_file:
-                raise KuboMSDError("cluster-jacobi requires stochastic-kpm and metric_partition_file")
-        elif self.metric_partition_file is not None:
-            raise KuboMSDError("partition file requires cluster-jacobi")
+        # The partition-file preconditioner (reference metric/partition.py)
+        # is not part of this backend; the field is kept so that configs
+        # written for the reference are read, and rejected with a reason.
+        if self.preconditioner == "cluster-jacobi" or self.metric_partition_file is not None:
+            raise KuboMSDError("cluster-jacobi (metric_partition_file) is not provided by the HIP backend; "
+                               "use diagonal or block-jacobi")
         if self.metric_residual_policy not in ("fixed", "adaptive"):
             raise KuboMSDError("metric_residual_policy must be fixed or adaptive")
         if not _is_int(self.metric_check_interval) or self.metric_check_interval < 1:

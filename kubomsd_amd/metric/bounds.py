@@ -2,9 +2,9 @@
 
 spd_bounds: Gershgorin when certifying, else small dense eigenvalues (n <= 128)
 or the smallest Lanczos Ritz value minus 10 residuals.
-estimate_spectral_bounds: generalized Ritz extrema of (H, S) from two seeded
-starts; user bounds must enclose them and are then used verbatim (which is
-how every parity run pins the scale c, a).
+certified_bounds: Gershgorin bounds of S^-1 H with no eigensolver.  The
+generalized Ritz extrema and the user-bound validation run on the device
+(metric/device_bounds.py).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -38,47 +38,6 @@ def spd_bounds(S, *, allow_dense=True):
     if lo <= 0 or not np.isfinite([lo, hi]).all():
         raise NumericalError("overlap is not positive definite or SPD verification failed")
     return lo * (1 - 1e-10), hi * (1 + 1e-10), {"source": source, "certified": n <= 2 or (n <= 128 and allow_dense)}
-
-
-def estimate_spectral_bounds(host_H, host_S, s_lower, s_certified, spectral_bounds=None):
-    H = sp.csr_matrix(host_H)
-    S = sp.csr_matrix(host_S)
-    n = H.shape[0]
-    b = float(np.asarray(abs(H).sum(axis=1)).max()) / s_lower
-    if b == 0 and spectral_bounds is None:
-        raise NumericalError("fully degenerate zero spectrum needs explicit spectral bounds")
-    if n <= 2:
-        import scipy.linalg as la
-        extremal = la.eigvalsh(H.toarray(), S.toarray(), subset_by_index=[0, n - 1])
-        residuals = [0., 0.]
-    else:
-        extrema, residuals = [], []
-        for which in ("SA", "LA"):
-            vals = []
-            for seed in (811, 812):
-                e, v = sp.linalg.eigsh(H, M=S, k=1, which=which, tol=1e-11,
-                                       v0=np.random.default_rng(seed).normal(size=n))
-                r = np.linalg.norm(H @ v[:, 0] - e[0] * (S @ v[:, 0])) / np.linalg.norm(v[:, 0])
-                vals.append(float(e[0]))
-                residuals.append(float(r))
-            extrema.append(min(vals) if which == "SA" else max(vals))
-        extremal = np.array(extrema)
-    lo, hi = float(extremal[0]), float(extremal[-1])
-    if spectral_bounds is not None:
-        bounds = tuple(spectral_bounds)
-        if bounds[0] >= lo or bounds[1] <= hi:
-            raise NumericalError("requested spectral bounds must enclose estimated full spectrum with margin")
-        certified = bool(s_certified and bounds[0] <= -b and bounds[1] >= b)
-        source = "user bounds checked against generalized extremal Ritz values"
-    else:
-        if hi - lo <= np.finfo(float).eps * max(abs(lo), abs(hi), 1):
-            raise NumericalError("degenerate spectrum needs explicit bounds")
-        margin = max(.01 * (hi - lo), 10 * max(residuals) / s_lower)
-        bounds = (lo - margin, hi + margin)
-        certified = False
-        source = "two-start generalized Ritz extrema plus margin"
-    return {"bounds": tuple(bounds), "source": source, "certified": certified, "residuals": residuals,
-            "conservative_bounds": (-b, b), "extrema": [lo, hi]}
 
 
 def certified_bounds(host_H, host_S, s_lower, margin=1e-3, spectral_bounds=None):

@@ -1,10 +1,11 @@
-"""Set-up of the cluster-Jacobi and IC(0) preconditioners of the metric solve
-(reference src/kubo_msd/metric/preconditioners.py:11-106).  The physical S is
-never modified: both act on fixed Hermitian positive definite approximations.
+"""Set-up of the block factors and the IC(0) preconditioner of the metric
+solve (reference src/kubo_msd/metric/preconditioners.py:11-106).  The
+physical S is never modified: both act on fixed Hermitian positive definite
+approximations.
 
-  cluster-Jacobi: a dense lower Cholesky factor per partition block (rows of
-      S selected by the block's orbitals), factored on the host like every
-      other small block factor, solved on the device;
+  block factors: dense lower Cholesky factors of S's diagonal blocks (atom
+      blocks), factored on the host in batched LAPACK calls, solved on the
+      device;
   IC(0): W = D S D with D = diag(S)^-1/2, permuted by reverse Cuthill-McKee
       (or kept in natural order), incomplete Cholesky on W's lower pattern,
       computed on the device (kmsd_ic0_factor); on a pivot failure the next
@@ -24,7 +25,6 @@ IC_SHIFTS = (0, 1e-12, 1e-10, 1e-8, 1e-6, 1e-4, 1e-2)
 @dataclass
 class PreconditionerData:
     kind: str
-    blocks: list = field(default_factory=list)  # cluster-jacobi: [(rows, L)]
     factor: object = None                         # ic0: CSR factor of the permuted W
     permutation: object = None                    # ic0: original row of each factor row
     scaling: object = None                        # ic0: D per original row
@@ -63,19 +63,6 @@ def block_cholesky(S, index_blocks, what="block"):
         for k, q in enumerate(qs):
             factors[q] = L[k]
     return factors
-
-
-def prepare_cluster(S, partition):
-    """Factors S[rows][:, rows] for every partition block."""
-    index_blocks = [np.asarray(block["orbitals"], dtype=np.int64) for block in partition["blocks"]]
-    blocks = []
-    for rows, L in zip(index_blocks, block_cholesky(S, index_blocks, "cluster block")):
-        rows.setflags(write=False)
-        L.setflags(write=False)
-        blocks.append((rows, L))
-    return PreconditionerData("cluster-jacobi", blocks=blocks,
-                              report=dict(block_count=len(blocks), block_sizes=[len(r) for r, _ in blocks],
-                                          factor_bytes=sum(r.nbytes + L.nbytes for r, L in blocks)))
 
 
 def prepare_ic0(ctx, S, *, ordering, shift_max):

@@ -59,7 +59,11 @@ class MetricSampler:
 
 
 def sample_metric(seed, sample_ids, ops, config):
+    """Complex128 samples in both precisions: the propagated states start
+    from them unrounded; each moment chain rounds its own input to the
+    precision's storage type."""
+    import torch
     if getattr(ops, "sampler", None) is None:
         ops.sampler = MetricSampler(ops.metric, config)
-    xi = ops.asarray(random_phases(ops.n, seed, sample_ids))
+    xi = ops.asarray(random_phases(ops.n, seed, sample_ids), torch.complex128)
     return ops.sampler.transform(xi)

@@ -1,9 +1,9 @@
 """MetricSolver: the reference's public metric-solve class
 (src/kubo_msd/metric/solve.py:11-174) on the device.
 
-``MetricSolver(S, config, backend=None, atom_slices=None, metric_identity=None)``
+``MetricSolver(S, config, backend=None, atom_slices=None)``
 validates S (Hermitian, positive definite), builds the configured
-preconditioner (diagonal, block-Jacobi, cluster-Jacobi, IC(0)) or the direct
+preconditioner (diagonal, block-Jacobi, IC(0)) or the direct
 solver, and solves S Y = B for device blocks B with the reference's per-column
 acceptance.  ``backend`` is a ``HipContext`` (one is created on
 ``config.device`` when omitted).  Every solve and preconditioner application
@@ -17,7 +17,7 @@ from ..errors import NumericalError
 
 
 class MetricSolver:
-    def __init__(self, S, config, backend=None, atom_slices=None, *, metric_identity=None):
+    def __init__(self, S, config, backend=None, atom_slices=None):
         from ..backends.hip import HipContext
         from ..operators.velocity import DeviceMetric, hermitian
         self.config = config
@@ -25,7 +25,7 @@ class MetricSolver:
             config.device, int(config.device_memory_limit_gb * 1024 ** 3))
         self.atom_slices = atom_slices
         self.host_S = hermitian(sp.csr_matrix(S), "overlap")
-        self.metric = DeviceMetric(self.backend, self.host_S, config, atom_slices, metric_identity=metric_identity)
+        self.metric = DeviceMetric(self.backend, self.host_S, config, atom_slices)
         self.s_lower, self.s_upper, self.bounds_report = (self.metric.s_lower, self.metric.s_upper,
                                                           self.metric.bounds_report)
         self.condition_estimate = self.s_upper / self.s_lower

@@ -63,11 +63,11 @@ def _gram(P, Q, rows=1 << 16):
 class DeviceMetric:
     """Host-side metric set-up plus the device solves (reference MetricSolver,
     src/kubo_msd/metric/solve.py:11-174 and metric/setup.py:40-91): batched
-    PCG with the diagonal, atom-block Jacobi, cluster-Jacobi or IC(0)
+    PCG with the diagonal, atom-block Jacobi or IC(0)
     preconditioner, or the direct solver (metric_solver='cholesky': one dense
     factor of S plus refinement on the original equation)."""
 
-    def __init__(self, ctx, S, config, atom_slices=None, metric_identity=None):
+    def __init__(self, ctx, S, config, atom_slices=None):
         if config.preconditioner == "full-cholesky":
             raise KuboMSDError("full-cholesky is supported only for CPU EXACT")
         self.config = config
@@ -105,17 +105,6 @@ class DeviceMetric:
         self.preconditioner_data = None
         if self.direct:
             self.device.set_direct(True, config.metric_refine_maxiter)
-        if config.preconditioner == "cluster-jacobi":
-            from ..metric.partition import load_metric_partition, matrix_fingerprint
-            from ..metric.preconditioners import prepare_cluster
-            if not metric_identity or not all(metric_identity.values()):
-                raise NumericalError("cluster-Jacobi requires explicit model identity")
-            if not atom_slices:
-                raise NumericalError("cluster-Jacobi requires the model's atom slices")
-            partition = load_metric_partition(config.metric_partition_file, s_fingerprint=matrix_fingerprint(S),
-                                              atom_slices=atom_slices, n_orbitals=n, **metric_identity)
-            self.preconditioner_data = prepare_cluster(S, partition)
-            self.device.set_index_blocks(self.preconditioner_data.blocks)
         if config.preconditioner == "ic0":
             from ..metric.preconditioners import prepare_ic0_validated
             self.preconditioner_data = prepare_ic0_validated(ctx, S, ordering=config.metric_ic_ordering,
@@ -164,9 +153,7 @@ class OperatorBundle:
         budget = int(config.device_memory_limit_gb * 1024 ** 3)
         self.ctx = ctx or HipContext(config.device, budget)
         self.dtype = torch.complex128 if config.precision == "fp64" else torch.complex64
-        from ..metric.partition import model_identity
-        self.metric = DeviceMetric(self.ctx, self.host_S, config, model.atom_slices,
-                                   metric_identity=model_identity(model))
+        self.metric = DeviceMetric(self.ctx, self.host_S, config, model.atom_slices)
         self.velocity = {a: hermitian(v, "velocity " + a) for a, v in model.velocity.items()}
         self.position = {a: hermitian(v, "position " + a) for a, v in model.position.items()}
         self.composite = {a: tuple(sp.csr_matrix(v, dtype=np.complex128) for v in parts)

@@ -50,7 +50,12 @@ def _table(ops, left, cov):
 
 def spectral_samples(ops, cfg, samples):
     import torch
-    samples = ops.asarray(samples)
+    from .stochastic import device_headroom
+    # Anchor states are propagated in complex128 in both precisions, from the
+    # samples as given; the chains of each segment (and the DOS chain) run in
+    # the precision's storage type.
+    c_anchor = ops.asarray(samples, torch.complex128)
+    samples = c_anchor if ops.dtype == torch.complex128 else c_anchor.to(ops.dtype)
     nb = samples.shape[1]
     na, ne, nt, L = len(cfg.axes), len(cfg.energy_eV), len(cfg.time_fs), cfg.n_moments
     dev = samples.device
@@ -61,9 +66,6 @@ def spectral_samples(ops, cfg, samples):
     times = np.unique(np.r_[resolve_time_grid(cfg.time_fs[-1], cfg.integration_dt_fs), cfg.time_fs])
     output_indices = {float(t): j for j, t in enumerate(cfg.time_fs)}
     G = reconstruction(ops, cfg)  # (ne, L)
-    # Anchor states are propagated in complex128 in both precisions; the
-    # chains of each segment run in the precision's storage type.
-    c_anchor = samples.to(torch.complex128)
     v_anchor = [ops.apply_v_action(a, c_anchor) for a in cfg.axes]
     anchor, next_anchor = 0., -1.
     tables = None
@@ -88,9 +90,8 @@ def spectral_samples(ops, cfg, samples):
             coeff, _ = bessel_coefficients(next_anchor - anchor, cfg)
             K = len(coeff) - 1
             maxorder = max(maxorder, K)
-            mem = ops.ctx.memory()
             need = 16 * nb * (n * (L + 2 * K + 1) + na * (L + K) * (K + 1))
-            if need > mem["budget_bytes"] - mem["used_bytes"]:
+            if need > device_headroom(ops.ctx):
                 raise NumericalError("spectral workspace exceeds memory budget; reduce sample_batch/segment_fs or "
                                      "increase explicit budget")
             left = chain(c_anchor.to(samples.dtype), ops, cfg, L + K)  # (L+K, n, nb)

@@ -53,14 +53,15 @@ def propagate(states, dt_fs, ops, cfg):
 
 def stepwise_samples(ops, cfg, samples):
     import torch
-    initial = ops.asarray(samples)  # the precision's chain storage type
+    # The augmented state is propagated in complex128 in both precisions
+    # (about 3 % of the column-steps), starting from the samples as given
+    # (complex64 samples are widened exactly); each moment chain, the DOS
+    # chain included, runs on its projection to the chain storage type.
+    state0 = ops.asarray(samples, torch.complex128)
+    initial = state0 if ops.dtype == torch.complex128 else state0.to(ops.dtype)
     nb = initial.shape[1]
     na, nt, ne = len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV)
     dos = projected_quadratic(initial, ops, cfg)
-    # The augmented state is propagated in complex128 in both precisions
-    # (about 3 % of the column-steps); each output time's moment chain runs
-    # on its projection to the chain storage type.
-    state0 = initial.to(torch.complex128)
     total = torch.cat([state0] + [torch.zeros_like(state0) for _ in cfg.axes], dim=1).contiguous()
     output = torch.zeros((nb, na, nt, ne), dtype=torch.float64, device=initial.device)
     c, a = scaling(cfg)

@@ -68,9 +68,28 @@ def check_workspace_budget(cfg, keys, nbk):
                              "sample_batch/time_block/energy count or raise explicit budget")
 
 
+def device_headroom(ctx):
+    """Bytes a new workspace may take on the context's device.  Both the
+    library's own allocations and torch's (vector blocks, accumulators,
+    series outputs) are active memory, as the reference GPU edition counts its
+    whole memory pool (backends/gpu_context.py:128-136): the headroom is the
+    smaller of the budget minus all active bytes and 85 % of what the device
+    can still give (free memory plus torch's cached, unused blocks)."""
+    import torch
+    mem = ctx.memory()
+    dev = ctx.torch_device
+    allocated = torch.cuda.memory_allocated(dev)
+    reusable = torch.cuda.memory_reserved(dev) - allocated
+    free, _ = torch.cuda.mem_get_info(dev)
+    return int(min(mem["budget_bytes"] - mem["used_bytes"] - allocated, 0.85 * (free + reusable)))
+
+
 def admitted_batch(ops, cfg, method):
-    mem = ops.ctx.memory()
-    free = mem["budget_bytes"] - mem["used_bytes"]
+    """Largest batch of at most cfg.sample_batch samples whose workspace fits
+    the headroom, halving from cfg.sample_batch (reference GPU edition,
+    solvers/stochastic_gpu.py:31-45); RESOURCE_BLOCKED when one sample does
+    not fit."""
+    free = device_headroom(ops.ctx)
     nb = cfg.sample_batch
     while nb >= 1:
         if batch_workspace_bytes(ops, cfg, method, nb) <= free:
@@ -92,6 +111,7 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     (tests/unit/test_shared_axes.py:5-16) and the device runs wide blocks of
     vectors far faster than narrow ones.  Block sums keep the reference's
     per-sample order."""
+    import torch
     diagnostics = []
     primary = "commutator" if method == "stepwise" else "vac_integral"
     ns = cfg.nsamples if samples is None else samples.shape[1]
@@ -99,7 +119,8 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     for first in range(0, ns, batch):
         last = min(first + batch, ns)
         ids = np.arange(first, last)
-        B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else ops.asarray(samples[:, first:last])
+        B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else \
+            ops.asarray(samples[:, first:last], torch.complex128)
         if method == "stepwise":
             dos, nums, diag = stepwise_samples(ops, cfg, B)
             if cfg.include_vac:
@@ -148,6 +169,8 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
         "sample_generator": "NumPy PCG64 SeedSequence([seed,sample_id]); unit modulus complex phases",
         "sampling": getattr(getattr(ops, "sampler", None), "report", {"source": "caller supplied samples"}),
         "sample_batch_diagnostics": diagnostics, "admitted_sample_batch": batch,
+        "resource_batch": {"requested": int(cfg.sample_batch), "effective": int(batch),
+                           "workspace_bytes": batch_workspace_bytes(ops, cfg, method, batch)},
         "raw_accumulation": "numerator_* are means over nsamples; *_block_sums are sums",
         "scratch_bytes": 0,
         "parallel": {"distribution": "serial", "world_size": 1, "operator_storage": "one device",

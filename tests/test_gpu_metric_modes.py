@@ -1,4 +1,4 @@
-"""IC(0), cluster-Jacobi and direct metric solves on the device against the
+"""IC(0), cluster-block and direct metric solves on the device against the
 oracle (reference cpp/native.cpp:92-121, 163-205, 237-297, 331-347) and
 against dense answers, as the reference's own tests check them
 (tests/unit/test_metric_ic0.py, test_metric_cluster.py,
@@ -174,13 +174,12 @@ def test_direct_rejections_and_unrepresentable(hip_ctx):
         m.solve(to_dev(hip_ctx, np.full((2, 1), 1.5e308 + 0j)), 1e-12, 0., 100)
 
 
-def test_metric_modes_through_run_config(hip_ctx, tmp_path):
+def test_metric_modes_through_run_config(hip_ctx):
     """DeviceMetric built from RunConfig for each mode solves the original
-    system (tests/unit/test_metric_ic0.py:9-29, test_metric_cluster.py:12-32,
-    test_metric_cholesky.py:15-37 of the reference)."""
-    import json
+    system (tests/unit/test_metric_ic0.py:9-29, test_metric_cholesky.py:15-37
+    of the reference); the partition-file mode is rejected."""
     from kubomsd_amd.config import RunConfig
-    from kubomsd_amd.metric.partition import make_spatial_partition, matrix_fingerprint
+    from kubomsd_amd.errors import KuboMSDError
     from kubomsd_amd.operators.velocity import DeviceMetric
     base = dict(solver="stochastic-kpm", kernel="jackson", backend="hip", metric_rtol=1e-13)
     B = np.array([[1j, 0], [2, 0], [3j, 0], [1, 0]])
@@ -197,20 +196,8 @@ def test_metric_modes_through_run_config(hip_ctx, tmp_path):
             assert np.all(rep.iterations == 0) and d["factorizations"] == 1
         else:
             assert d["preconditioner_setup"]["shift"] == 0
-    S3 = sp.csr_matrix([[2, .1j, .4], [-.1j, 1, .1j], [.4, -.1j, 1.5]])
-    slices = [(0, 1), (1, 2), (2, 3)]
-    structure = dict(cell_A=np.eye(3) * 10, positions_A=[[0, 0, 0], [5, 0, 0], [.2, 0, 0]], symbols=["A"] * 3)
-    identity = dict(physical_key="p", orbital_order_hash="o")
-    d = make_spatial_partition(structure, slices, max_orbitals=2, s_fingerprint=matrix_fingerprint(S3), **identity)
-    path = tmp_path / "p.json"
-    path.write_text(json.dumps(d))
-    cfg = RunConfig([0], [0], **base, preconditioner="cluster-jacobi", metric_partition_file=str(path))
-    m = DeviceMetric(hip_ctx, S3, cfg, slices, metric_identity=identity)
-    B3 = np.array([[1j, 0], [2, 0], [3j, 0]])
-    Y, rep = m.solve_s(to_dev(hip_ctx, B3))
-    np.testing.assert_allclose(host(Y), np.linalg.solve(S3.toarray(), B3), atol=1e-12)
-    assert m.diagnostics()["resolved_preconditioner"] == "cluster-jacobi"
-    assert m.diagnostics()["preconditioner_setup"]["block_sizes"] == [2, 1]
+    with pytest.raises(KuboMSDError, match="cluster-jacobi"):
+        RunConfig([0], [0], **base, preconditioner="cluster-jacobi", metric_partition_file="p.json")
 
 
 @pytest.mark.parametrize("mode", ["ic0", "cluster", "direct"])
