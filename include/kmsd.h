@@ -63,12 +63,28 @@ typedef struct {
  * (0 = 85% of free device memory) that every library allocation is charged
  * against, as the reference GPU edition's require_bytes does
  * (kubo_msd_gpu/src/kubo_msd/backends/gpu_context.py:128-136). */
+/* Threads: every call on a context, or on a matrix, metric or engine made
+ * from it, is serialized by the context's lock, so any thread may call (or
+ * destroy) them.  All live contexts of one process belong to the host thread
+ * that created the first of them: a create from another thread while those
+ * live returns KMSD_EINVAL ("one host thread"); run concurrent device work in
+ * separate processes. */
 int kmsd_ctx_create(int device, uint64_t hbm_budget_bytes, void *stream, kmsd_ctx **out);
 int kmsd_ctx_destroy(kmsd_ctx *ctx);
 int kmsd_ctx_set_stream(kmsd_ctx *ctx, void *stream);
 int kmsd_ctx_synchronize(kmsd_ctx *ctx);
 /* bytes currently charged to the budget and the budget itself */
 int kmsd_ctx_memory(const kmsd_ctx *ctx, uint64_t *used, uint64_t *budget);
+/* Transfer ledger, the counterpart of the reference GPU edition's transfer
+ * accounting (src/kubo_msd/backends/gpu.py:16-29 upload_bytes /
+ * download_bytes; kubo_msd_gpu/src/kubo_msd/backends/gpu_context.py:75-120):
+ * bytes the library copied on this context since it was created.
+ * out3[0] host-to-device (matrix and metric uploads, coefficient tables,
+ * solver state), out3[1] device-to-host array data (only host-returning
+ * set-up calls such as kmsd_ic0_factor), out3[2] device-to-host status
+ * (nonfinite flags, per-column solver state, reports and norms).  Compute
+ * calls on device blocks copy no array data to the host. */
+int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
 /* Launch tuning: key "groups" (workgroups per launch, multiple of 8; 0 =
  * automatic, the default: each fused step launches one workgroup per
  * resident slot of its kernel on the device), "cpl" (complex columns per

@@ -37,6 +37,7 @@ SIGNATURES = {
     "kmsd_ctx_set_stream": [_P, _P],
     "kmsd_ctx_synchronize": [_P],
     "kmsd_ctx_memory": [_P, _P, _P],
+    "kmsd_ctx_transfers": [_P, _P],
     "kmsd_ctx_set_tuning": [_P, C.c_char_p, _I64],
     "kmsd_autotune": [_P, _P, _I64, _I, _P],
     "kmsd_last_error": [_P],
@@ -159,8 +160,14 @@ class HipContext:
         stream = torch.cuda.current_stream(self.torch_device).cuda_stream
         rc = self.lib.kmsd_ctx_create(self.device, int(hbm_budget_bytes), C.c_void_p(stream), C.byref(h))
         if rc:
-            self._raise(rc, self.lib.kmsd_last_error(None).decode())
+            err = self.lib.kmsd_last_error(None).decode()
+            if rc == KMSD_EINVAL:
+                raise KuboMSDError(err)
+            self._raise(rc, err)
         self.h = h
+        # host-side transfer ledger (bytes moved by this package's own
+        # torch copies; the library keeps its own, kmsd_ctx_transfers)
+        self.host_transfers = {"h2d": 0, "d2h_data": 0, "d2h_status": 0}
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -243,6 +250,35 @@ class HipContext:
         self.set_tuning("cpl_c64" if dtype == torch.complex64 else "cpl_c128", choice["cpl"])
         self.set_tuning("groups", choice["groups"])
         return dict(choice)
+
+    def upload(self, array, dtype=None):
+        """Host array -> device tensor on this context (counted as H2D)."""
+        torch = _torch()
+        t = torch.from_numpy(np.ascontiguousarray(array)).to(self.torch_device)
+        self.host_transfers["h2d"] += t.numel() * t.element_size()
+        return t if dtype is None else t.to(dtype)
+
+    def to_host(self, t, status=False):
+        """Device tensor -> NumPy array, counted as array data or (status=True)
+        as control information (flags, per-column norms, scalars)."""
+        if not hasattr(t, "detach"):
+            return np.asarray(t)
+        out = t.detach().cpu().numpy()
+        self.host_transfers["d2h_status" if status else "d2h_data"] += out.nbytes
+        return out
+
+    def status_value(self, t):
+        """One device scalar read for a control decision (counted as status)."""
+        return self.to_host(t.reshape(()), status=True).item()
+
+    def transfers(self):
+        """Bytes moved between host and device on this context: by the library
+        (kmsd_ctx_transfers) and by this package's torch copies."""
+        v = np.zeros(3, np.uint64)
+        self.check(self.lib.kmsd_ctx_transfers(self.h, _np_ptr(v)))
+        return dict(library_h2d=int(v[0]), library_d2h_data=int(v[1]), library_d2h_status=int(v[2]),
+                    host_h2d=self.host_transfers["h2d"], host_d2h_data=self.host_transfers["d2h_data"],
+                    host_d2h_status=self.host_transfers["d2h_status"])
 
     def memory(self):
         used, budget = C.c_uint64(), C.c_uint64()

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace kmsd;
@@ -18,12 +19,31 @@ namespace kmsd {
 thread_local std::string g_create_error;
 }
 
+namespace {
+// Live contexts of this process and the host thread they belong to: all of a
+// process's contexts are created from one thread (see kmsd_ctx_create).
+std::mutex g_owner_mu;
+std::thread::id g_owner;
+int64_t g_live = 0;
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // context
 
 extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_ctx **out) {
     if (!out) return KMSD_EINVAL;
     *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_owner_mu);
+        if (g_live > 0 && g_owner != std::this_thread::get_id()) {
+            g_create_error =
+                "libkmsd contexts of one process belong to one host thread: another thread holds live "
+                "contexts (run concurrent device work in separate processes)";
+            return KMSD_EINVAL;
+        }
+        g_owner = std::this_thread::get_id();
+        ++g_live;
+    }
     kmsd_ctx *c = new kmsd_ctx();
     const int rc = guard(nullptr, [&] {
         int count = 0;
@@ -44,14 +64,16 @@ extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_c
     if (rc) {
         g_create_error = c->err;
         delete c;
+        std::lock_guard<std::mutex> lock(g_owner_mu);
+        --g_live;
         return rc;
     }
     *out = c;
     return KMSD_OK;
 }
 
-extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
-    if (!c) return KMSD_OK;
+namespace kmsd {
+void ctx_free(kmsd_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->flag) (void)hipFree(c->flag);
@@ -59,11 +81,29 @@ extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
     if (c->stats_tab) (void)hipFree(c->stats_tab);
     if (c->capture) (void)hipStreamDestroy(c->capture);
     delete c;
+}
+}  // namespace kmsd
+
+extern "C" int kmsd_ctx_destroy(kmsd_ctx *c) {
+    if (!c) return KMSD_OK;
+    bool free_now = false;
+    {
+        std::lock_guard<std::recursive_mutex> lock(c->mu);
+        if (c->closed) return KMSD_EINVAL;
+        c->closed = true;
+        free_now = c->children == 0;
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_owner_mu);
+        --g_live;
+    }
+    if (free_now) ctx_free(c);
     return KMSD_OK;
 }
 
 extern "C" int kmsd_ctx_set_stream(kmsd_ctx *c, void *stream) {
     if (!c) return KMSD_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(c->mu);
     c->stream = static_cast<hipStream_t>(stream);
     return KMSD_OK;
 }
@@ -71,6 +111,15 @@ extern "C" int kmsd_ctx_set_stream(kmsd_ctx *c, void *stream) {
 extern "C" int kmsd_ctx_synchronize(kmsd_ctx *c) {
     if (!c) return KMSD_EINVAL;
     return guard(c, [&] { HIPCHECK(hipStreamSynchronize(c->stream)); });
+}
+
+extern "C" int kmsd_ctx_transfers(const kmsd_ctx *c, uint64_t *out3) {
+    if (!c || !out3) return KMSD_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(c->mu);
+    out3[0] = c->h2d_bytes;
+    out3[1] = c->d2h_data_bytes;
+    out3[2] = c->d2h_status_bytes;
+    return KMSD_OK;
 }
 
 extern "C" int kmsd_ctx_memory(const kmsd_ctx *c, uint64_t *used, uint64_t *budget) {
@@ -255,7 +304,10 @@ extern "C" int kmsd_matrix_create(kmsd_ctx *c, int fmt, int64_t n, int64_t nnz, 
     *out = nullptr;
     kmsd_matrix *m = new kmsd_matrix();
     m->ctx = c;
-    const int rc = guard(c, [&] { matrix_upload(c, m, fmt, n, nnz, rowptr, colidx, values); });
+    const int rc = guard(c, [&] {
+        matrix_upload(c, m, fmt, n, nnz, rowptr, colidx, values);
+        ctx_adopt(c);
+    });
     if (rc) {
         matrix_release(m);
         delete m;
@@ -267,8 +319,15 @@ extern "C" int kmsd_matrix_create(kmsd_ctx *c, int fmt, int64_t n, int64_t nnz, 
 
 extern "C" int kmsd_matrix_destroy(kmsd_matrix *m) {
     if (!m) return KMSD_OK;
-    matrix_release(m);
-    delete m;
+    kmsd_ctx *c = m->ctx;
+    bool free_ctx = false;
+    {
+        std::lock_guard<std::recursive_mutex> lock(c->mu);
+        matrix_release(m);
+        delete m;
+        free_ctx = ctx_orphan(c);
+    }
+    if (free_ctx) ctx_free(c);
     return KMSD_OK;
 }
 

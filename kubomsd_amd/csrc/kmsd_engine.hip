@@ -206,6 +206,7 @@ extern "C" int kmsd_engine_create(kmsd_ctx *c, kmsd_matrix *H, kmsd_metric *metr
         e->atol = atol;
         e->maxiter = maxiter;
         e->hbar = hbar;
+        ctx_adopt(c);
     });
     if (rc) {
         delete e;
@@ -216,12 +217,21 @@ extern "C" int kmsd_engine_create(kmsd_ctx *c, kmsd_matrix *H, kmsd_metric *metr
 }
 
 extern "C" int kmsd_engine_destroy(kmsd_engine *e) {
-    delete e;
+    if (!e) return KMSD_OK;
+    kmsd_ctx *c = e->ctx;
+    bool free_ctx = false;
+    {
+        std::lock_guard<std::recursive_mutex> lock(c->mu);
+        delete e;
+        free_ctx = ctx_orphan(c);
+    }
+    if (free_ctx) ctx_free(c);
     return KMSD_OK;
 }
 
 extern "C" int kmsd_engine_set_controls(kmsd_engine *e, double rtol, double atol, int maxiter) {
     if (!e) return KMSD_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(e->ctx->mu);
     e->rtol = rtol;
     e->atol = atol;
     e->maxiter = maxiter;
@@ -380,39 +390,50 @@ extern "C" int kmsd_chain(kmsd_engine *e, const void *B, void *out, int64_t L, i
 
 namespace {
 
-// V's values in the ELL slots of H (zero where V has no entry), for a
-// velocity matrix whose nonzero off-diagonal entries all lie in H's pattern
-// and come in the same column order in every row; false otherwise.
-bool build_slot_values(kmsd_engine *e, const kmsd_matrix *H, const kmsd_matrix *V, kmsd::DBuf &out) {
-    kmsd_ctx *c = e->ctx;
-    const int64_t n = H->n, w = H->ell;
-    std::vector<int32_t> hidx(size_t(n) * size_t(w)), vptr, vidx(size_t(V->noff));
-    std::vector<double> vval(size_t(V->noff));
-    HIPCHECK(hipMemcpyAsync(hidx.data(), H->idx, hidx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    if (V->noff) {
-        HIPCHECK(hipMemcpyAsync(vidx.data(), V->idx, vidx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipMemcpyAsync(vval.data(), V->val64, vval.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    }
-    if (!V->ell) {
-        vptr.resize(size_t(n) + 1);
-        HIPCHECK(hipMemcpyAsync(vptr.data(), V->ptr, vptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    std::vector<double> slots(hidx.size(), 0.0);
-    for (int64_t r = 0; r < n; ++r) {
-        const int64_t k0 = V->ell ? r * V->ell : vptr[r], k1 = V->ell ? (r + 1) * V->ell : vptr[r + 1];
-        int64_t pos = 0;
+// V's values in the ELL slots of H (zero where V has no entry), one thread
+// per row: each nonzero entry of V's row is matched, in column order, to the
+// next slot of H's row with the same column.  *fail is set when an entry has
+// no such slot (V's pattern is not inside H's, or its column order differs).
+__global__ __launch_bounds__(256) void slot_values_kernel(const int32_t *hidx, int64_t n, int32_t w,
+                                                          const int32_t *vptr, int32_t vell, const int32_t *vidx,
+                                                          const double *vval, double *slots, int32_t *fail) {
+    for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) {
+        const int64_t k0 = vell ? r * vell : vptr[r], k1 = vell ? (r + 1) * vell : vptr[r + 1];
+        double *row = slots + r * w;
+        const int32_t *hrow = hidx + r * w;
+        for (int32_t s = 0; s < w; ++s) row[s] = 0.0;
+        int32_t pos = 0;
         for (int64_t k = k0; k < k1; ++k) {
-            if (vval[k] == 0.0) continue;  // ELL padding or a stored zero: adds nothing to a finite row sum
-            while (pos < w && hidx[r * w + pos] != vidx[k]) ++pos;
-            if (pos == w) return false;
-            slots[r * w + pos++] = vval[k];
+            const double v = vval[k];
+            if (v == 0.0) continue;  // ELL padding or a stored zero: adds nothing to a finite row sum
+            const int32_t col = vidx[k];
+            while (pos < w && hrow[pos] != col) ++pos;
+            if (pos == w) {
+                atomicOr(fail, 1);
+                break;
+            }
+            row[pos++] = v;
         }
     }
-    void *d = out.ensure(c, slots.size() * sizeof(double));
-    HIPCHECK(hipMemcpyAsync(d, slots.data(), slots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
+
+// Builds the slot values of V on the device; false when V does not fit H's
+// slots.  Only the 4-byte failure flag comes back to the host.
+bool build_slot_values(kmsd_engine *e, const kmsd_matrix *H, const kmsd_matrix *V, kmsd::DBuf &out) {
+    kmsd_ctx *c = e->ctx;
+    const int64_t n = H->n;
+    const int32_t w = H->ell;
+    double *d = static_cast<double *>(out.ensure(c, size_t(n) * size_t(w) * sizeof(double)));
+    int32_t zero = 0, fail = 0;
+    copy_h2d(c, c->flag, &zero, sizeof(zero));
+    const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)));
+    hipLaunchKernelGGL(slot_values_kernel, dim3(g), dim3(256), 0, c->stream, H->idx, n, w, V->ptr, V->ell, V->idx,
+                       static_cast<const double *>(V->val64), d, c->flag);
+    HIPCHECK(hipGetLastError());
+    copy_d2h(c, &fail, c->flag, sizeof(fail), true);
     HIPCHECK(hipStreamSynchronize(c->stream));
-    return true;
+    if (fail) out.release();
+    return fail == 0;
 }
 
 // The slot values of the one velocity matrix that makes up direction `dir`

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -10,6 +11,11 @@
 #include "kmsd.h"
 
 struct kmsd_ctx {
+    // Serializes every call on this context and on the objects created from
+    // it, whichever host thread makes it (a destroy from a garbage-collector
+    // thread waits for a running call instead of freeing its buffers or
+    // touching its stream mid-call).
+    mutable std::recursive_mutex mu;
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
@@ -30,6 +36,16 @@ struct kmsd_ctx {
     size_t mu_part_bytes = 0;
     void *stats_tab = nullptr;  // coefficient table of kmsd_time_gradient (grow-only)
     size_t stats_tab_bytes = 0;
+    // Transfer ledger (kmsd_ctx_transfers): bytes the library copied between
+    // host and device on this context.
+    uint64_t h2d_bytes = 0;         // uploads: matrices, metric data, coefficient tables, solver state
+    uint64_t d2h_data_bytes = 0;    // downloads of array data (host-returning set-up calls)
+    uint64_t d2h_status_bytes = 0;  // downloads of flags, per-column solver state and norms
+    // Matrices, metrics and engines made from this context that are still
+    // alive.  kmsd_ctx_destroy only closes a context that has any; the last
+    // child's destroy frees it, so objects may be destroyed in any order.
+    int64_t children = 0;
+    bool closed = false;
 };
 
 // Value kinds of a stored matrix.
@@ -81,10 +97,31 @@ struct Error : std::runtime_error {
         }                                                                                                   \
     } while (0)
 
+// Host <-> device copies on the context's stream, counted in its transfer
+// ledger.  status: the bytes are flags, solver reports or per-column norms
+// (control information), not array data.
+inline void copy_h2d(kmsd_ctx *c, void *dst, const void *src, size_t bytes) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    c->h2d_bytes += bytes;
+}
+inline void copy_d2h(kmsd_ctx *c, void *dst, const void *src, size_t bytes, bool status) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    (status ? c->d2h_status_bytes : c->d2h_data_bytes) += bytes;
+}
+
 extern thread_local std::string g_create_error;
+
+// Child objects of a context (c->mu held): adopt on creation; orphan on
+// destruction returns true when c was closed and this was its last child, and
+// the caller then frees c with ctx_free after releasing c->mu.
+inline void ctx_adopt(kmsd_ctx *c) { ++c->children; }
+inline bool ctx_orphan(kmsd_ctx *c) { return --c->children == 0 && c->closed; }
+void ctx_free(kmsd_ctx *c);
 
 template <class F>
 int guard(kmsd_ctx *c, F &&f) {
+    std::unique_lock<std::recursive_mutex> lock;
+    if (c) lock = std::unique_lock<std::recursive_mutex>(c->mu);
     try {
         f();
         if (c) c->err.clear();

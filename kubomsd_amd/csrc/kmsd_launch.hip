@@ -84,7 +84,7 @@ T *upload(kmsd_ctx *c, kmsd_matrix *m, const std::vector<T> &h) {
     const size_t bytes = sizeof(T) * std::max<size_t>(1, h.size());
     T *d = static_cast<T *>(dalloc(c, bytes));
     m->bytes += bytes;
-    if (!h.empty()) HIPCHECK(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    if (!h.empty()) copy_h2d(c, d, h.data(), sizeof(T) * h.size());
     return d;
 }
 }  // namespace
@@ -536,7 +536,7 @@ void launch_convert(kmsd_ctx *c, int din, const void *X, int64_t ldx, int dout, 
 
 bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w) {
     int32_t zero = 0, got = 0;
-    HIPCHECK(hipMemcpyAsync(c->flag, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    copy_h2d(c, c->flag, &zero, sizeof(zero));
     const int64_t g = blocks_for(n * w);
     if (dtype == KMSD_C128)
         hipLaunchKernelGGL(nonfinite_kernel<double>, dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx, n, w,
@@ -545,7 +545,7 @@ bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_
         hipLaunchKernelGGL(nonfinite_kernel<float>, dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx, n, w,
                            c->flag);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(&got, c->flag, sizeof(got), hipMemcpyDeviceToHost, c->stream));
+    copy_d2h(c, &got, c->flag, sizeof(got), true);
     HIPCHECK(hipStreamSynchronize(c->stream));
     return got != 0;
 }

@@ -9,12 +9,20 @@
 
 using namespace kmsd;
 
+namespace {
+void metric_free(kmsd_metric *m);
+}
+
 extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *diag, int64_t nblocks,
                                   const int64_t *block_ptr, const double *factors, kmsd_metric **out) {
     if (!c || !out) return KMSD_EINVAL;
     *out = nullptr;
     kmsd_metric *m = new kmsd_metric();
     m->ctx = c;
+    {
+        std::lock_guard<std::recursive_mutex> lock(c->mu);
+        ctx_adopt(c);  // the failure path below destroys m, which orphans it
+    }
     const int rc = guard(c, [&] {
         if (!S || !diag) throw Error(KMSD_EINVAL, "invalid metric diagonal/threads");
         const int64_t n = S->n;
@@ -38,7 +46,7 @@ extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *dia
         m->identity = S->identity && diag_one && nblocks == 0;
         m->diag = static_cast<double *>(dalloc(c, sizeof(double) * n));
         m->bytes = sizeof(double) * n;
-        HIPCHECK(hipMemcpyAsync(m->diag, diag, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        copy_h2d(c, m->diag, diag, sizeof(double) * n);
         if (nblocks) metric_upload_blocks(m, nblocks, block_ptr, nullptr, factors);
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
@@ -53,6 +61,19 @@ extern "C" int kmsd_metric_create(kmsd_ctx *c, kmsd_matrix *S, const double *dia
 extern "C" int kmsd_metric_destroy(kmsd_metric *m) {
     if (!m) return KMSD_OK;
     kmsd_ctx *c = m->ctx;
+    bool free_ctx = false;
+    {
+        std::lock_guard<std::recursive_mutex> lock(c->mu);
+        metric_free(m);
+        free_ctx = ctx_orphan(c);
+    }
+    if (free_ctx) ctx_free(c);
+    return KMSD_OK;
+}
+
+namespace {
+void metric_free(kmsd_metric *m) {
+    kmsd_ctx *c = m->ctx;
     (void)hipStreamSynchronize(c->stream);
     pcg_release(m);
     if (m->W) {
@@ -66,8 +87,8 @@ extern "C" int kmsd_metric_destroy(kmsd_metric *m) {
     if (m->diag) (void)hipFree(m->diag);
     c->used -= std::min<uint64_t>(c->used, nfac_bytes);
     delete m;
-    return KMSD_OK;
 }
+}  // namespace
 
 extern "C" int kmsd_metric_set_residual_policy(kmsd_metric *m, int adaptive, int interval, double gap) {
     if (!m) return KMSD_EINVAL;

@@ -231,7 +231,7 @@ struct Solver {
 
     PcgFlags read_flags() {
         PcgFlags h{};
-        HIPCHECK(hipMemcpyAsync(&h, fl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        copy_d2h(c, &h, fl, sizeof(h), true);
         HIPCHECK(hipStreamSynchronize(c->stream));
         return h;
     }
@@ -338,8 +338,8 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
     Outcome o;
     o.cs.resize(w);
-    HIPCHECK(hipMemcpyAsync(o.cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(&o.fl, S.fl, sizeof(PcgFlags), hipMemcpyDeviceToHost, c->stream));
+    copy_d2h(c, o.cs.data(), S.cs, sizeof(ColState) * w, true);
+    copy_d2h(c, &o.fl, S.fl, sizeof(PcgFlags), true);
     HIPCHECK(hipStreamSynchronize(c->stream));
     return o;
 }
@@ -381,7 +381,7 @@ void account(kmsd_metric *m, const Outcome &o, int64_t extra_checks) {
 std::vector<double> true_norms(Solver &S) {
     S.row(PS_TRUEGAP);
     std::vector<double> h(size_t(S.rs.gx) * size_t(S.w) * 2), out(size_t(S.w), 0.0);
-    HIPCHECK(hipMemcpyAsync(h.data(), S.part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S.c->stream));
+    copy_d2h(S.c, h.data(), S.part, h.size() * sizeof(double), true);
     HIPCHECK(hipStreamSynchronize(S.c->stream));
     for (int64_t g = 0; g < S.rs.gx; ++g)
         for (int32_t j = 0; j < S.w; ++j) out[size_t(j)] += h[(size_t(g) * S.w + j) * 2];
@@ -424,7 +424,7 @@ Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo
     S.precondition(false);  // Z = M^-1 q
     copy_into_y(false);
     std::vector<ColState> cs(static_cast<size_t>(w));
-    HIPCHECK(hipMemcpyAsync(cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
+    copy_d2h(c, cs.data(), S.cs, sizeof(ColState) * w, true);
     HIPCHECK(hipStreamSynchronize(c->stream));
     for (int step = 0; step < m->refine_maxiter; ++step) {
         S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);  // T = q - S y
@@ -437,7 +437,7 @@ Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo
         }
         if (!needed) break;
         // R = T on the columns still above threshold, 0 elsewhere
-        HIPCHECK(hipMemcpyAsync(S.cs, cs.data(), sizeof(ColState) * w, hipMemcpyHostToDevice, c->stream));
+        copy_h2d(c, S.cs, cs.data(), sizeof(ColState) * w);
         launch_scale(c, KMSD_C128, S.R, w, S.R, w, n, w, Cplx{0, 0});
         S.row(PS_RESTART);
         S.precondition(false);
@@ -445,15 +445,15 @@ Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo
         ++m->refinement_steps;
     }
     for (ColState &x : cs) x.restart = 0;
-    HIPCHECK(hipMemcpyAsync(S.cs, cs.data(), sizeof(ColState) * w, hipMemcpyHostToDevice, c->stream));
+    copy_h2d(c, S.cs, cs.data(), sizeof(ColState) * w);
     S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
     S.row(PS_TRUEGAP);
     S.col(PC_FINAL, S.part, S.rs.gx, rtol, atol);
     S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
     Outcome o;
     o.cs.resize(w);
-    HIPCHECK(hipMemcpyAsync(o.cs.data(), S.cs, sizeof(ColState) * w, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(&o.fl, S.fl, sizeof(PcgFlags), hipMemcpyDeviceToHost, c->stream));
+    copy_d2h(c, o.cs.data(), S.cs, sizeof(ColState) * w, true);
+    copy_d2h(c, &o.fl, S.fl, sizeof(PcgFlags), true);
     HIPCHECK(hipStreamSynchronize(c->stream));
     return o;
 }
@@ -547,7 +547,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.w = nb;
         const int64_t g = launch_step(c, KMSD_C128, s);
         std::vector<double> h(size_t(g) * nb * 2), out(nb, 0.0);
-        HIPCHECK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        copy_d2h(c, h.data(), nrm, h.size() * sizeof(double), true);
         HIPCHECK(hipStreamSynchronize(c->stream));
         for (int64_t q = 0; q < g; ++q)
             for (int32_t j = 0; j < nb; ++j) out[j] += h[(size_t(q) * nb + j) * 2];
@@ -576,7 +576,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
             sc[j] = ColState{};
             sc[j].safe = rn[j] > 0 ? rn[j] / std::sqrt(double(n)) : 1.0;
         }
-        HIPCHECK(hipMemcpyAsync(cs2, sc.data(), sizeof(ColState) * nb, hipMemcpyHostToDevice, c->stream));
+        copy_h2d(c, cs2, sc.data(), sizeof(ColState) * nb);
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
         hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream, rr,
                            int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);

@@ -273,7 +273,7 @@ template <class T>
 T *upload(kmsd_ctx *c, const T *host, size_t count, std::vector<std::pair<void *, size_t>> *track) {
     T *d = static_cast<T *>(dalloc(c, sizeof(T) * std::max<size_t>(count, 1)));
     if (track) track->push_back({d, sizeof(T) * std::max<size_t>(count, 1)});
-    if (count) HIPCHECK(hipMemcpyAsync(d, host, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    if (count) copy_h2d(c, d, host, sizeof(T) * count);
     return d;
 }
 
@@ -686,8 +686,8 @@ extern "C" int kmsd_ic0_factor(kmsd_ctx *c, int64_t n, int64_t nnz, const int64_
                                    reinterpret_cast<cx<double> *>(df), shift, dfail);
             }
             HIPCHECK(hipGetLastError());
-            HIPCHECK(hipMemcpyAsync(&fail, dfail, sizeof(fail), hipMemcpyDeviceToHost, c->stream));
-            if (nz) HIPCHECK(hipMemcpyAsync(out, df, 16 * size_t(nz), hipMemcpyDeviceToHost, c->stream));
+            copy_d2h(c, &fail, dfail, sizeof(fail), true);
+            if (nz) copy_d2h(c, out, df, 16 * size_t(nz), false);
             HIPCHECK(hipStreamSynchronize(c->stream));
         } catch (...) {
             for (auto &a : track) dfree(c, a.first, a.second);

@@ -236,7 +236,7 @@ extern "C" int kmsd_time_gradient(kmsd_ctx *c, const double *src, double *dst, i
         // The table is copied from host memory that lives until the
         // synchronization below, which also orders it before a later call
         // rewrites it.
-        HIPCHECK(hipMemcpyAsync(c->stats_tab, tab.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        copy_h2d(c, c->stats_tab, tab.data(), bytes);
         hipLaunchKernelGGL(gradient_kernel, dim3(unsigned(grid_for(outer * nt * inner))), dim3(256), 0, c->stream,
                            src, dst, outer, nt, inner, static_cast<const GradCoef *>(c->stats_tab), scale);
         HIPCHECK(hipGetLastError());

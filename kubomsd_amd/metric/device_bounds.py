@@ -65,7 +65,7 @@ class _Lanczos:
             b = torch.sqrt(torch.clamp(_sinner(self.ops, U, U), min=0.0))
             self.alphas.append(a)
             self.betas.append(b)
-            bh = b.cpu().numpy()
+            bh = self.ops.ctx.to_host(b, status=True)
             if np.any(bh <= self.tiny):
                 # invariant subspace in some column: its Ritz values are exact
                 self.broken = True
@@ -75,7 +75,9 @@ class _Lanczos:
 
     def coefficients(self):
         import torch
-        return torch.stack(self.alphas).cpu().numpy(), torch.stack(self.betas).cpu().numpy()
+        ctx = self.ops.ctx
+        return (ctx.to_host(torch.stack(self.alphas), status=True),
+                ctx.to_host(torch.stack(self.betas), status=True))
 
 
 def _ritz(alpha, beta):
@@ -100,7 +102,7 @@ def ritz_extrema(ops, seeds=SEEDS, max_steps=512, check=16, tol=1e-10):
     cap = min(n, max_steps)
     V0 = ops.asarray(_start_block(n, seeds), torch.complex128)
     nrm = torch.sqrt(_sinner(ops, V0, V0))
-    if not bool(torch.all(nrm > 0)):
+    if not ops.ctx.status_value(torch.all(nrm > 0)):
         raise NumericalError("empty generalized Ritz subspace")
     V0 = V0 / nrm.to(V0.dtype)[None, :]
     tiny = 1e-14 * max(hnorm, 1e-300)
@@ -129,7 +131,8 @@ def ritz_extrema(ops, seeds=SEEDS, max_steps=512, check=16, tol=1e-10):
         _Lanczos(ops, V0, tiny).advance(steps, accumulate)
         theta = torch.from_numpy(np.concatenate(thetas)).to(V0.device)
         R = ops.apply_h_cov(X) - theta.to(X.dtype)[None, :] * ops.S.matmul(X)
-        res = (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(X, dim=0)).cpu().numpy()
+        res = ops.ctx.to_host(torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(X, dim=0),
+                              status=True)
         res = res.reshape(ns, 2)
         if np.all(np.isfinite(res)) and np.all(res <= tol * scale):
             return dict(extrema=np.array(thetas), residuals=res, steps=steps)

@@ -53,7 +53,7 @@ class DeviceStatistics:
     def check_flags(self):
         """Raises the reference's errors for nonfinite samples seen since the
         last clear (DOS first, as the reference checks it first)."""
-        f = int(self.flags.item())
+        f = int(self.ctx.status_value(self.flags[0]))
         if f & FLAG_DOS:
             raise NumericalError("nonfinite stochastic DOS")
         if f & FLAG_NUMERATOR:

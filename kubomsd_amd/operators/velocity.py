@@ -208,8 +208,7 @@ class OperatorBundle:
         import torch
         if isinstance(a, torch.Tensor):
             return a.to(device=self.ctx.torch_device, dtype=dtype or self.dtype).contiguous()
-        return torch.from_numpy(np.ascontiguousarray(a, np.complex128)).to(self.ctx.torch_device).to(
-            dtype or self.dtype).contiguous()
+        return self.ctx.upload(np.asarray(a, np.complex128), dtype or self.dtype).contiguous()
 
     def _check_velocity(self, axis, host_probe):
         """Hermiticity of the constructed velocity in FP64 (reference
@@ -231,7 +230,7 @@ class OperatorBundle:
                 exc.diagnostics = {**(getattr(exc, "diagnostics", None) or {}),
                                    "velocity_probe": dict(axis=axis, threshold=1e-10, attempts=attempts)}
                 raise
-            V = _gram(probe, v).cpu().numpy()
+            V = self.ctx.to_host(_gram(probe, v), status=True)
             error = float(np.linalg.norm(V - V.conj().T) / max(np.linalg.norm(V), 1e-300))
             attempts.append(dict(metric_rtol=rtol, metric_atol=atol, relative_error=error))
             if np.isfinite(error) and error <= 1e-10:

@@ -27,7 +27,7 @@ def drift_limit(cfg):
 
 def _drift(ops, B, norm0):
     norm = ops.inner(B, B).real
-    return float(((norm - norm0).abs() / norm0.clamp(min=np.finfo(float).tiny)).max())
+    return ops.ctx.status_value(((norm - norm0).abs() / norm0.clamp(min=np.finfo(float).tiny)).max())
 
 
 def propagate(states, dt_fs, ops, cfg):

@@ -98,10 +98,6 @@ def admitted_batch(ops, cfg, method):
     raise ResourceBlocked("RESOURCE_BLOCKED: one sample's device workspace exceeds the HBM budget")
 
 
-def _host(x):
-    return x.detach().cpu().numpy() if hasattr(x, "detach") else np.asarray(x)
-
-
 def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     """Runs the samples in global order, `batch` at a time, and adds each
     sample's DOS and numerators to its statistics block in sample order
@@ -152,7 +148,8 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
     nbk = len(counts)
     primary = "commutator" if method == "stepwise" else "vac_integral"
     import torch
-    dcounts = torch.from_numpy(counts[:, None].astype(float)).to(dos_blocks.device)
+    _host = ops.ctx.to_host
+    dcounts = ops.ctx.upload(counts[:, None].astype(float))
     rho, stderr_dos, stat = stats.estimate_ratio(dos_blocks, dcounts, counts)
     threshold = dos_floor(ops.n, cfg.spectral_bounds[1] - cfg.spectral_bounds[0], cfg)
     valid, reason = stats.dos_validity(rho, stderr_dos, threshold, nbk)
@@ -189,6 +186,7 @@ def _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method, diagnostics
 
 
 def _normalize_slab(raw, dos_blocks, counts, ns, valid, key, stats):
+    _host = stats.ctx.to_host
     raw = raw.contiguous()
     value, se, _, total = stats.estimate_ratio(raw, dos_blocks, counts, valid=valid, want_total=True)
     arrays = {key: _host(value), "numerator_" + key: _host(total) / ns,
@@ -231,7 +229,7 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
         valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
                                                          diagnostics, batch, stats, resolution.result())
     metadata["sample_seconds"] = seconds
-    valid_host = _host(valid)
+    valid_host = ops.ctx.to_host(valid)
     for start in range(0, len(cfg.time_fs), cfg.time_block):
         stop = min(start + cfg.time_block, len(cfg.time_fs))
         arrays = {}

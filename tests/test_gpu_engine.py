@@ -3,7 +3,9 @@
 Tolerances: complex128 blocks within 1e-11 relative to the largest entry
 (reduction order differs from the oracle's serial row order; the reference's
 own cpp-vs-reference bound is 2e-9..2e-11, tests/unit/test_native.py:128-143);
-complex64 storage within 5e-6 (FP32 rounding of every stored iterate).
+complex64 storage within 5e-7 (FP32 rounding of the input block and of every
+stored iterate; measured 3e-8 .. 7e-8 for products, chains and moments, and
+3e-7 for the 14-term augmented series, whose tolerance is ten times larger).
 """
 import numpy as np
 import pytest
@@ -13,7 +15,7 @@ from kubomsd_amd.synthetic import honeycomb_lattice, square_lattice
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"c128": 1e-11, "c64": 5e-6}
+TOL = {"c128": 1e-11, "c64": 5e-7}
 
 
 @pytest.fixture(scope="module")
@@ -64,7 +66,7 @@ def test_moments_honeycomb_long_chain(hip_ctx, honey, dtype):
     for chunk in (16, 7):
         hip_ctx.set_tuning("moment_chunk", chunk)
         got = host(he.moments(to_dev(hip_ctx, B, dtype), 300, c, a))
-        assert rel_err(got, want) < (1e-10 if dtype == "c128" else 2e-5)
+        assert rel_err(got, want) < (1e-10 if dtype == "c128" else TOL["c64"])
     hip_ctx.set_tuning("moment_chunk", 16)
 
 
@@ -134,7 +136,7 @@ def test_moment_doubling_matches_direct_oracle(hip_ctx, honey, dtype, L):
     want = oe.moments(B, L, 0.05, 8.8)
     got = host(hip_engine(hip_ctx, m).moments(to_dev(hip_ctx, B, dtype), L, 0.05, 8.8, doubling=True))
     assert got.shape == want.shape
-    assert rel_err(got, want) < (1e-11 if dtype == "c128" else 2e-5)
+    assert rel_err(got, want) < TOL[dtype]
 
 
 @pytest.mark.parametrize("nb", [1, 2, 3, 5])

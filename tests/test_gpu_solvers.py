@@ -1,9 +1,11 @@
 """Device stepwise / spectral / stochastic orchestration against the FP64
 oracle orchestration (oracle/kpm.py) on identical models and samples.
 
-Tolerances: fp64 runs within 1e-9 of the largest entry (the reference's own
-cpp-vs-reference stepwise bound is 2e-9, tests/unit/test_native.py:139-143);
-mixed (complex64 storage) within 2e-5.
+Tolerances (max entry error over max entry): fp64 within 1e-9 (the
+reference's own cpp-vs-reference stepwise bound is 2e-9,
+tests/unit/test_native.py:139-143; measured about 1e-15); mixed (complex64
+chains, complex128 propagated states, FP64 reductions) within 2e-7 for DOS and
+3e-7 for the MSD and VAC numerators (measured 2-3e-8 on these models).
 """
 import numpy as np
 import pytest
@@ -14,7 +16,8 @@ from oracle import kpm
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-9, "mixed": 2e-5}
+TOL = {"fp64": 1e-9, "mixed": 2e-7}
+NTOL = {"fp64": 1e-9, "mixed": 3e-7}
 
 
 def run_config(**kw):
@@ -56,7 +59,7 @@ def test_stepwise_nonorthogonal_matches_oracle(hip_ctx, precision):
     kcfg = kpm.config(energy_eV=E, time_fs=cfg.time_fs, spectral_bounds=bounds, n_moments=40, segment_fs=.3)
     edos, enums = kpm.stepwise_samples(kpm.Ops(H, S, {"x": X}, rtol=1e-13), kcfg, B)
     close(dos, edos, TOL[precision])
-    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], TOL[precision] * 10)
+    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], NTOL[precision])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -73,7 +76,7 @@ def test_stepwise_lattice_two_axes_matches_oracle(hip_ctx, precision):
                       axes_vectors=np.eye(3)[:2])
     edos, enums = kpm.stepwise_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
     close(dos, edos, TOL[precision])
-    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], TOL[precision] * 10)
+    close(nums["msd_A2/commutator"], enums["msd_A2/commutator"], NTOL[precision])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -91,7 +94,7 @@ def test_spectral_matches_oracle(hip_ctx, precision):
     edos, enums = kpm.spectral_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
     close(dos, edos, TOL[precision])
     for key in enums:
-        close(nums[key], enums[key], TOL[precision] * 10)
+        close(nums[key], enums[key], NTOL[precision])
 
 
 def test_stochastic_blocks_batch_invariance_and_oracle_sums(hip_ctx):
