@@ -47,7 +47,7 @@ def test_mixed_precision_diffusion_vs_fp64():
         a, b = mixed[jid].diffusion, fp64[jid].diffusion
         assert mixed[jid].error is None and fp64[jid].error is None
         err = np.max(np.abs(a - b)) / np.max(np.abs(b))
-        assert err < 1e-4, (jid, err)
+        assert err < 2e-7, (jid, err)  # measured 5.5e-9 .. 1.6e-8 on this trace
 
 
 def test_budget_admission_rejects_oversized_job():
