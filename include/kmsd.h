@@ -280,6 +280,20 @@ int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t n
 int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
                           double half);
 
+/* No reference counterpart: kmsd_moments for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid allocation).  Same
+ * arguments and bitwise the same moments as kmsd_moments; needs an identity
+ * metric (S = I with the diagonal preconditioner; else KMSD_EINVAL).  The two
+ * chain blocks (n*nb elements each) and the partial sums live in a fixed
+ * 256 MiB per-device array that the library holds from the first context
+ * created on a device until the process ends; a block beyond it is rejected
+ * with KMSD_EINVAL ("fixed scratch").  Allocates no device memory and charges
+ * nothing to the budget.  Every call on one GPU shares that array, and the
+ * call synchronizes before it returns: calls that overlap on one GPU (from
+ * several processes' graphs or streams replaying it) race. */
+int kmsd_moments_noalloc(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
+                         double center, double half);
+
 /* ---- spectral double moments -------------------------------------------- */
 /* Reference: einsum('jnb,knb->bjk', left.conj(), cov) (src/kubo_msd/solvers/
  * spectral.py:37; GPU edition solvers/spectral.py:65).  left: (m, N, nb),

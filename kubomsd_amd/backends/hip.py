@@ -71,6 +71,7 @@ SIGNATURES = {
     "kmsd_chain": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments_doubling": [_P, _P, _P, _I64, _I64, _I, _D, _D],
+    "kmsd_moments_noalloc": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_mu_table": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I],
     "kmsd_block_accumulate": [_P, _P, _P, _I64, _I64, _I64, _P, C.c_int32],
     "kmsd_time_gradient": [_P, _P, _P, _I64, _I64, _I64, _P, _D],

@@ -7,7 +7,7 @@ OUT="$ROOT/kubomsd_amd/lib"
 mkdir -p "$OUT" "$HERE/_build"
 FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -I"$ROOT/include"
        -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-command-line-argument)
-SRCS=(kmsd_launch kmsd_api kmsd_metric kmsd_precond kmsd_pcg kmsd_engine kmsd_mu kmsd_mu_mfma kmsd_stats)
+SRCS=(kmsd_launch kmsd_api kmsd_metric kmsd_precond kmsd_pcg kmsd_engine kmsd_mu kmsd_mu_mfma kmsd_stats kmsd_noalloc)
 pids=()
 for s in "${SRCS[@]}"; do
   if [ -f "$HERE/$s.hip" ]; then

@@ -546,11 +546,30 @@ extern "C" int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, vo
     });
 }
 
-extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
-                            double center, double half) {
-    if (!e) return KMSD_EINVAL;
+namespace kmsd {
+
+MomentScratch moment_scratch_need(kmsd_engine *e, const void *B, int64_t L, int64_t nb, int dtype,
+                                  int64_t chunk_max) {
     kmsd_ctx *c = e->ctx;
-    return guard(c, [&] {
+    MomentScratch need;
+    const int64_t n = n_of(e);
+    need.block_bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
+    // columns-per-lane ceiling of the chain: B's alignment and the width (the
+    // scratch blocks are 256-byte aligned, so only their width counts)
+    const int cap = cpl_cap_for(dtype, {{B, nb}});
+    need.groups = step_groups(c, n, int32_t(nb), dtype, cap, e->metric->identity ? e->H : nullptr);
+    need.chunk = std::min<int64_t>(std::min<int64_t>(c->moment_chunk, chunk_max), L);
+    need.part_bytes = sizeof(double) * size_t(need.groups) * size_t(nb) * 2 * size_t(need.chunk);
+    return need;
+}
+
+// The moment chain of kmsd_moments.  Buffers: the engine's own grow-only
+// blocks (sc == null), or the caller's scratch (two chain blocks and the
+// partial sums at the given addresses; identity metric only, eager launches).
+void moment_chain(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
+                  double half, const MomentScratch *sc) {
+    kmsd_ctx *c = e->ctx;
+    {
         check_block_args(B, mu, nb, dtype);
         if (L < 1) throw Error(KMSD_EINVAL, "moment count must be positive");
         const int64_t n = n_of(e);
@@ -559,17 +578,21 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
         const size_t bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
         const void *W = B;
         if (!e->metric->identity) {
+            if (sc) throw Error(KMSD_EINVAL, "caller scratch needs an identity metric");
             void *sb = e->wbuf.ensure(c, bytes);
             spmm(c, dtype, e->metric->S, B, nb, sb, nb, w, 1.0, 0.0, false);
             W = sb;
         }
-        void *t1 = e->t0.ensure(c, bytes);
-        void *spare = e->t1.ensure(c, bytes);
+        void *t1 = sc ? sc->t1 : e->t0.ensure(c, bytes);
+        void *spare = sc ? sc->t2 : e->t1.ensure(c, bytes);
         const int cap = cpl_cap_for(dtype, {{B, nb}, {W, nb}, {t1, nb}, {spare, nb}});
         const int64_t G = step_groups(c, n, w, dtype, cap, e->metric->identity ? e->H : nullptr);
-        const int64_t chunk = std::min<int64_t>(c->moment_chunk, L);
+        const int64_t chunk = sc ? sc->chunk : std::min<int64_t>(c->moment_chunk, L);
         const size_t slot_doubles = size_t(G) * size_t(w) * 2;
-        double *part = static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
+        if (sc && (G != sc->groups || sizeof(double) * slot_doubles * size_t(chunk) > sc->part_bytes))
+            throw Error(KMSD_EDEVICE, "internal error: moment scratch layout");
+        double *part = sc ? sc->part
+                          : static_cast<double *>(e->part.ensure(c, sizeof(double) * slot_doubles * size_t(chunk)));
         int64_t slot = 0, base = 0;
         auto flush = [&] {
             launch_reduce(c, part, G, w, slot, mu + base * nb, nb, 0);
@@ -639,7 +662,9 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
             };
             // Graph replay of whole chunks: identity metric (one kernel per
             // order, no host round trips), even chunk length, chunk-aligned.
-            const bool graphs = c->graphs && e->metric->identity && chunk % 2 == 0 && chunk >= 2;
+            // A caller's scratch runs eager launches (instantiating a graph
+            // allocates).
+            const bool graphs = !sc && c->graphs && e->metric->identity && chunk % 2 == 0 && chunk >= 2;
             ChainGraph cg;
             const void *g_prev = nullptr;
             const void *g_cur = nullptr;
@@ -686,7 +711,15 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
         }
         flush();
         HIPCHECK(hipStreamSynchronize(c->stream));
-    });
+    }
+}
+
+}  // namespace kmsd
+
+extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
+                            double center, double half) {
+    if (!e) return KMSD_EINVAL;
+    return guard(e->ctx, [&] { moment_chain(e, B, mu, L, nb, dtype, center, half, nullptr); });
 }
 
 extern "C" int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,

@@ -313,6 +313,25 @@ void metric_release_ic(kmsd_metric *m);
 void ic_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags);
 // Blocks larger than SMALL_BLOCK rows (the small ones are the PCG's own kernel).
 void large_blocks_apply(kmsd_metric *m, int dtype, const void *R, void *Z, int32_t w, const void *flags);
+// Buffers of one moment chain (kmsd_moments): two chain blocks of
+// block_bytes, partial sums of `groups` workgroups for `chunk` orders.
+struct MomentScratch {
+    void *t1 = nullptr, *t2 = nullptr;
+    double *part = nullptr;
+    size_t block_bytes = 0, part_bytes = 0;
+    int64_t groups = 0, chunk = 0;
+};
+// Sizes a moment chain of L orders on nb columns needs, with at most
+// chunk_max orders per partial reduction.
+MomentScratch moment_scratch_need(kmsd_engine *e, const void *B, int64_t L, int64_t nb, int dtype,
+                                  int64_t chunk_max);
+// kmsd_moments' chain; sc: caller-provided buffers (identity metric only,
+// eager launches) or null for the engine's own.  Call inside guard().
+void moment_chain(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
+                  double half, const MomentScratch *sc);
+// Device address of kmsd_moments_noalloc's fixed scratch on `device`, looked
+// up once per device (kmsd_noalloc.hip); kmsd_ctx_create calls it.
+void noalloc_setup(int device);
 // Metric solve on device blocks (any dtype); throws Error(KMSD_ENUMERIC) on
 // failure with the report stored in the metric.
 void metric_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb,
