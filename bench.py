@@ -290,13 +290,18 @@ def double_moments(args):
     W = 1.5 eV; per random vector r: A = [T_n(H~) V r] and B = [V T_m(H~) r]
     for n, m < M = 1024 (complex64 chains, (M, N) each), then the table
     mu_nm = sum_i conj(A_ni) B_mi on the matrix cores, FP32 MFMA and the bf16x3
-    split variant.  Algorithmic flops per table 8 N M^2 (complex MAC = 8 real
-    flops).  Errors against FP64: the same GEMM in complex128 on the same
-    inputs (all entries), and the full FP64 pipeline (complex128 chains and
-    GEMM) for the columns m < 64 of the first vector."""
+    split variant, through the package API
+    (kubomsd_amd.solvers.velocity_correlation).  Algorithmic flops per table
+    8 N M^2 (complex MAC = 8 real flops).  Errors against FP64: the FP64
+    table of the same complex64 chains (all entries, every vector), and the
+    full FP64 pipeline (complex128 chains and table) for n, m < 64 of the
+    first vector."""
     import torch
-    from kubomsd_amd.backends.hip import (HipContext, HipEngine, HipMetric, KMSD_MU_BF16X3, KMSD_MU_FP32,
-                                          mu_table)
+    from kubomsd_amd.backends.hip import HipContext
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.velocity_correlation import velocity_correlation_moments
     from kubomsd_amd.synthetic import square_lattice
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
@@ -304,64 +309,73 @@ def double_moments(args):
     model = square_lattice(args.side, t=-1.0, W=1.5)
     n, M = model.n, args.moments
     bound = 4.0 + 0.75 + 0.05  # Gershgorin |E| <= 4|t| + W/2, plus margin
-    c, a = 0.0, bound
-    metric = HipMetric(ctx, model.S, model.S.diagonal().real)
-    V = model.velocity["x"]
-    eng = HipEngine(ctx, model.H, metric, [(0, [V]), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
-    from kubomsd_amd.backends.hip import HipMatrix
-    Vd = HipMatrix(ctx, V)
+    bundle = ModelBundle(model.H, model.S, velocity={"x": model.velocity["x"]})
+
+    def config(precision, moments):
+        return RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", energy_eV=[0.0], time_fs=[0.0],
+                         axes=("x",), spectral_bounds=(-bound, bound), n_moments=moments, precision=precision,
+                         autotune=False)
+    cfg = config("mixed", M)
+    ops = build_operators(bundle, cfg, ctx)
     flops = 8.0 * n * M * M
     print(f"# setup {time.perf_counter() - t_build:.1f}s: N={n} nnz={model.H.nnz} M={M} samples={args.samples}",
           file=sys.stderr, flush=True)
-    stream = torch.cuda.current_stream()
-    ev = lambda: torch.cuda.Event(enable_timing=True)
-    times = {"chain_ms": 0.0, "velocity_chain_ms": 0.0, "fp32_ms": 0.0, "bf16x3_ms": 0.0}
-    err = None
+    totals = {}
+    errors = {"fp32": [], "bf16x3": []}
     t_all = time.perf_counter()
     for sid in range(args.samples):
-        r = torch.from_numpy(random_phases(n, SEED, [sid])).to(torch.complex64).to(ctx.torch_device)
-        e = [ev() for _ in range(7)]
-        e[0].record(stream)
-        A = eng.chain(eng.v_cov(r, (1.0, 0.0, 0.0)), M, c, a)
-        e[1].record(stream)
-        Bm = eng.chain_v(r, M, c, a, (1.0, 0.0, 0.0))  # V T_m r, one velocity product for all m
-        e[2].record(stream)
-        mu32 = mu_table(ctx, A, Bm, KMSD_MU_FP32)
-        e[3].record(stream)
-        mub = mu_table(ctx, A, Bm, KMSD_MU_BF16X3)
-        e[4].record(stream)
-        torch.cuda.synchronize()
-        times["chain_ms"] += e[0].elapsed_time(e[1])
-        times["velocity_chain_ms"] += e[1].elapsed_time(e[2])
-        times["fp32_ms"] += e[2].elapsed_time(e[3])
-        times["bf16x3_ms"] += e[3].elapsed_time(e[4])
+        r = random_phases(n, SEED, [sid])
+        out = velocity_correlation_moments(ops, cfg, r, "x", mode=("fp32", "bf16x3"),
+                                           reference=not args.no_fp64_check)
+        for key, v in out["ms"].items():
+            totals[key] = totals.get(key, 0.0) + v
+        if out["error_vs_fp64"]:
+            for md in errors:
+                errors[md].extend(out["error_vs_fp64"][md])
         if sid == 0 and not args.no_fp64_check:
-            err = mu_errors(ctx, eng, Vd, r, A, Bm, mu32[0], mub[0], M, c, a)
-        del A, Bm
-        print(f"# sample {sid}: fp32 {e[2].elapsed_time(e[3]):.1f} ms, bf16x3 {e[3].elapsed_time(e[4]):.1f} ms",
-              file=sys.stderr, flush=True)
+            # full FP64 pipeline for n, m < 64 (chains are prefix-consistent)
+            k = 64
+            cfg64 = config("fp64", k)
+            ops64 = build_operators(bundle, cfg64, ctx)
+            full = velocity_correlation_moments(ops64, cfg64, r, "x", mode="fp64")["mu"][0]
+            del ops64
+            s2 = full.abs().max().item()
+            pipe = {md: (out["mu"][md][0, :k, :k] - full).abs().max().item() / s2 for md in errors}
+        del out
+        print(f"# sample {sid}: fp32 {totals['table_fp32']:.1f} ms, bf16x3 {totals['table_bf16x3']:.1f} ms "
+              f"(cumulative)", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t_all
     ns = args.samples
-    tf32 = flops * ns / (times["fp32_ms"] / 1e3) / 1e12
-    tfb = flops * ns / (times["bf16x3_ms"] / 1e3) / 1e12
+    tf32 = flops * ns / (totals["table_fp32"] / 1e3) / 1e12
+    tfb = flops * ns / (totals["table_bf16x3"] / 1e3) / 1e12
+    err = None
+    if not args.no_fp64_check:
+        err = {"gemm_fp32_max_rel": max(errors["fp32"]), "gemm_bf16x3_max_rel": max(errors["bf16x3"]),
+               "pipeline_fp32_max_rel_64x64": pipe["fp32"], "pipeline_bf16x3_max_rel_64x64": pipe["bf16x3"],
+               "reference": "FP64 table (FP64 matrix cores) of the same complex64 chains, every vector; complex128 "
+                            "chains and table for n, m < 64 of the first vector"}
     line = {
         "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
         "value": tf32, "unit": "TFLOP/s (mu_nm table, FP32 MFMA)", "n_gpus": 1, "steps": ns, "warmup": 0,
-        "ms_per_step": times["fp32_ms"] / ns, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 (FP32 MFMA, FP64 split-sum)", "data": "synthetic (seeded square lattice, NumPy PCG64 phases)",
+        "ms_per_step": totals["table_fp32"] / ns, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (FP32 MFMA, FP64 split-sum)",
+        "data": "synthetic (seeded square lattice, NumPy PCG64 phases)",
         "config": {"workload": f"square {args.side}^2 (N={n}), t=-1 eV, W=1.5 eV, {ns} random vectors, "
-                               f"mu_nm {M}x{M} complex per vector", "n": n, "moments": M, "samples": ns,
-                   "parallelism": "single GPU"},
+                               f"mu_nm {M}x{M} complex per vector (package API velocity_correlation_moments)",
+                   "n": n, "moments": M, "samples": ns, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": tf32, "peak": MFMA_PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                      "frac": tf32 / MFMA_PEAK_TFLOPS["fp32"], "traffic": None,
                      "algorithmic_flops_per_launch": flops, "kernel": "mu_f32_kernel (+ split sum)"},
         "bf16x3": {"achieved_TFLOPs": tfb, "mfma_executed_TFLOPs": 3 * tfb,
-                   "frac_of_bf16_peak": 3 * tfb / MFMA_PEAK_TFLOPS["bf16"], "ms_per_table": times["bf16x3_ms"] / ns},
-        "pipeline_ms_per_vector": {k: v / ns for k, v in times.items()},
+                   "frac_of_bf16_peak": 3 * tfb / MFMA_PEAK_TFLOPS["bf16"],
+                   "ms_per_table": totals["table_bf16x3"] / ns},
+        "pipeline_ms_per_vector": {k: v / ns for k, v in totals.items()},
         "wall_s": wall,
         "err_vs_fp64": err,
         "cpu_baseline": None,
     }
+    if not args.no_fp64_check:
+        line["fp64_table_TFLOPs"] = flops * ns / (totals["table_fp64_reference"] / 1e3) / 1e12
     print(json.dumps(line))
 
 
@@ -583,39 +597,6 @@ def square(args):
     }
     line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line))
-
-
-def mu_errors(ctx, eng, Vd, r, A, Bm, mu32, mub, M, c, a):
-    """Max-entry relative errors of the FP32 and bf16x3 tables against FP64:
-    (1) the complex128 GEMM of the same complex64 chains, over N in chunks;
-    (2) the complex128 pipeline (chains, velocity, GEMM) for columns m < 64."""
-    import torch
-    n = A.shape[1]
-    A2, B2 = A.view(M, n), Bm.view(M, n)
-    ref = torch.zeros((M, M), dtype=torch.complex128, device=A.device)
-    step = 1 << 18
-    for i0 in range(0, n, step):
-        a64 = A2[:, i0:i0 + step].to(torch.complex128)
-        b64 = B2[:, i0:i0 + step].to(torch.complex128)
-        ref += a64.conj() @ b64.transpose(0, 1).contiguous()
-        del a64, b64
-    scale = ref.abs().max().item()
-    out = {"gemm_fp32_max_rel": (mu32 - ref).abs().max().item() / scale,
-           "gemm_bf16x3_max_rel": (mub - ref).abs().max().item() / scale}
-    k = 64
-    r64 = r.to(torch.complex128)
-    A64 = eng.chain(eng.v_cov(r64, (1.0, 0.0, 0.0)), M, c, a).view(M, n)
-    C64 = eng.chain(r64, k, c, a).view(k, n)
-    B64 = Vd.matmul(C64.t().contiguous()).t().contiguous()
-    del C64
-    full = torch.zeros((M, k), dtype=torch.complex128, device=A.device)
-    for i0 in range(0, n, step):
-        full += A64[:, i0:i0 + step].conj().resolve_conj() @ B64[:, i0:i0 + step].transpose(0, 1).contiguous()
-    del A64, B64
-    s2 = full.abs().max().item()
-    out["pipeline_fp32_max_rel_cols64"] = (mu32[:, :k] - full).abs().max().item() / s2
-    out["pipeline_bf16x3_max_rel_cols64"] = (mub[:, :k] - full).abs().max().item() / s2
-    return out
 
 
 def plain_gpu_fp64(H, Bh, c, a, orders, device):

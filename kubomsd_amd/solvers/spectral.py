@@ -38,14 +38,10 @@ def _max_order(cfg):
 
 
 def _table(ops, left, cov):
+    """(nb, L+K, K+1) complex128 tables on the FP64 matrix cores (both
+    precisions: complex64 chains are widened exactly)."""
     from ..backends.hip import KMSD_MU_FP64, mu_table
-    nb = left.shape[2]
-    if nb <= 64:
-        return mu_table(ops.ctx, left, cov, KMSD_MU_FP64)
-    import torch
-    parts = [mu_table(ops.ctx, left[:, :, s:s + 64].contiguous(), cov[:, :, s:s + 64].contiguous(), KMSD_MU_FP64)
-             for s in range(0, nb, 64)]
-    return torch.cat(parts, 0)
+    return mu_table(ops.ctx, left, cov, KMSD_MU_FP64)
 
 
 def spectral_samples(ops, cfg, samples):

@@ -1,5 +1,8 @@
 """Double-moment tables table[b,j,q] = sum_i conj(L[j,i,b]) R[q,i,b] against an
-FP64 einsum (the reference's own contraction, src/kubo_msd/solvers/spectral.py:37)."""
+FP64 einsum (the reference's own contraction, src/kubo_msd/solvers/spectral.py:37).
+The FP64 tables (FP64 matrix cores) cover several row and column tiles, batch
+widths that are and are not multiples of four, and N ranges split over
+workgroups; the FP32 and bf16x3 square tables take one batch column."""
 import numpy as np
 import pytest
 
@@ -20,7 +23,8 @@ def dev3(ctx, a, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("shape", [(37, 11, 500, 3), (5, 70, 129, 16), (16, 1, 33, 1)])
+@pytest.mark.parametrize("shape", [(37, 11, 500, 3), (5, 70, 129, 16), (16, 1, 33, 1), (300, 43, 2000, 64),
+                                   (33, 130, 17, 8), (70, 17, 5000, 6)])
 def test_fp64_table_matches_einsum(hip_ctx, dtype, shape):
     from kubomsd_amd.backends.hip import KMSD_MU_FP64, mu_table
     m, k, n, nb = shape
