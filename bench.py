@@ -254,7 +254,9 @@ def nonorthogonal(args):
         del B64
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(model, Bh[:, :8], c, a, args.cpu_seconds, blocks=blocks)
+        # 64 orders of this chain take minutes on the host (each order is a
+        # PCG solve): a slice of about --cpu-seconds, 2 vectors
+        cpu = cpu_baseline(model, Bh[:, :2], c, a, args.cpu_seconds, blocks=blocks)
     line = {
         "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
         "value": value, "unit": "vector-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -686,9 +688,11 @@ def cpu_threads():
     return max(1, min(aff, int(env))) if env else aff
 
 
-def cpu_baseline(model, B, c, a, target_s, blocks=()):
+def cpu_baseline(model, B, c, a, target_s, blocks=(), orders=None):
     """Times the first orders of the FP64 DOS moment chain on the host cores
-    (median of 3 after a warm-up, as benchmarks/benchmark_native.py:68-75)."""
+    (median of 3 after a warm-up, as benchmarks/benchmark_native.py:68-75):
+    exactly `orders` orders when given (the stated 64-order slice), else as
+    many as take about target_s seconds."""
     threads = cpu_threads()
     os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
     from oracle import native as O
@@ -711,7 +715,8 @@ def cpu_baseline(model, B, c, a, target_s, blocks=()):
     t0 = time.perf_counter()
     eng.moments(HB, 2, c, a)  # warm-up: one H action (SpMM + metric PCG)
     one = time.perf_counter() - t0
-    orders = int(np.clip(target_s / max(one, 1e-3), 3, 256))
+    if orders is None:
+        orders = int(np.clip(target_s / max(one, 1e-3), 3, 256))
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -859,8 +864,9 @@ def main():
             plain = {"error": str(exc)[:200], "fused_fp64_moments_per_s": fused64}
     cpu = None
     if not args.no_cpu_baseline:
-        cb_nb = 8
-        cpu = cpu_baseline(model, Bh[:, :cb_nb], c, a, args.cpu_seconds)
+        # the stated slice: the first 64 orders of the DOS chain, on 2 of the
+        # 64 vectors (about 10-15 s per run on 16 host cores)
+        cpu = cpu_baseline(model, Bh[:, :2], c, a, args.cpu_seconds, orders=64)
 
     line = {
         "metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",

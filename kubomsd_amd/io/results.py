@@ -86,7 +86,8 @@ class ResultsWriter:
         for key, value in block.arrays.items():
             a = np.asarray(value)
             axis = 2 if key.startswith("numerator_block_sums/") else 1
-            if a.ndim != (4 if axis == 2 else 3) or a.shape[axis] != len(t):
+            if (a.ndim != (4 if axis == 2 else 3) or a.shape[axis] != len(t) or a.shape[-1] != len(self.cfg.energy_eV)
+                    or a.shape[axis - 1] != len(self.cfg.axes)):
                 raise KuboMSDError(f"invalid shape for {key}: {a.shape}")
             if key not in self.arrays:
                 shape = list(a.shape)
@@ -98,6 +99,8 @@ class ResultsWriter:
         if self.next and (block.static or block.metadata):
             raise KuboMSDError("static fields/metadata are allowed only in first block")
         for key, value in block.static.items():
+            if key in self.arrays:
+                raise KuboMSDError("duplicate static field " + key)
             self.arrays[key] = np.asarray(value)
         self.metadata.update(json_safe(block.metadata))
         self.keys = keys
@@ -105,11 +108,39 @@ class ResultsWriter:
         self.metadata["written_time_points"] = self.next
         self._save_meta()
 
+    def _derive_gradient(self):
+        """diffusion_A2_per_fs/msd_gradient = 1/2 d/dt of the primary MSD on the
+        output times when the estimator did not supply it (with its numerator
+        and masks), as the reference's writer does (io/results.py:56-72);
+        numpy.gradient over the whole time axis equals the reference's
+        per-time-block evaluation with one-point halos."""
+        primary = self.metadata["primary_estimator"]
+        key = "diffusion_A2_per_fs/msd_gradient"
+        nt = len(self.cfg.time_fs)
+        if nt < 2 or key in self.arrays or "msd_A2/" + primary not in self.arrays:
+            return
+        edge = 2 if nt > 2 else 1
+        t = np.asarray(self.cfg.time_fs)
+        self.arrays[key] = .5 * np.gradient(self.arrays["msd_A2/" + primary], t, axis=1, edge_order=edge)
+        num = self.arrays.get("numerator_msd_A2/" + primary)
+        if num is not None:
+            self.arrays["numerator_diffusion_A2_per_fs/msd_gradient"] = .5 * np.gradient(num, t, axis=1,
+                                                                                        edge_order=edge)
+        for prefix in ("valid/", "reason_code/"):
+            if prefix + primary in self.arrays:
+                self.arrays[prefix + "msd_gradient"] = self.arrays[prefix + primary].copy()
+        self.metadata["msd_gradient"] = {"source": primary, "edge_order": edge, "time_array": "actual output time_fs"}
+
     def finalize(self, summary):
-        if self.done or self.next != len(self.cfg.time_fs):
+        status = getattr(summary, "run_status", "complete")
+        if self.done or self.next != len(self.cfg.time_fs) or status != "complete":
             raise KuboMSDError("cannot mark incomplete computation complete")
         if "dos_states_per_eV" not in self.arrays or "primary_estimator" not in self.metadata:
             raise KuboMSDError("required result metadata missing")
+        total = sum(np.asarray(a).nbytes for a in self.arrays.values())
+        if total > self.cfg.memory_limit_gb * 1024 ** 3 * .5:
+            raise KuboMSDError("results exceed half the RAM budget (memory_limit_gb) for the NPZ container")
+        self._derive_gradient()
         primary = self.metadata["primary_estimator"]
         dp = self.metadata.get("primary_diffusion_estimator", primary)
         aliases = {}
@@ -124,6 +155,8 @@ class ResultsWriter:
                     aliases[f"{quantity}_{axis}_{unit}"] = key
             self.arrays[f"{quantity}_direction_average_{unit}"] = np.mean(data, axis=0)
         self.metadata["legacy_aliases"] = aliases
+        self.metadata["direction_average"] = ("arithmetic mean over declared orthonormal directions; no volume "
+                                              "normalization")
         self.metadata["summary"] = json_safe(summary)
         self.metadata["run_status"] = "complete"
         arrays = dict(self.arrays)

@@ -75,16 +75,20 @@ def bessel_coefficients(dt, cfg, weighted_tail=False):
 
 
 class Ops:
-    """Oracle operator bundle: engine + metric + explicit velocities."""
+    """Oracle operator bundle: engine + metric + explicit velocities (kind 0)
+    or, for the axes in `position`, position matrices (kind 1: the velocity is
+    the commutator (i/hbar)(H S^-1 X - X S^-1 H), native.cpp:506-533)."""
 
-    def __init__(self, H, S, velocity, rtol=1e-12, threads=1, blocks=()):
+    def __init__(self, H, S, velocity, rtol=1e-12, threads=1, blocks=(), position=None):
         import scipy.sparse as sp
         self.host_H = sp.csr_matrix(H, dtype=complex)
         self.host_S = sp.csr_matrix(S, dtype=complex)
         self.n = self.host_H.shape[0]
         self.S = O.Csr(self.host_S, threads)
         self.metric = O.Metric(self.S, self.host_S.diagonal().real, blocks, threads)
-        defs = [(0, [O.Csr(velocity[a], threads)]) if a in velocity else (-1, []) for a in "xyz"]
+        position = position or {}
+        defs = [(0, [O.Csr(velocity[a], threads)]) if a in velocity else
+                (1, [O.Csr(position[a], threads)]) if a in position else (-1, []) for a in "xyz"]
         self.native = O.Engine(O.Csr(self.host_H, threads), self.metric, defs, rtol, 0., 1000)
         self.rtol = rtol
 

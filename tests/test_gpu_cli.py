@@ -42,3 +42,43 @@ def test_cli_run_matches_library(hip_ctx, tmp_path, example, precision):
     assert np.isfinite(res["dos_states_per_eV"]).all()
     assert res["msd_x_A2"].shape == (len(cfg.time_fs), 5)
     assert np.all(res["numerator_msd_A2/commutator"][:, 0] == 0)  # MSD(t=0) = 0
+
+
+@pytest.mark.parametrize("example, precision, tol", [("nonorthogonal", "fp64", 1e-9), ("two_level", "fp64", 1e-9),
+                                                     ("nonorthogonal", "mixed", 3e-7)])
+def test_cli_run_matches_oracle(hip_ctx, tmp_path, example, precision, tol):
+    """The CLI's block sums equal the FP64 oracle's on the same samples: the
+    reference's phases (seed, ids 0..2), the same metric sampling polynomial
+    applied by the oracle's Clenshaw sampler (bit-identical to the reference
+    kernel), the oracle's stepwise estimator, and sums per statistics block."""
+    import scipy.sparse as sp
+    from kubomsd_amd.cli import config_from_args, main, parser
+    from kubomsd_amd.io.model import load_model
+    from kubomsd_amd.io.results import read_results
+    from kubomsd_amd.metric.bounds import spd_bounds
+    from kubomsd_amd.metric.sampling import inverse_sqrt_series, random_phases
+    from oracle import kpm
+    inp = EXAMPLES / example / "model.json"
+    out = tmp_path / "run"
+    argv = ["run", "--input", str(inp), "--out", str(out), "--precision", precision] + ARGS
+    assert main(argv) == 0
+    res = read_results(out)["arrays"]
+    cfg = config_from_args(parser().parse_args(argv))
+    model = load_model(inp)
+    S = sp.csr_matrix(model.S)
+    ops = kpm.Ops(model.H_cov, S, model.velocity, rtol=1e-13, position=model.position)
+    z = 1 / np.sqrt(S.diagonal().real)
+    lo, hi, _ = spd_bounds(sp.diags(z) @ S @ sp.diags(z))
+    center, half = (hi + lo) / 2, (hi - lo) / 2
+    coeff, _ = inverse_sqrt_series(center, half, cfg.sampling_tol, cfg.sampling_max_order)
+    B = ops.metric.sample(random_phases(S.shape[0], cfg.seed, range(cfg.nsamples)), coeff, center, half)
+    kcfg = kpm.config(energy_eV=cfg.energy_eV, time_fs=cfg.time_fs, spectral_bounds=cfg.spectral_bounds,
+                      n_moments=cfg.n_moments, segment_fs=cfg.segment_fs, axes_vectors=cfg.axes_vectors)
+    dos, nums = kpm.stepwise_samples(ops, kcfg, B)
+    blocks = [slice(0, 2), slice(2, 3)]
+    want_dos = np.array([dos[s].sum(0) for s in blocks])
+    want = np.array([nums["msd_A2/commutator"][s].sum(0) for s in blocks])
+    got = res["numerator_block_sums/msd_A2/commutator"]
+    assert got.shape == want.shape
+    assert np.max(np.abs(res["dos_block_sums"] - want_dos)) <= tol * np.max(np.abs(want_dos))
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
