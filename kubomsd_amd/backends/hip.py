@@ -190,7 +190,7 @@ class HipContext:
             err = NumericalError("HIP device error: " + msg)
         else:
             err = NumericalError(msg)
-        if diagnostics is not None:
+        if diagnostics is not None and rc == KMSD_ENUMERIC:  # per-column reports of failed solves only
             err.diagnostics = diagnostics
         raise err
 
