@@ -271,6 +271,20 @@ int kmsd_chain_v(kmsd_engine *e, const double *dir, const void *B, void *out, in
 int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype, double center,
                  double half);
 
+/* No reference counterpart: kmsd_moments for `count` engines of one context
+ * at once (continuous batching of small transport jobs): each engine e[q]
+ * with an identity metric, its block B[q] (n_q x nb, dtype), its output
+ * mu[q] (DEVICE f64 (L, nb)) and its own scale (center[q], half[q]; host
+ * arrays of count entries).  One width nb, length L and dtype for all.
+ * Orders 0-2 run chain by chain; from order 3 on, every chain's step of one
+ * order is one launch shared by all (each chain keeps its own matrix format
+ * and the workgroup count, partial sums and reductions of its own
+ * kmsd_moments call, so its moments are the same bits); chains whose
+ * kernels cannot share a launch (4x4-block or complex matrices) run one
+ * launch each, with the same result. */
+int kmsd_moments_group(int64_t count, kmsd_engine *const *e, const void *const *B, double *const *mu, int64_t L,
+                       int64_t nb, int dtype, const double *center, const double *half);
+
 /* No reference counterpart: the same moments as kmsd_moments (S = I only),
  * from the Chebyshev doubling identities T_2k = 2 T_k^2 - 1 and
  * T_2k+1 = 2 T_k T_k+1 - T_1: mu_2k = 2<T_k B, T_k B> - mu_0 and

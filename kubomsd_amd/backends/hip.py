@@ -72,6 +72,7 @@ SIGNATURES = {
     "kmsd_moments": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments_doubling": [_P, _P, _P, _I64, _I64, _I, _D, _D],
     "kmsd_moments_noalloc": [_P, _P, _P, _I64, _I64, _I, _D, _D],
+    "kmsd_moments_group": [_I64, _P, _P, _P, _I64, _I64, _I, _P, _P],
     "kmsd_mu_table": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I],
     "kmsd_block_accumulate": [_P, _P, _P, _I64, _I64, _I64, _P, C.c_int32],
     "kmsd_time_gradient": [_P, _P, _P, _I64, _I64, _I64, _P, _D],
@@ -632,6 +633,34 @@ class HipEngine:
         self._check(fn(self.h, C.c_void_p(B.data_ptr()), C.c_void_p(mu.data_ptr()), int(length), B.shape[1],
                        _dtype_code(B), float(center), float(half)))
         return mu
+
+
+def moments_group(engines, blocks, length, centers, halves):
+    """Device float64 (length, nb) moments of every engine's chain, as
+    HipEngine.moments would give them one by one, with the chains' orders
+    in shared launches (kmsd_moments_group).  engines share one context;
+    blocks are (n_q, nb) device tensors of one dtype."""
+    torch = _torch()
+    engines = list(engines)
+    if not engines or len(blocks) != len(engines) or len(centers) != len(engines) or len(halves) != len(engines):
+        raise KuboMSDError("one block, center and half per engine")
+    ctx = engines[0].ctx
+    prepped = [e._prep(B) for e, B in zip(engines, blocks)]
+    nb, dt = prepped[0].shape[1], prepped[0].dtype
+    if any(B.shape[1] != nb or B.dtype != dt for B in prepped):
+        raise KuboMSDError("grouped chains need blocks of one width and dtype")
+    mus = [torch.empty((int(length), nb), dtype=torch.float64, device=B.device) for B in prepped]
+    k = len(engines)
+    eh = (C.c_void_p * k)(*[e.h for e in engines])
+    bp = (C.c_void_p * k)(*[B.data_ptr() for B in prepped])
+    mp = (C.c_void_p * k)(*[m.data_ptr() for m in mus])
+    cs = np.ascontiguousarray(centers, np.float64)
+    hs = np.ascontiguousarray(halves, np.float64)
+    ctx.bind_stream()
+    rc = ctx.lib.kmsd_moments_group(k, eh, bp, mp, int(length), nb, _dtype_code(prepped[0]), _np_ptr(cs), _np_ptr(hs))
+    if rc:
+        engines[0]._check(rc)
+    return mus
 
 
 def mu_table(ctx, left, right, mode=KMSD_MU_FP64, table=None, accumulate=False):

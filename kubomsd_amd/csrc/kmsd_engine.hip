@@ -722,6 +722,162 @@ extern "C" int kmsd_moments(kmsd_engine *e, const void *B, double *mu, int64_t L
     return guard(e->ctx, [&] { moment_chain(e, B, mu, L, nb, dtype, center, half, nullptr); });
 }
 
+namespace {
+
+// The moment chains of several engines (identity metrics, one dtype, width
+// and length) with their steady-state orders in shared launches.  Each
+// chain keeps the steps, workgroup count, partial slots and reductions of
+// its own kmsd_moments call, so its moments are the same bits.
+void moment_chain_group(const std::vector<kmsd_engine *> &es, const void *const *Bs, double *const *mus, int64_t L,
+                        int64_t nb, int dtype, const double *centers, const double *halves) {
+    kmsd_ctx *c = es[0]->ctx;
+    const int32_t w = int32_t(nb);
+    const size_t count = es.size();
+    struct Job {
+        kmsd_engine *e;
+        const void *B;
+        double *mu;
+        void *t1, *spare;
+        double *part;
+        int cap;
+        int64_t G;
+        size_t slot_doubles;
+        double c, a;
+    };
+    std::vector<Job> jobs(count);
+    const int64_t chunk = std::min<int64_t>(c->moment_chunk, L);
+    for (size_t q = 0; q < count; ++q) {
+        Job &j = jobs[q];
+        j.e = es[q];
+        j.B = Bs[q];
+        j.mu = mus[q];
+        j.c = centers[q];
+        j.a = halves[q];
+        check_block_args(j.B, j.mu, nb, dtype);
+        if (L > 1) check_half(j.a);
+        const int64_t n = n_of(j.e);
+        require_finite(c, dtype, j.B, nb, n, w);
+        const size_t bytes = size_t(n) * size_t(nb) * elem_bytes(dtype);
+        j.t1 = j.e->t0.ensure(c, bytes);
+        j.spare = j.e->t1.ensure(c, bytes);
+        j.cap = cpl_cap_for(dtype, {{j.B, nb}, {j.t1, nb}, {j.spare, nb}});
+        j.G = step_groups(c, n, w, dtype, j.cap, j.e->H);
+        j.slot_doubles = size_t(j.G) * size_t(w) * 2;
+        j.part = static_cast<double *>(j.e->part.ensure(c, sizeof(double) * j.slot_doubles * size_t(chunk)));
+    }
+    int64_t slot = 0, base = 0;
+    auto flush = [&] {
+        for (Job &j : jobs) launch_reduce(c, j.part, j.G, w, slot, j.mu + base * nb, nb, 0);
+        base += slot;
+        slot = 0;
+    };
+    auto step_of = [&](const Job &j, int64_t k, const void *X, const void *P, void *O) {
+        Step s;
+        s.A = j.e->H;
+        s.W = j.B;
+        s.ldw = nb;
+        s.n = n_of(j.e);
+        s.w = w;
+        s.cpl_cap = j.cap;
+        s.groups = j.G;
+        if (k == 0) {  // mu_0 = <B, B>
+            s.A = nullptr;
+            s.Z = j.B;
+            s.ldz = nb;
+        } else if (k == 1) {  // T_1 = (H B - c B) / a
+            s.X = j.B;
+            s.ldx = nb;
+            s.OUT = O;
+            s.ldo = nb;
+            s.alpha = Cplx{1 / j.a, 0};
+            s.beta = Cplx{-j.c / j.a, 0};
+            s.use_beta = true;
+        } else {  // T_(k+1) = 2 (H T_k - c T_k) / a - T_(k-1)
+            s.X = X;
+            s.ldx = nb;
+            s.P = P;
+            s.ldp = nb;
+            s.OUT = O;
+            s.ldo = nb;
+            s.alpha = Cplx{2 / j.a, 0};
+            s.beta = Cplx{-2 * j.c / j.a, 0};
+            s.gamma = -1.0;
+            s.use_beta = s.use_p = true;
+        }
+        return s;
+    };
+    // orders 0, 1 and 2 chain by chain (orders 0 and 1 read B; order 2 writes
+    // the second block), then T_k alternates between t1 and spare
+    for (int64_t k = 0; k < std::min<int64_t>(L, 3); ++k) {
+        if (slot == chunk) flush();
+        for (Job &j : jobs) {
+            Step s = k == 0 ? step_of(j, 0, nullptr, nullptr, nullptr)
+                            : k == 1 ? step_of(j, 1, nullptr, nullptr, j.t1) : step_of(j, 2, j.t1, j.B, j.spare);
+            s.part = j.part + size_t(slot) * j.slot_doubles;
+            launch_step(c, dtype, s);
+            if (k) ++j.e->h_calls;
+        }
+        ++slot;
+    }
+    if (L > 3) {
+        std::vector<GroupChain> chains(count);
+        for (size_t q = 0; q < count; ++q) {
+            const Job &j = jobs[q];
+            chains[q].step = step_of(j, 3, j.spare, j.t1, j.t1);
+            chains[q].step.part = j.part;
+            chains[q].buf[0] = j.t1;
+            chains[q].buf[1] = j.spare;
+            chains[q].part0 = j.part;
+            chains[q].slot_doubles = int64_t(j.slot_doubles);
+        }
+        GroupPlan plan;
+        const bool shared = count > 1 && group_prepare(c, dtype, chains, plan);
+        for (int64_t k = 3; k < L; ++k) {
+            if (slot == chunk) flush();
+            const int parity = int(k & 1);  // buf[parity] holds T_k
+            if (shared) {
+                group_launch(c, plan, parity, int(slot));
+            } else {
+                for (size_t q = 0; q < count; ++q) {
+                    const Job &j = jobs[q];
+                    void *cur = parity ? j.spare : j.t1, *other = parity ? j.t1 : j.spare;
+                    Step s = step_of(j, k, cur, other, other);
+                    s.part = j.part + size_t(slot) * j.slot_doubles;
+                    launch_step(c, dtype, s);
+                }
+            }
+            for (Job &j : jobs) ++j.e->h_calls;
+            ++slot;
+        }
+        flush();
+        HIPCHECK(hipStreamSynchronize(c->stream));  // the plan's table is freed on return
+        return;
+    }
+    flush();
+    HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+extern "C" int kmsd_moments_group(int64_t count, kmsd_engine *const *engines, const void *const *B, double *const *mu,
+                                  int64_t L, int64_t nb, int dtype, const double *center, const double *half) {
+    if (count < 1 || !engines || !B || !mu || !center || !half || !engines[0]) return KMSD_EINVAL;
+    kmsd_ctx *c = engines[0]->ctx;
+    return guard(c, [&] {
+        if (L < 1) throw Error(KMSD_EINVAL, "moment count must be positive");
+        std::vector<kmsd_engine *> es;
+        for (int64_t q = 0; q < count; ++q) {
+            kmsd_engine *e = engines[q];
+            if (!e || e->ctx != c) throw Error(KMSD_EINVAL, "grouped moment chains need engines of one context");
+            if (!e->metric->identity) throw Error(KMSD_EINVAL, "grouped moment chains need identity metrics");
+            for (kmsd_engine *o : es)
+                if (o == e) throw Error(KMSD_EINVAL, "grouped moment chains need distinct engines");
+            es.push_back(e);
+        }
+        moment_chain_group(es, B, mu, L, nb, dtype, center, half);
+    });
+}
+
 extern "C" int kmsd_moments_doubling(kmsd_engine *e, const void *B, double *mu, int64_t L, int64_t nb, int dtype,
                                      double center, double half) {
     if (!e) return KMSD_EINVAL;

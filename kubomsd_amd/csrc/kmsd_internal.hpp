@@ -207,6 +207,27 @@ struct Step {
 // Launches one fused step; returns the number of workgroups used (the
 // partial-sum row count) so callers can size/reduce partials.
 int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s);
+// Steady-state steps of several chains in shared launches (kmsd_moments_group):
+// each chain's step (X, P, OUT and part are taken from buf, part0 per
+// launch; groups pinned), its two alternating blocks and its partial sums.
+struct GroupChain {
+    Step step;
+    void *buf[2] = {nullptr, nullptr};
+    double *part0 = nullptr;
+    int64_t slot_doubles = 0;
+};
+struct GroupPlan {
+    const void *fn = nullptr;  // grouped kernel
+    int gy = 0;
+    int64_t total = 0;         // workgroups per launch (the sum of the chains' own counts)
+    int32_t njobs = 0;
+    DBuf table;                // device table of the chains
+};
+// False when the chains' kernels differ or have no grouped form (then each
+// runs alone); else uploads the chain table.
+bool group_prepare(kmsd_ctx *c, int dtype, const std::vector<GroupChain> &chains, GroupPlan &plan);
+// One shared launch: buf[parity] of every chain is T_k, partial slot `slot`.
+void group_launch(kmsd_ctx *c, const GroupPlan &plan, int parity, int slot);
 // Number of workgroups launch_step would choose for (n, w, dtype) with
 // product operator A (null: no product).  Chains pass it as Step::groups to
 // every launch that writes partial rows of one reduction.

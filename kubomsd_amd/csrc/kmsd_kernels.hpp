@@ -184,7 +184,7 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
 // one partial per (workgroup, column) is written.  Fixed order.
 template <int CPL>
 __device__ __forceinline__ void step_partials(const StepArgs &a, cx<double> (&mom)[CPL], int lane, int wave, int lpr,
-                                              int col, bool colok) {
+                                              int col, bool colok, uint32_t bx) {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         for (int off = lpr; off < 64; off <<= 1) {
@@ -207,7 +207,7 @@ __device__ __forceinline__ void step_partials(const StepArgs &a, cx<double> (&mo
                 s.re += red[q][lane * CPL + c].re;
                 s.im += red[q][lane * CPL + c].im;
             }
-            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(bx) * a.w + col + c] = s;
         }
     }
 }
@@ -221,8 +221,11 @@ __device__ __forceinline__ void step_partials(const StepArgs &a, cx<double> (&mo
 // Per row the row-local streams (X_i, P_i, W_i) are issued first, then the
 // off-diagonal gathers, so their latencies overlap; P and W are read once per
 // step and use nontemporal loads, OUT a nontemporal store.
+// The body takes its workgroup index bx among gx workgroups and its column
+// chunk by (blockIdx.x, gridDim.x, blockIdx.y for one operator per launch;
+// the job's own range in a launch shared by several, see step_group_kernel).
 template <class VT, class MR, bool MC, class AT, int CPL>
-__global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
+__device__ __forceinline__ void step_body(const StepArgs &a, uint32_t bx, uint32_t gx, uint32_t by) {
     using V = cx<VT>;
     using MV = MatVal<MR, MC>;
     using MT = typename MV::type;
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     const int rpb = 4 * rpw;                    // rows per workgroup pass
     const int rloc = wave * rpw + (lane >> a.lpr_log2);
     const int cgrp = lane & (lpr - 1);
-    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const int col = (int(by) * lpr + cgrp) * CPL;
     const bool colok = col < a.w;
 
     const V *X = static_cast<const V *>(a.X);
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 
     // XCD-aware row-block walk.
     const int64_t nblk = (a.n + rpb - 1) / rpb;
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const uint32_t xcd = bx & 7u, slot = bx >> 3, per = gx >> 3;
     const int64_t span = (nblk + 7) / 8;
     const int64_t b0 = int64_t(xcd) * span;
     const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
@@ -328,7 +331,12 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     }
 
     if (!(fl & (F_MOM | F_MOM2))) return;
-    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, bx);
+}
+
+template <class VT, class MR, bool MC, class AT, int CPL>
+__global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
+    step_body<VT, MR, MC, AT, CPL>(a, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
 // step_kernel for an ELL off-diagonal part of compile-time width EW
@@ -344,7 +352,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 // epilogue scales (so equal to a separate product of V when V's entries come
 // in the same column order), and stores it to VOUT.
 template <class VT, class MR, bool MC, class AT, int CPL, int EW, bool VO = false>
-__global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
+__device__ __forceinline__ void ell_step_body(const StepArgs &a, uint32_t bx, uint32_t gx, uint32_t by) {
     using V = cx<VT>;
     using MV = MatVal<MR, MC>;
     using MT = typename MV::type;
@@ -356,7 +364,7 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     const int rpb = 4 * rpw;
     const int rloc = wave * rpw + (lane >> a.lpr_log2);
     const int cgrp = lane & (lpr - 1);
-    const int col = (blockIdx.y * lpr + cgrp) * CPL;
+    const int col = (int(by) * lpr + cgrp) * CPL;
     const bool colok = col < a.w;
 
     const V *X = static_cast<const V *>(a.X);
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     for (int c = 0; c < CPL; ++c) mom[c] = {0.0, 0.0};
 
     const int64_t nblk = (a.n + rpb - 1) / rpb;
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const uint32_t xcd = bx & 7u, slot = bx >> 3, per = gx >> 3;
     const int64_t span = (nblk + 7) / 8;
     const int64_t b0 = int64_t(xcd) * span;
     const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
@@ -454,7 +462,59 @@ __global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
     }
 
     if (!(fl & (F_MOM | F_MOM2))) return;
-    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, bx);
+}
+
+template <class VT, class MR, bool MC, class AT, int CPL, int EW, bool VO = false>
+__global__ __launch_bounds__(256) void ell_step_kernel(const StepArgs a) {
+    ell_step_body<VT, MR, MC, AT, CPL, EW, VO>(a, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+// One chain of a launch shared by several (step_group_kernel): its
+// steady-state Chebyshev step, whose gathered input alternates between two
+// blocks while the other holds T_(k-1) and receives T_(k+1) in place, its
+// partial sums and its workgroups [wg0, wg0 + wgn) of the launch (wg0 a
+// multiple of 8, so each chain's XCD-aware row walk is its own launch's).
+struct GroupJob {
+    StepArgs a;  // X, P, OUT and part are set per launch from the fields below
+    void *buf[2];
+    double *part0;
+    int64_t slot_doubles;
+    uint32_t wg0, wgn;
+};
+
+// Chain owning workgroup b (chains sorted by wg0).
+__device__ __forceinline__ int group_find(const GroupJob *jobs, int njobs, uint32_t b) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].wg0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Steady-state steps of several chains (different matrices, sizes and
+// scalings, one column width) in one launch: each workgroup runs its chain's
+// step with its index inside the chain's own range, so every chain's row
+// partition, partial sums and results are those of a launch of its own.
+// parity: buf[parity] is T_k; slot: the partial-sum slot of this order.
+// EW: ELL width of the prefetching body, 0 for the generic body.
+template <class VT, class MR, bool MC, class AT, int CPL, int EW>
+__global__ __launch_bounds__(256) void step_group_kernel(const GroupJob *jobs, int32_t njobs, int32_t parity,
+                                                         int32_t slot) {
+    const int j = group_find(jobs, njobs, blockIdx.x);
+    const GroupJob &g = jobs[j];
+    StepArgs a = g.a;
+    a.X = g.buf[parity];
+    a.P = g.buf[parity ^ 1];
+    a.OUT = g.buf[parity ^ 1];
+    a.part = g.part0 + int64_t(slot) * g.slot_doubles;
+    const uint32_t bx = blockIdx.x - g.wg0;
+    if constexpr (EW > 0)
+        ell_step_body<VT, MR, MC, AT, CPL, EW>(a, bx, g.wgn, blockIdx.y);
+    else
+        step_body<VT, MR, MC, AT, CPL>(a, bx, g.wgn, blockIdx.y);
 }
 
 // Broadcast of lane E of every 16-lane row to the whole row (DPP
@@ -623,7 +683,7 @@ __global__ __launch_bounds__(256) void bsr4_step_kernel(const StepArgs a) {
     }
 
     if (!(fl & (F_MOM | F_MOM2))) return;
-    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok);
+    step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, blockIdx.x);
 }
 
 // out[s][c] = sum_g part[s][g][c] in a fixed tree order (deterministic).

@@ -346,7 +346,7 @@ int resident_per_cu(KernelFn fn) {
 // resident slot, so the grid-stride row walk runs as a single wave of
 // workgroups with no partially filled second round.
 Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const kmsd_matrix *A, bool f32,
-                KernelFn *fn_out = nullptr, bool vout = false) {
+                KernelFn *fn_out = nullptr, bool vout = false, KernelKey *key_out = nullptr) {
     Shape s;
     const int tuned = c->cpl[dtype == KMSD_C64 ? 0 : 1];
     const int want = tuned ? tuned : default_cpl(dtype, A);
@@ -372,6 +372,7 @@ Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const
     }
     const KernelFn fn = select_kernel(c, k);
     if (fn_out) *fn_out = fn;
+    if (key_out) *key_out = k;
     int64_t g = c->groups;
     if (c->auto_groups) {
         const int64_t resident = int64_t(c->cus) * resident_per_cu(fn);
@@ -422,20 +423,23 @@ int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t
     return lim;
 }
 
-int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
-    if (s.n < 1 || s.w < 1) return 0;
+namespace {
+
+// Kernel arguments, launch shape and kernel of one fused step.
+void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh, KernelFn &fn, KernelKey &key) {
     const kmsd_matrix *A = s.A;
     const bool f32 = f32_values(c, dtype, A, s.f32_matrix);
-    KernelFn fn = nullptr;
+    fn = nullptr;
     if (s.VOUT && (!A || A->fmt == KMSD_BSR4 || !c->ell_kernel || f32 || A->vkind == VK_COMPLEX || !s.vval ||
                    (A->ell != 3 && A->ell != 4 && A->ell != 6)))
         throw Error(KMSD_EDEVICE, "internal error: fused velocity output needs a real width-3/4/6 ELL operator");
-    Shape sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)), A, f32, &fn, s.VOUT != nullptr);
+    sh = shape_for(c, s.n, s.w, dtype, std::min(s.cpl_cap, cpl_limit(s, dtype)), A, f32, &fn, s.VOUT != nullptr,
+                   &key);
     if (s.groups) {
         if (s.groups < 8 || s.groups % 8) throw Error(KMSD_EDEVICE, "internal error: pinned workgroup count");
         sh.gx = s.groups;
     }
-    StepArgs a{};
+    a = StepArgs{};
     uint32_t fl = 0;
     Cplx alpha = s.alpha;
     const bool bsr = A && A->fmt == KMSD_BSR4;
@@ -490,10 +494,81 @@ int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
     a.lpr_log2 = sh.lpr_log2;
     a.flags = fl;
     a.gate = s.gate;
+}
+
+using GroupFn = void (*)(const GroupJob *, int32_t, int32_t, int32_t);
+
+template <class VT, class AT, int CPL>
+GroupFn pick_group(int ell) {
+    if (ell == 3) return step_group_kernel<VT, double, false, AT, CPL, 3>;
+    if (ell == 4) return step_group_kernel<VT, double, false, AT, CPL, 4>;
+    if (ell == 6) return step_group_kernel<VT, double, false, AT, CPL, 6>;
+    return step_group_kernel<VT, double, false, AT, CPL, 0>;
+}
+
+// Grouped form of a step's kernel: FP64 real (or imaginary) matrix values,
+// CSR/ELL storage, no fused second product; null otherwise.
+GroupFn select_group_kernel(const kmsd_ctx *c, const KernelKey &k) {
+    if (k.bsr || k.cv || k.vout || k.f32 || c->fp32_math) return nullptr;
+    if (k.dtype == KMSD_C128) return k.cpl == 2 ? pick_group<double, double, 2>(k.ell) : pick_group<double, double, 1>(k.ell);
+    if (k.cpl == 4) return pick_group<float, double, 4>(k.ell);
+    return k.cpl == 2 ? pick_group<float, double, 2>(k.ell) : pick_group<float, double, 1>(k.ell);
+}
+
+}  // namespace
+
+int64_t launch_step(kmsd_ctx *c, int dtype, const Step &s) {
+    if (s.n < 1 || s.w < 1) return 0;
+    StepArgs a{};
+    Shape sh;
+    KernelFn fn = nullptr;
+    KernelKey key;
+    prepare_step(c, dtype, s, a, sh, fn, key);
     void *args[] = {&a};
     HIPCHECK(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(unsigned(sh.gx), unsigned(sh.gy)), dim3(256),
                              args, 0, c->stream));
     return sh.gx;
+}
+
+bool group_prepare(kmsd_ctx *c, int dtype, const std::vector<GroupChain> &chains, GroupPlan &plan) {
+    if (chains.empty()) return false;
+    std::vector<GroupJob> jobs(chains.size());
+    GroupFn fn = nullptr;
+    int gy = 0;
+    uint32_t wg = 0;
+    for (size_t q = 0; q < chains.size(); ++q) {
+        const GroupChain &ch = chains[q];
+        if (ch.step.groups < 8 || ch.step.groups % 8) return false;  // the chain's own pinned workgroup count
+        Shape sh;
+        KernelFn single = nullptr;
+        KernelKey key;
+        prepare_step(c, dtype, ch.step, jobs[q].a, sh, single, key);
+        const GroupFn g = select_group_kernel(c, key);
+        if (!g || (fn && g != fn) || (gy && sh.gy != gy)) return false;
+        fn = g;
+        gy = sh.gy;
+        jobs[q].buf[0] = ch.buf[0];
+        jobs[q].buf[1] = ch.buf[1];
+        jobs[q].part0 = ch.part0;
+        jobs[q].slot_doubles = ch.slot_doubles;
+        jobs[q].wg0 = wg;
+        jobs[q].wgn = uint32_t(ch.step.groups);
+        wg += uint32_t(ch.step.groups);
+    }
+    void *d = plan.table.ensure(c, sizeof(GroupJob) * jobs.size());
+    copy_h2d(c, d, jobs.data(), sizeof(GroupJob) * jobs.size());
+    plan.fn = reinterpret_cast<const void *>(fn);
+    plan.gy = gy;
+    plan.total = wg;
+    plan.njobs = int32_t(jobs.size());
+    return true;
+}
+
+void group_launch(kmsd_ctx *c, const GroupPlan &plan, int parity, int slot) {
+    const GroupJob *jobs = static_cast<const GroupJob *>(plan.table.p);
+    int32_t nj = plan.njobs, par = parity, sl = slot;
+    void *args[] = {&jobs, &nj, &par, &sl};
+    HIPCHECK(hipLaunchKernel(plan.fn, dim3(unsigned(plan.total), unsigned(plan.gy)), dim3(256), args, 0, c->stream));
 }
 
 void launch_reduce(kmsd_ctx *c, const double *part, int64_t groups, int32_t w, int64_t nsteps, double *out,
