@@ -53,6 +53,7 @@ class TransportJob:
     est_bytes: int = 0
     size_class: int = 0
     meta: dict = field(default_factory=dict)
+    group_key: object = None  # jobs with equal keys may share launches (None: always alone)
 
 
 @dataclass
@@ -66,6 +67,7 @@ class JobResult:
     diffusion: np.ndarray = None
     blocks: list = None
     error: str = None
+    shared: int = 1  # jobs whose moment chains ran in shared launches with this one's (1: alone)
 
     @property
     def latency_s(self):
@@ -84,9 +86,13 @@ def estimate_job_bytes(n, nnz, nb, na, precision, nonorthogonal, n_velocity=1):
 
 class TransportServer:
     def __init__(self, device=0, slots=4, hbm_budget_bytes=None, class_edges=(1 << 26, 1 << 30),
-                 max_wait_s=30.0, keep_blocks=False, on_result=None):
-        """on_result: optional callable(JobResult), called as each job finishes."""
+                 max_wait_s=30.0, keep_blocks=False, on_result=None, group_max=8):
+        """on_result: optional callable(JobResult), called as each job finishes.
+        group_max: most jobs of one group key a slot takes at once (their
+        moment chains share launches); 1 disables grouping."""
         self.on_result = on_result
+        self.group_max = max(1, int(group_max))
+        self.groups_formed = []
         self.device = device
         self.slots = slots
         self.procs = []
@@ -139,20 +145,38 @@ class TransportServer:
                     return job
         return None
 
+    def _admit(self, job):
+        self.queues[job.size_class].remove(job)
+        self.pending -= 1
+        self.used += job.est_bytes
+        self.running += 1
+        self.peak_used = max(self.peak_used, self.used)
+        self.peak_running = max(self.peak_running, self.running)
+
     def _take(self):
+        """The next admitted work for a slot: a list of one job, or of up to
+        group_max waiting jobs with the first one's group key (in arrival
+        order, each within the budget left)."""
         with self.cv:
             while True:
                 if self.closed and self.pending == 0:
                     return None
                 job = self._pick(self.clock())
                 if job is not None:
-                    self.queues[job.size_class].remove(job)
-                    self.pending -= 1
-                    self.used += job.est_bytes
-                    self.running += 1
-                    self.peak_used = max(self.peak_used, self.used)
-                    self.peak_running = max(self.peak_running, self.running)
-                    return job
+                    self._admit(job)
+                    group = [job]
+                    if job.group_key is not None and self.group_max > 1:
+                        waiting = sorted((j for q in self.queues for j in q if j.group_key == job.group_key),
+                                         key=lambda j: j.arrival_s)
+                        for other in waiting:
+                            if len(group) == self.group_max:
+                                break
+                            if other.est_bytes <= self.budget - self.used:
+                                self._admit(other)
+                                group.append(other)
+                    if len(group) > 1:
+                        self.groups_formed.append([j.job_id for j in group])
+                    return group
                 self.cv.wait(0.05)
 
     def _release(self, job):
@@ -246,38 +270,40 @@ class TransportServer:
 
     def _dispatch(self, slot):
         while True:
-            job = self._take()
-            if job is None:
+            group = self._take()
+            if group is None:
                 return
             start = self.clock()
-            res, err = {}, None
+            outs = [({}, None)] * len(group)
             lost = False
             for attempt in (0, 1):
                 proc, conn = self.procs[slot]
                 try:
-                    conn.send(job)
-                except (EOFError, OSError) as exc:  # the worker had ended before this job
-                    err = f"worker process unavailable ({type(exc).__name__})"
+                    conn.send(group)
+                except (EOFError, OSError) as exc:  # the worker had ended before this work
+                    outs = [({}, f"worker process unavailable ({type(exc).__name__})")] * len(group)
                     lost = not self._replace(slot)
                     if lost:
                         break
                     continue  # one retry on a fresh worker
-                except Exception as exc:  # the job cannot be sent (e.g. not picklable)
-                    err = f"job could not be sent to a worker: {type(exc).__name__}: {exc}"
+                except Exception as exc:  # the jobs cannot be sent (e.g. not picklable)
+                    outs = [({}, f"job could not be sent to a worker: {type(exc).__name__}: {exc}")] * len(group)
                     break
                 try:
-                    res, err = conn.recv()
-                except (EOFError, OSError) as exc:  # the worker died running this job
+                    outs = conn.recv()
+                except (EOFError, OSError) as exc:  # the worker died running this work
                     lost = not self._replace(slot)
-                    res, err = {}, f"worker process ended ({type(exc).__name__}, exit code {proc.exitcode})"
+                    outs = [({}, f"worker process ended ({type(exc).__name__}, exit code {proc.exitcode})")] * \
+                        len(group)
                 break
             finish = self.clock()
-            self._release(job)
-            result = JobResult(job.job_id, job.arrival_s, start, finish, slot, res.get("dos"), res.get("diffusion"),
-                               res.get("blocks"), err)
-            self.results[job.job_id] = result
-            if self.on_result is not None:
-                self.on_result(result)
+            for job, (res, err) in zip(group, outs):
+                self._release(job)
+                result = JobResult(job.job_id, job.arrival_s, start, finish, slot, res.get("dos"),
+                                   res.get("diffusion"), res.get("blocks"), err, res.get("shared", 1))
+                self.results[job.job_id] = result
+                if self.on_result is not None:
+                    self.on_result(result)
             if lost:  # no worker could be started for this slot: the other slots keep serving
                 return
 
@@ -334,15 +360,28 @@ def _worker_main(conn, device, budget, keep_blocks):
     ctx = HipContext(device, hbm_budget_bytes=budget)
     conn.send("ready")
     while True:
-        job = conn.recv()
-        if job is None:
+        group = conn.recv()
+        if group is None:
             break
-        try:
-            res, err = run_job(job, ctx, keep_blocks), None
-        except Exception as exc:  # reported per job, the slot keeps serving
-            res, err = {}, f"{type(exc).__name__}: {exc}"
-        conn.send((res, err))
+        conn.send(run_jobs(group, ctx, keep_blocks))
     conn.close()
+
+
+def run_jobs(group, ctx, keep_blocks=False):
+    """[(result, error)] for a list of jobs on one context: one job alone, or
+    several through run_job_group; errors are reported per job."""
+    if len(group) > 1:
+        try:
+            return run_job_group(group, ctx, keep_blocks)
+        except Exception:  # noqa: BLE001 -- the jobs are then run one by one, each reporting its own error
+            pass
+    outs = []
+    for job in group:
+        try:
+            outs.append((run_job(job, ctx, keep_blocks), None))
+        except Exception as exc:  # reported per job, the slot keeps serving
+            outs.append(({}, f"{type(exc).__name__}: {exc}"))
+    return outs
 
 
 class FlakeModel:
@@ -359,34 +398,91 @@ class FlakeModel:
         return ModelBundle(m.H, m.S, velocity={"x": m.velocity["x"]})
 
 
-def run_job(job, ctx, keep_blocks=False):
-    """Builds the job's operators on the slot's context and runs the
-    stochastic estimator; returns DOS and the diffusion estimate D(E, t)."""
+def _prepare(job, ctx):
+    """(ops, bounds report) of a job: its operators on the slot's context and
+    its spectral bounds, resolved as iter_stochastic_blocks resolves them."""
     from .metric.bounds import certified_bounds
+    from .metric.device_bounds import estimate_spectral_bounds_device
     from .operators.velocity import build_operators
-    from .solvers.stochastic import iter_stochastic_blocks
     model = job.model() if callable(job.model) else job.model
     ops = build_operators(model, job.cfg, ctx)
-    report = None if job.cfg.spectral_bounds is not None else \
-        certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower)
-    blocks = list(iter_stochastic_blocks(ops, job.cfg, job.method, samples=job.samples, bounds_report=report))
+    if job.cfg.spectral_bounds is None:
+        report = certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower)
+    else:
+        report = estimate_spectral_bounds_device(ops, job.cfg.spectral_bounds, ops.metric.bounds_report["certified"])
+    return ops, report
+
+
+def _result(blocks, keep_blocks):
     first = blocks[0]
     key = "numerator_diffusion_A2_per_fs/msd_gradient"
     diff = np.concatenate([b.arrays[key] for b in blocks], axis=1) if key in first.arrays else None
     out = {"dos": first.static["dos_states_per_eV"], "diffusion": diff}
     if keep_blocks:
         out["blocks"] = blocks
-    del ops
     return out
+
+
+def run_job(job, ctx, keep_blocks=False):
+    """Builds the job's operators on the slot's context and runs the
+    stochastic estimator; returns DOS and the diffusion estimate D(E, t)."""
+    from .solvers.stochastic import iter_stochastic_blocks
+    ops, report = _prepare(job, ctx)
+    blocks = list(iter_stochastic_blocks(ops, job.cfg, job.method, samples=job.samples, bounds_report=report))
+    del ops
+    return _result(blocks, keep_blocks)
+
+
+def run_job_group(jobs, ctx, keep_blocks=False):
+    """Several stepwise jobs on one context whose moment chains share launches
+    (solvers.stepwise.stepwise_samples_group): each job's samples form one
+    batch, computed together with the other jobs' batches, then its
+    statistics.  A job that cannot join (nonorthogonal metric, VAC path,
+    batch not admitted whole) runs alone.  Every job's result is bit for bit
+    what run_job gives it.  Returns [(result, error)] in the jobs' order; an
+    error in the shared part propagates (run_jobs then runs the jobs one by
+    one)."""
+    from dataclasses import replace
+    import torch
+    from .metric.sampling import sample_metric
+    from .solvers.stepwise import stepwise_samples_group
+    from .solvers.stochastic import admitted_batch, iter_stochastic_blocks
+    outs = [None] * len(jobs)
+    members = []  # (position, job, ops, report, samples)
+    for q, job in enumerate(jobs):
+        ops, report = _prepare(job, ctx)
+        cfg = replace(job.cfg, spectral_bounds=report["bounds"])
+        ns = cfg.nsamples if job.samples is None else job.samples.shape[1]
+        if (job.method != "stepwise" or cfg.include_vac or not ops.metric.identity
+                or admitted_batch(ops, cfg, "stepwise") < ns):
+            blocks = list(iter_stochastic_blocks(ops, job.cfg, job.method, samples=job.samples, bounds_report=report))
+            outs[q] = (_result(blocks, keep_blocks), None)
+            continue
+        B = sample_metric(cfg.seed, np.arange(ns), ops, cfg) if job.samples is None else \
+            ops.asarray(job.samples, torch.complex128)
+        members.append((q, job, ops, report, cfg, B))
+    shared = stepwise_samples_group([(ops, cfg, B) for _, _, ops, _, cfg, B in members])
+    for (q, job, ops, report, cfg, B), res in zip(members, shared):
+        ns = B.shape[1]
+        try:
+            blocks = list(iter_stochastic_blocks(ops, job.cfg, "stepwise", samples=job.samples, bounds_report=report,
+                                                 batch_results={(0, ns): res}))
+            outs[q] = ({**_result(blocks, keep_blocks), "shared": len(members)}, None)
+        except Exception as exc:  # this job's statistics failed; the others stand
+            outs[q] = ({}, f"{type(exc).__name__}: {exc}")
+    return outs
 
 
 def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256, 512, 1024, 2048, 4096),
                     vectors=(4, 16, 64), nonorthogonal_fraction=0.2, precision="mixed", times=(0.0, 0.5, 1.0),
-                    seed=20261208, moment_doubling=False):
+                    seed=20261208, moment_doubling=False, group_elements=1 << 20):
     """The serving workload: Poisson arrivals; honeycomb flakes with N
     log-uniform in n_range, W ~ U[0.5, 3] eV, M and the vector count drawn from
     the given sets, a fraction with a nonorthogonal overlap; each job is a
-    stepwise transport run (DOS + MSD + D(E) at `times`) on one batch."""
+    stepwise transport run (DOS + MSD + D(E) at `times`) on one batch.
+    Orthogonal jobs with at most group_elements orbitals x vectors get a
+    group key (M, vectors, times, precision), so a slot may run several of
+    them with shared launches."""
     from .config import RunConfig
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, n_jobs))
@@ -407,6 +503,9 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
                         statistics_block=max(1, nb // 4), include_vac=False, precision=precision,
                         metric_rtol=1e-10, seed=20261019 + i, autotune=False, moment_doubling=moment_doubling)
         est = estimate_job_bytes(n_act, 4 * n_act, nb, 1, precision, nonorth)
+        # launch-bound jobs (a Chebyshev order moves under ~32 MB) share launches
+        # with others of the same shape
+        key = (M, nb, tuple(times), precision) if not nonorth and n_act * nb <= group_elements else None
         jobs.append(TransportJob(i, build, cfg, "stepwise", None, float(arrivals[i]), est,
-                                 meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth)))
+                                 meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth), group_key=key))
     return jobs

@@ -51,7 +51,13 @@ def propagate(states, dt_fs, ops, cfg):
     return B, {**diag, "segments": nsegments, "relative_S_norm_drift": drift}
 
 
-def stepwise_samples(ops, cfg, samples):
+def _stepwise_program(ops, cfg, samples):
+    """The stepwise estimator as a generator: it yields each moment-chain
+    input (the DOS chain's, then one per output time) as (ops, block),
+    receives that chain's DOS projection (width, ne), and returns
+    (dos, numerators, diagnostics).  stepwise_samples runs it with one chain
+    per request; stepwise_samples_group runs several programs in lockstep and
+    serves their requests with shared launches."""
     import torch
     # The augmented state is propagated in complex128 in both precisions
     # (about 3 % of the column-steps), starting from the samples as given
@@ -61,7 +67,7 @@ def stepwise_samples(ops, cfg, samples):
     initial = state0 if ops.dtype == torch.complex128 else state0.to(ops.dtype)
     nb = initial.shape[1]
     na, nt, ne = len(cfg.axes), len(cfg.time_fs), len(cfg.energy_eV)
-    dos = projected_quadratic(initial, ops, cfg)
+    dos = yield ops, initial
     total = torch.cat([state0] + [torch.zeros_like(state0) for _ in cfg.axes], dim=1).contiguous()
     output = torch.zeros((nb, na, nt, ne), dtype=torch.float64, device=initial.device)
     c, a = scaling(cfg)
@@ -82,13 +88,65 @@ def stepwise_samples(ops, cfg, samples):
             drift = max(drift, _drift(ops, total[:, :nb].contiguous(), norm0))
             if drift > drift_limit(cfg):
                 raise NumericalError(f"stepwise base propagation S-norm drift {drift:g}")
-            proj = projected_quadratic(total[:, nb:].to(initial.dtype).contiguous(), ops, cfg)  # (na*nb, ne)
+            proj = yield ops, total[:, nb:].to(initial.dtype).contiguous()  # (na*nb, ne)
             output[:, :, j, :] = proj.reshape(na, nb, ne).permute(1, 0, 2)
             log.info("stepwise: output time %g fs done (%d of %d), %.1f s", t, j, nt - 1, time.perf_counter() - t0)
     return dos, {"msd_A2/commutator": output}, {
         "algorithm": "velocity-driven augmented Chebyshev commutator", "segments": segments,
         "max_propagation_order": max(orders, default=0), "max_relative_S_norm_drift": drift,
         "seconds": time.perf_counter() - t0}
+
+
+def stepwise_samples(ops, cfg, samples):
+    program = _stepwise_program(ops, cfg, samples)
+    request = next(program)
+    while True:
+        try:
+            request = program.send(projected_quadratic(request[1], ops, cfg))
+        except StopIteration as done:
+            return done.value
+
+
+def stepwise_samples_group(items):
+    """stepwise_samples for several (ops, cfg, samples) at once, on one device
+    context: the programs advance in lockstep, and the moment chains they
+    request together (identity metrics, one moment count, chain width and
+    storage type, no moment doubling) run with their orders in shared
+    launches (kmsd_moments_group), bit for bit what each would compute alone.
+    Returns the list of (dos, numerators, diagnostics)."""
+    from ..backends.hip import moments_group
+    from .chebyshev import reconstruction
+    programs = [_stepwise_program(ops, cfg, B) for ops, cfg, B in items]
+    cfgs = [cfg for _, cfg, _ in items]
+    requests = [next(p) for p in programs]
+    results = [None] * len(programs)
+    while any(r is not None for r in requests):
+        live = [q for q, r in enumerate(requests) if r is not None]
+        buckets = {}
+        for q in live:
+            ops, B = requests[q]
+            cfg = cfgs[q]
+            key = (cfg.n_moments, B.shape[1], B.dtype) \
+                if ops.metric.identity and not getattr(cfg, "moment_doubling", False) else ("alone", q)
+            buckets.setdefault(key, []).append(q)
+        answers = {}
+        for key, qs in buckets.items():
+            if len(qs) == 1:
+                ops, B = requests[qs[0]]
+                answers[qs[0]] = projected_quadratic(B, ops, cfgs[qs[0]])
+                continue
+            sc = [scaling(cfgs[q]) for q in qs]
+            mus = moments_group([requests[q][0].native for q in qs], [requests[q][1] for q in qs],
+                                cfgs[qs[0]].n_moments, [c for c, _ in sc], [a for _, a in sc])
+            for q, mu in zip(qs, mus):
+                answers[q] = (reconstruction(requests[q][0], cfgs[q]) @ mu).T
+        for q in live:
+            try:
+                requests[q] = programs[q].send(answers[q])
+            except StopIteration as done:
+                results[q] = done.value
+                requests[q] = None
+    return results
 
 
 def iter_stepwise_blocks(ops, config, samples=None):

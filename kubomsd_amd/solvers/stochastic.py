@@ -98,7 +98,7 @@ def admitted_batch(ops, cfg, method):
     raise ResourceBlocked("RESOURCE_BLOCKED: one sample's device workspace exceeds the HBM budget")
 
 
-def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
+def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats, batch_results=None):
     """Runs the samples in global order, `batch` at a time, and adds each
     sample's DOS and numerators to its statistics block in sample order
     (reference _sample_batches, solvers/stochastic.py:32-71).  The reference
@@ -115,9 +115,11 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
     for first in range(0, ns, batch):
         last = min(first + batch, ns)
         ids = np.arange(first, last)
-        B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else \
-            ops.asarray(samples[:, first:last], torch.complex128)
-        if method == "stepwise":
+        if batch_results and (first, last) in batch_results:
+            dos, nums, diag = batch_results[(first, last)]  # computed with other jobs' batches
+        elif method == "stepwise":
+            B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else \
+                ops.asarray(samples[:, first:last], torch.complex128)
             dos, nums, diag = stepwise_samples(ops, cfg, B)
             if cfg.include_vac:
                 _, extra, extra_diag = spectral_samples(ops, cfg, B)
@@ -125,6 +127,8 @@ def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats):
                 diag["additional_vac_path"] = extra_diag
                 diag["additional_cost_recorded"] = True
         else:
+            B = sample_metric(cfg.seed, ids, ops, cfg) if samples is None else \
+                ops.asarray(samples[:, first:last], torch.complex128)
             dos, nums, diag = spectral_samples(ops, cfg, B)
         if len(cfg.time_fs) > 1:
             nums["diffusion_A2_per_fs/msd_gradient"] = stats.msd_gradient(cfg.time_fs, nums["msd_A2/" + primary],
@@ -196,11 +200,13 @@ def _normalize_slab(raw, dos_blocks, counts, ns, valid, key, stats):
     return arrays
 
 
-def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None):
+def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None, batch_results=None):
     """Spectral bounds come from the device two-start generalized Ritz check
     (metric/device_bounds.py), which also validates user bounds as the
     reference does; bounds_report: precomputed bounds (e.g.
-    metric.bounds.certified_bounds) used instead."""
+    metric.bounds.certified_bounds) used instead.  batch_results:
+    {(first, last): (dos, numerators, diagnostics)} of sample batches already
+    computed (the serving group runner's); other batches are computed here."""
     cfg = config
     report = bounds_report or estimate_spectral_bounds_device(ops, cfg.spectral_bounds,
                                                               ops.metric.bounds_report["certified"])
@@ -216,7 +222,8 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     accum = {key: torch.zeros(shape, dtype=torch.float64, device=dev) for key in keys}
     dos_blocks = torch.zeros((nbk, len(cfg.energy_eV)), dtype=torch.float64, device=dev)
     stats = DeviceStatistics(ops.ctx)
-    batch = admitted_batch(ops, cfg, method)
+    # precomputed batches were admitted by their runner
+    batch = max(b - a for a, b in batch_results) if batch_results else admitted_batch(ops, cfg, method)
     # The reported kernel resolution is host root-finding only; it runs on a
     # helper thread while the samples' device work (inside library calls
     # that release the GIL) proceeds.
@@ -224,7 +231,7 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     with ThreadPoolExecutor(max_workers=1) as pool:
         resolution = pool.submit(kernel_resolution, cfg)
         t0 = time.perf_counter()
-        diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats)
+        diagnostics = _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats, batch_results)
         seconds = time.perf_counter() - t0
         valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
                                                          diagnostics, batch, stats, resolution.result())

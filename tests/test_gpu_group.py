@@ -71,3 +71,39 @@ def test_grouped_moments_match_oracle_and_reject_mixed_contexts(hip_ctx):
                       [0, 0], [9, 9])
     with pytest.raises(KuboMSDError, match="distinct"):
         moments_group([engines[0], engines[0]], [to_dev(hip_ctx, Bh[0], "c128")] * 2, 8, [0, 0], [9, 9])
+
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stepwise_group_equals_alone_and_oracle(hip_ctx, precision):
+    """Three stepwise requests of one size class (square lattices of
+    different sizes and disorder) run with shared launches: each one's DOS
+    and MSD numerators are bitwise what stepwise_samples gives it alone, and
+    match the oracle's stepwise orchestration (fp64 within 1e-9, mixed
+    within 3e-7)."""
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.contracts import ModelBundle
+    from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.stepwise import stepwise_samples, stepwise_samples_group
+    from oracle import kpm
+    E = np.linspace(-4.5, 4.5, 21)
+    cfg = RunConfig(energy_eV=E, time_fs=[0.0, 0.5, 1.0], solver="stochastic-kpm", kernel="jackson", backend="hip",
+                    n_moments=64, axes=("x",), nsamples=4, sample_batch=4, include_vac=False, precision=precision,
+                    metric_rtol=1e-13, segment_fs=1.0, spectral_bounds=(-5.6, 5.6))
+    items, models = [], []
+    for q, (L, W) in enumerate([(20, 2.0), (26, 1.0), (15, 3.0)]):
+        m = square_lattice(L, W=W, seed=70 + q)
+        ops = build_operators(ModelBundle(m.H, m.S, velocity=m.velocity), cfg, hip_ctx)
+        items.append((ops, cfg, phases(m.n, 4, 80 + q)))
+        models.append(m)
+    grouped = stepwise_samples_group(items)
+    tol = 1e-9 if precision == "fp64" else 3e-7
+    kcfg = kpm.config(energy_eV=E, time_fs=cfg.time_fs, spectral_bounds=cfg.spectral_bounds, n_moments=64,
+                      segment_fs=1.0)
+    for (ops, _, B), m, (dos, nums, _) in zip(items, models, grouped):
+        ados, anums, _ = stepwise_samples(ops, cfg, B)
+        np.testing.assert_array_equal(host(dos), host(ados))
+        np.testing.assert_array_equal(host(nums["msd_A2/commutator"]), host(anums["msd_A2/commutator"]))
+        edos, enums = kpm.stepwise_samples(kpm.Ops(m.H, m.S, m.velocity, rtol=1e-13), kcfg, B)
+        assert rel_err(host(dos), edos) <= tol
+        assert rel_err(host(nums["msd_A2/commutator"]), enums["msd_A2/commutator"]) <= tol

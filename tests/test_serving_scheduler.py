@@ -11,13 +11,14 @@ class _Srv(TransportServer):
         self.max_wait_s, self.cv = 5.0, __import__("threading").Condition()
         self.pending, self.closed, self.running, self.peak_used, self.peak_running = 0, False, 0, 0, 0
         self.t0 = 0.0
+        self.group_max, self.groups_formed = 8, []
 
     def clock(self):
         return self.now
 
 
-def job(i, b, t):
-    return TransportJob(i, None, None, arrival_s=t, est_bytes=b)
+def job(i, b, t, key=None):
+    return TransportJob(i, None, None, arrival_s=t, est_bytes=b, group_key=key)
 
 
 def test_oldest_fitting_job_first_and_small_jobs_pass_a_blocked_large_one():
@@ -25,22 +26,22 @@ def test_oldest_fitting_job_first_and_small_jobs_pass_a_blocked_large_one():
     s.now = 1.0
     for j in (job(0, 900, 0.0), job(1, 50, 0.1), job(2, 40, 0.2), job(3, 700, 0.3)):
         s.submit(j)
-    assert s._take().job_id == 0          # oldest fits: admitted
+    assert s._take()[0].job_id == 0          # oldest fits: admitted
     assert s._pick(1.0).job_id == 1       # 900 used: small ones pass the waiting 700
-    assert s._take().job_id == 1
-    assert s._take().job_id == 2
+    assert s._take()[0].job_id == 1
+    assert s._take()[0].job_id == 2
     assert s.used == 990
     # 700 does not fit until the big one releases
     assert s._pick(1.0) is None
     s._release(job(0, 900, 0.0))
-    assert s._take().job_id == 3
+    assert s._take()[0].job_id == 3
 
 
 def test_starving_job_blocks_new_admissions():
     s = _Srv(1000)
     s.submit(job(0, 800, 0.0))
     s.now = 0.5
-    assert s._take().job_id == 0
+    assert s._take()[0].job_id == 0
     s.submit(job(1, 900, 1.0))  # cannot fit while job 0 runs
     s.submit(job(2, 10, 2.0))
     s.now = 2.0
@@ -81,7 +82,8 @@ class _FakeConn:
         self.script = list(script)  # per send: "ok", "dead", "unpicklable", "dies"
         self.action = None
 
-    def send(self, job):
+    def send(self, group):
+        self.group = group
         self.action = self.script.pop(0)
         if self.action == "dead":
             raise BrokenPipeError("worker gone")
@@ -91,7 +93,7 @@ class _FakeConn:
     def recv(self):
         if self.action == "dies":
             raise EOFError
-        return {"dos": "dos-of-job"}, None
+        return [({"dos": "dos-of-job"}, None) for _ in self.group]
 
     def close(self):
         pass
@@ -152,3 +154,49 @@ def test_lost_slot_reports_remaining_jobs():
     assert "exit code" in res[0].error
     assert all("no serving slot" in res[j].error for j in (1, 2))
     assert srv.used == 100 * 0 and srv.pending == 0
+
+
+def test_jobs_of_one_group_key_are_taken_together():
+    """A slot takes the oldest admissible job and then, up to group_max, the
+    waiting jobs with its group key in arrival order, each within the budget
+    left; keyless jobs and other keys stay queued."""
+    s = _Srv(1000)
+    s.now = 1.0
+    for j in (job(0, 100, 0.0, "a"), job(1, 100, 0.1, "b"), job(2, 100, 0.2, "a"), job(3, 100, 0.3),
+              job(4, 700, 0.4, "a"), job(5, 100, 0.5, "a")):
+        s.submit(j)
+    s.group_max = 3
+    assert [j.job_id for j in s._take()] == [0, 2, 4]
+    assert s.used == 900 and s.running == 3
+    # job 5 (key "a") no longer fits next to the admitted ones; "b" and the keyless job go alone
+    assert [j.job_id for j in s._take()] == [1]
+    s._release(job(4, 700, 0.4))
+    assert [j.job_id for j in s._take()] == [3]
+    assert [j.job_id for j in s._take()] == [5]
+    assert s.groups_formed == [[0, 2, 4]]
+    s.group_max = 1
+    for j in (job(6, 10, 0.6, "c"), job(7, 10, 0.7, "c")):
+        s.submit(j)
+    assert [j.job_id for j in s._take()] == [6]
+
+
+def test_grouped_dispatch_reports_each_job():
+    jobs = [TransportJob(i, None, None, arrival_s=0.0, est_bytes=100, group_key="k") for i in range(3)]
+    srv = _DispatchSrv([["ok"], []])
+    res = srv.run(jobs, realtime=False)
+    assert sorted(res) == [0, 1, 2] and all(res[j].error is None for j in range(3))
+    assert srv.groups_formed == [[0, 1, 2]] and srv.used == 0
+    # a worker that dies with a group reports the error on every member
+    srv = _DispatchSrv([["dies"], []])
+    res = srv.run(jobs, realtime=False)
+    assert all("exit code -11" in res[j].error for j in range(3)) and srv.used == 0
+
+
+def test_trace_group_keys():
+    jobs = synthetic_trace(40, rate_per_s=8.0, n_range=(1e3, 1e4), nonorthogonal_fraction=0.3, seed=5)
+    for j in jobs:
+        if j.meta["nonorthogonal"] or j.meta["n"] * j.meta["vectors"] > 1 << 20:
+            assert j.group_key is None
+        else:
+            assert j.group_key == (j.meta["M"], j.meta["vectors"], tuple(j.cfg.time_fs), j.cfg.precision)
+    assert any(j.group_key is not None for j in jobs)
