@@ -55,15 +55,21 @@ def parse():
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
     p.add_argument("--pcg-poll", type=int, default=0, help="override PCG iterations between host polls")
+    p.add_argument("--tile-bytes", type=int, default=None,
+                   help="nonorthogonal: gathered-block bytes per tile of the plane-tiled 4x4-block walk "
+                        "(0: natural row order; default: the library's)")
     p.add_argument("--moment-doubling", action="store_true",
                    help="serving/stepwise: orthogonal jobs take moments from the product identities "
                         "(errors are still reported against the FP64 three-term recursion)")
     p.add_argument("--serving", action="store_true",
                    help="run the continuous-batching serving trace instead of the chain benchmark")
     p.add_argument("--jobs", type=int, default=64)
-    p.add_argument("--rate", type=float, default=8.0, help="Poisson arrival rate (jobs/s)")
+    p.add_argument("--rate", type=float, default=8.0,
+                   help="Poisson arrival rate (jobs/s); 0: every job waiting at the start (service capacity)")
     p.add_argument("--n-max", type=float, default=4e5, help="largest job size (orbitals); reference: 4e6")
     p.add_argument("--slots", type=int, default=6)
+    p.add_argument("--group-max", type=int, default=8,
+                   help="serving: most jobs of one shape a slot runs with shared launches (1: none)")
     p.add_argument("--check-jobs", type=int, default=6, help="jobs re-run alone in FP64 for the D(E) error")
     p.add_argument("--tmax", type=float, default=5.0, help="stepwise: last output time (fs); reference: 50")
     p.add_argument("--dt", type=float, default=1.0, help="stepwise: output time step (fs)")
@@ -112,12 +118,16 @@ def serving(args):
     vectors in {4,16,64}, 20% nonorthogonal), admitted by slots and HBM budget."""
     import torch
     from kubomsd_amd.serving import TransportServer, synthetic_trace
-    jobs = synthetic_trace(args.jobs, rate_per_s=args.rate, n_range=(1e4, args.n_max),
+    jobs = synthetic_trace(args.jobs, rate_per_s=args.rate if args.rate > 0 else 1.0, n_range=(1e4, args.n_max),
                            moment_doubling=args.moment_doubling)
+    if args.rate == 0:
+        for j in jobs:
+            j.arrival_s = 0.0
     def report(r):
         print(f"# job {r.job_id} done on slot {r.slot}: latency {r.latency_s:.2f} s"
+              + (f", shared launches with {r.shared - 1} others" if r.shared > 1 else "")
               + (f", error {r.error}" if r.error else ""), file=sys.stderr, flush=True)
-    srv = TransportServer(slots=args.slots, on_result=report)
+    srv = TransportServer(slots=args.slots, on_result=report, group_max=args.group_max)
     srv.start()  # worker processes up before the clock starts
     t0 = time.perf_counter()
     res = srv.run(jobs, realtime=True)
@@ -127,7 +137,7 @@ def serving(args):
     # D(E) error vs FP64: the first check_jobs re-run alone in complex128
     from dataclasses import replace
     errs = []
-    check = [j for j in synthetic_trace(args.jobs, rate_per_s=args.rate, n_range=(1e4, args.n_max),
+    check = [j for j in synthetic_trace(args.jobs, rate_per_s=args.rate if args.rate > 0 else 1.0, n_range=(1e4, args.n_max),
                                         precision="fp64")][:args.check_jobs]
     with TransportServer(slots=1) as one:
         ref = one.run([replace(j, arrival_s=0.0) for j in check], realtime=False)
@@ -136,16 +146,21 @@ def serving(args):
         if a is not None and b is not None and np.max(np.abs(b)) > 0:
             errs.append(float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
     work = sum(3 * j.meta["M"] * j.meta["vectors"] for j in jobs)
+    arrivals = "all waiting at t = 0" if args.rate == 0 else f"Poisson {args.rate}/s"
     line = {"metric": "Chebyshev vector-steps/s & HBM GB/s vs peak; p50/p99 job latency; err vs FP64",
             "value": stats["p99_latency_s"], "unit": "s (p99 job latency)", "n_gpus": 1, "steps": len(jobs),
             "warmup": 0, "ms_per_step": 1e3 * wall / len(jobs), "higher_is_better": False, "scaling": "weak",
             "vs_baseline": None, "dtype": "c64+f64", "data": "synthetic (seeded honeycomb flakes, Poisson arrivals)",
-            "config": {"workload": f"serving: {len(jobs)} jobs, rate {args.rate}/s, N logU[1e4,{args.n_max:.0e}], "
-                                   f"M in 256..4096, vectors in 4/16/64, 20% nonorthogonal, {args.slots} slots"
+            "config": {"workload": f"serving: {len(jobs)} jobs, arrivals {arrivals}, N logU[1e4,{args.n_max:.0e}], "
+                                   f"M in 256..4096, vectors in 4/16/64, 20% nonorthogonal, {args.slots} slots, "
+                                   f"up to {args.group_max} jobs per shared launch"
                                    + (", moments by doubling (S = I jobs)" if args.moment_doubling else ""),
                        "parallelism": "single GPU, one worker process per slot"},
             "serving": {**stats, "peak_concurrent_jobs": srv.peak_running, "peak_budget_bytes": srv.peak_used,
-                        "vector_orders_total": work, "diffusion_rel_err_vs_fp64": errs}}
+                        "groups": len(srv.groups_formed),
+                        "jobs_in_shared_launches": sum(len(g) for g in srv.groups_formed),
+                        "vector_orders_total": work, "vector_orders_per_s": work / wall,
+                        "diffusion_rel_err_vs_fp64": errs}}
     print(json.dumps(line))
 
 
@@ -189,6 +204,8 @@ def nonorthogonal(args):
     ctx = HipContext(0)
     if args.pcg_poll:
         ctx.set_tuning("pcg_poll", args.pcg_poll)
+    if args.tile_bytes is not None:
+        ctx.set_tuning("tile_bytes", args.tile_bytes)
     t_build = time.perf_counter()
     model = cubic_orbital_model(args.atoms)
     n, nb, M = model.n, args.vectors, args.moments

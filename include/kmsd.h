@@ -133,12 +133,14 @@ int kmsd_build_info(char *buf, int64_t len);
 int kmsd_matrix_create(kmsd_ctx *ctx, int fmt, int64_t n, int64_t nnz, const int64_t *rowptr,
                        const int64_t *colidx, const double *values, kmsd_matrix **out);
 int kmsd_matrix_destroy(kmsd_matrix *m);
-/* No reference counterpart (the reference's CSR exposes no layout): out[7] =
+/* No reference counterpart (the reference's CSR exposes no layout): out[8] =
  * {storage format (KMSD_CSR/KMSD_BSR4), n, nnz as given, stored entries
  * (off-diagonal or block entries), device bytes, value kind (0 real,
  * 1 imaginary, 2 complex), uniform values (1: ELL storage whose stored
  * off-diagonal values, padding aside, are all equal: the width-3/4/6 ELL
- * kernel reads the value once)}. */
+ * kernel reads the value once), plane stride (BSR4: block rows between a
+ * block row and its outermost stencil neighbour, which products walk in
+ * tiles across planes; 0: natural order)}. */
 int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out);
 /* Reference: CSR.matmul(B) (cpp/native.cpp:75-89, 642-646): out = A B. */
 int kmsd_matrix_matmul(kmsd_matrix *m, const void *B, void *out, int64_t nb, int dtype);

@@ -332,11 +332,11 @@ class HipMatrix:
         self.host = m
 
     def info(self):
-        v = np.zeros(7, np.int64)
+        v = np.zeros(8, np.int64)
         self.ctx.check(self.ctx.lib.kmsd_matrix_info(self.h, _np_ptr(v)))
         return dict(format={KMSD_CSR: "csr", KMSD_BSR4: "bsr4"}[int(v[0])], n=int(v[1]), nnz=int(v[2]),
                     stored=int(v[3]), device_bytes=int(v[4]), values=("real", "imag", "complex")[int(v[5])],
-                    uniform_values=bool(v[6]))
+                    uniform_values=bool(v[6]), plane_stride=int(v[7]))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

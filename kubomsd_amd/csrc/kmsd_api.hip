@@ -156,6 +156,9 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             c->ell_kernel = value != 0;
         } else if (k == "fuse_velocity") {
             c->fuse_velocity = value != 0;
+        } else if (k == "tile_bytes") {
+            if (value < 0 || value > (int64_t(1) << 30)) throw Error(KMSD_EINVAL, "tile_bytes must be in [0, 2^30]");
+            c->tile_bytes = value;
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
@@ -341,6 +344,7 @@ extern "C" int kmsd_matrix_info(const kmsd_matrix *m, int64_t *out) {
     out[4] = int64_t(m->bytes);
     out[5] = m->vkind;
     out[6] = m->uniform_slot >= 0 ? 1 : 0;
+    out[7] = m->plane;
     return KMSD_OK;
 }
 

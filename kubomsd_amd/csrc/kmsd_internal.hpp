@@ -29,6 +29,7 @@ struct kmsd_ctx {
     int pcg_poll = 4;       // PCG iterations between host status polls
     bool graphs = true;     // replay moment-chain chunks as HIP graphs (identity metric)
     bool ell_kernel = true; // fixed-width ELL products (widths 3, 4, 6) use the prefetching kernel
+    int64_t tile_bytes = 1 << 20; // BSR4 plane-tiled walk: gathered-block bytes per tile (0: natural order)
     bool fuse_velocity = true; // kmsd_chain_v forms V T_k inside the chain's steps where V fits H's ELL slots
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
@@ -71,6 +72,11 @@ struct kmsd_matrix {
     void *dval64 = nullptr;  // diagonal values (n)
     void *dval32 = nullptr;
     bool identity = false;   // exactly the identity matrix
+    // BSR4: block-row stride of the matrix's plane neighbours (a 3D stencil's
+    // outermost stride), 0 when there is none.  Products then walk the block
+    // rows tile by tile across planes (kmsd::tile_row), so the +-plane rows a
+    // block row gathers were fetched a tile earlier and are still in L2.
+    int32_t plane = 0;
     size_t bytes = 0;        // device bytes held
 };
 

@@ -93,7 +93,24 @@ struct StepArgs {
     int64_t ldv;
     double valpha_re, valpha_im;
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
+    uint32_t tile_plane;  // BSR4: > 0 walks block rows plane-tiled (tile_row), block rows per plane
+    uint32_t tile_rows;   // block rows per tile within a plane
 };
+
+// Block row at walk position pos of the plane-tiled walk: the nq block rows
+// form nq/P planes of P; the walk takes tile t (block rows [tT, tT + T) of
+// every plane, the last tile narrower when T does not divide P) plane after
+// plane, then the next tile.  A block row's +-P neighbours are then T walk
+// positions away instead of P.  Requires nq % P == 0 and nq < 2^32.
+__device__ __forceinline__ uint32_t tile_row(uint32_t pos, uint32_t nq, uint32_t P, uint32_t T) {
+    const uint32_t planes = nq / P, full = P / T, span = T * planes;
+    if (pos < full * span) {
+        const uint32_t t = pos / span, w = pos - t * span, x = w / T;
+        return x * P + t * T + (w - x * T);
+    }
+    const uint32_t rem = P - full * T, w = pos - full * span, x = w / rem;
+    return x * P + full * T + (w - x * rem);
+}
 
 template <class A>
 __device__ __forceinline__ cx<A> cmul(double ar, double ai, cx<A> x) {
@@ -608,10 +625,11 @@ __global__ __launch_bounds__(256) void bsr4_step_kernel(const StepArgs a) {
     const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
 
     for (int64_t blk = b0 + slot; blk < b1; blk += per) {
-        const int64_t q = blk * qpb + qloc;
+        const int64_t pos = blk * qpb + qloc;
         // The whole lane group of a block row leaves together; on the DPP path
         // lanes past the last column stay to share their block values.
-        if (q >= nq || (!dpp && !colok)) continue;
+        if (pos >= nq || (!dpp && !colok)) continue;
+        const int64_t q = a.tile_plane ? int64_t(tile_row(uint32_t(pos), uint32_t(nq), a.tile_plane, a.tile_rows)) : pos;
         const int64_t row0 = q * 4;
         cx<AT> y[4][CPL];
 #pragma unroll

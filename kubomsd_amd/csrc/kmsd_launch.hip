@@ -132,6 +132,16 @@ bool bsr4_upload(kmsd_ctx *c, kmsd_matrix *m, bool want_auto, int64_t n, int64_t
     m->idx = upload(c, m, bidx);
     m->val64 = upload(c, m, v64);
     m->val32 = upload(c, m, v32);
+    // Plane stride: the largest positive block offset that at least a quarter
+    // of the block rows have and that divides the block rows into planes (a
+    // 3D stencil's outermost neighbour).  Small strides need no tiling.
+    std::unordered_map<int32_t, int64_t> offsets;
+    for (int64_t q = 0; q < nq; ++q)
+        for (int32_t k = bptr[q]; k < bptr[q + 1]; ++k)
+            if (bidx[k] > q) ++offsets[int32_t(bidx[k] - q)];
+    m->plane = 0;
+    for (const auto &o : offsets)
+        if (o.first >= 256 && 4 * o.second >= nq && nq % o.first == 0 && o.first > m->plane) m->plane = o.first;
     HIPCHECK(hipStreamSynchronize(c->stream));
     return true;
 }
@@ -453,6 +463,13 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
         a.val = f32 ? A->val32 : A->val64;
         a.dval = f32 ? A->dval32 : A->dval64;
         a.uniform_slot = (!bsr && A->ell) ? A->uniform_slot : -1;
+        if (bsr && A->plane && c->tile_bytes > 0) {
+            const int64_t rows = c->tile_bytes / (4 * int64_t(s.w) * int64_t(elem_bytes(dtype)));
+            if (rows >= 1 && rows < A->plane) {
+                a.tile_plane = uint32_t(A->plane);
+                a.tile_rows = uint32_t(rows);
+            }
+        }
         if (s.VOUT) {
             fl |= F_VOUT;
             a.vval = s.vval;

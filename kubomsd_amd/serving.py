@@ -352,8 +352,8 @@ class TransportServer:
 
 
 def _worker_main(conn, device, budget, keep_blocks):
-    """One serving slot: a process with its own HIP context, running the jobs
-    it receives one at a time (None ends it)."""
+    """One serving slot: a process with its own HIP context, running the work
+    it receives (a list of jobs, through run_jobs; None ends it)."""
     import torch
     from .backends.hip import HipContext
     torch.cuda.set_device(device)

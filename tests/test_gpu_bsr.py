@@ -136,3 +136,39 @@ def test_autotune_applies_and_caches_shape(hip_ctx, tmp_path):
     assert rel_err(got, m.host @ B.astype(np.complex64).astype(np.complex128)) < 2e-6
     hip_ctx.set_tuning("cpl", 0)  # restore the session defaults for later tests
     hip_ctx.set_tuning("groups", 0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_plane_tiled_walk_equals_natural_order(hip_ctx, dtype):
+    """A 16^3 simple-cubic block matrix has a plane stride of 256 block rows;
+    products walk it in tiles across planes.  Tiles of 48 block rows (five
+    full tiles and a 16-row remainder per plane) and the natural order give
+    bitwise the same products (each row's sum keeps its order) and S^-1 H
+    moments within rounding of each other and of the oracle."""
+    from types import SimpleNamespace
+    from kubomsd_amd.backends.hip import HipEngine, HipMatrix, HipMetric
+    m = cubic_orbital_model(16, seed=4412)
+    n, w = m.n, 8
+    eb = 16 if dtype == "c128" else 8
+    B = random_block(n, w, 21)
+    H = HipMatrix(hip_ctx, m.H)
+    assert H.info()["format"] == "bsr4" and H.info()["plane_stride"] == 256
+    bound = 1.02 * float(abs(m.H).sum(axis=1).max()) / 0.25  # lambda_min(S) > 0.3 for this model
+    outs, moms = [], []
+    try:
+        for tile in (0, 48 * 4 * w * eb, 1 << 20):
+            hip_ctx.set_tuning("tile_bytes", tile)
+            outs.append(host(H.matmul(to_dev(hip_ctx, B, dtype))))
+            S = HipMatrix(hip_ctx, m.S)
+            eng = HipEngine(hip_ctx, H, HipMetric(hip_ctx, S, m.S.diagonal().real), [(-1, [])] * 3, 1e-12, 0.0,
+                            1000)
+            moms.append(host(eng.moments(to_dev(hip_ctx, B[:, :4], dtype), 24, 0.0, bound)))
+    finally:
+        hip_ctx.set_tuning("tile_bytes", 1 << 20)
+    np.testing.assert_array_equal(outs[1], outs[0])
+    np.testing.assert_array_equal(outs[2], outs[0])
+    tol = 1e-12 if dtype == "c128" else 5e-6
+    assert rel_err(moms[1], moms[0]) < tol and rel_err(moms[2], moms[0]) < tol
+    Bq = B[:, :4].astype(np.complex64).astype(complex) if dtype == "c64" else B[:, :4]
+    want = oracle_engine(SimpleNamespace(H=m.H, S=m.S, velocity={})).moments(Bq, 24, 0.0, bound)
+    assert rel_err(moms[1], want) < tol
