@@ -75,6 +75,9 @@ def parse():
     p.add_argument("--dt", type=float, default=1.0, help="stepwise: output time step (fs)")
     p.add_argument("--cpu-vectors", type=int, default=2,
                    help="square: vectors of the CPU reference run (the full estimator on the first ones)")
+    p.add_argument("--method", choices=["stepwise", "spectral"], default="stepwise",
+                   help="square: the stochastic estimator (spectral: VAC by segmented Chebyshev products, "
+                        "the mu tables on the FP64 matrix cores)")
     p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise", "square"],
                    default="honeycomb",
                    help="honeycomb: orthogonal moment chain (default); nonorthogonal: simple-cubic 64^3 x 4 "
@@ -515,9 +518,23 @@ class _RefStepwiseOps:
         defs = [(0, [csr(model.velocity["x"])]), (-1, []), (-1, [])]
         self.eng = ref.Engine(csr(model.H), self.metric, defs, 1e-10, 0.0, 1000, 0.6582119569509067)
         self.native = self
+        self.host_S = sp.csr_matrix(model.S, dtype=np.complex128)
+        self.n = model.n
 
     def moments(self, B, length, center, half):
         return self.eng.moments(np.ascontiguousarray(B), int(length), center, half)
+
+    def chain(self, B, length, center, half):
+        return self.eng.chain(np.ascontiguousarray(B), int(length), center, half)
+
+    def inner(self, A, B):
+        return np.sum(A.conj() * (self.host_S @ B), axis=0)
+
+    def apply_v_cov(self, direction, B):
+        return self.eng.v_cov(np.ascontiguousarray(B), [float(v) for v in direction])
+
+    def solve(self, B):
+        return self.metric.solve(np.ascontiguousarray(B), 1e-10, 0.0, 1000)[0]
 
     def series(self, B, coeff, center, half, directions=()):
         return self.eng.series(np.ascontiguousarray(B), np.ascontiguousarray(coeff, np.complex128), center, half,
@@ -540,6 +557,7 @@ def square(args):
     from kubomsd_amd.contracts import ModelBundle
     from kubomsd_amd.metric.bounds import certified_bounds
     from kubomsd_amd.operators.velocity import build_operators
+    from kubomsd_amd.solvers.spectral import spectral_samples
     from kubomsd_amd.solvers.stepwise import stepwise_samples
     from kubomsd_amd.solvers.stochastic import iter_stochastic_blocks
     from kubomsd_amd.synthetic import square_lattice
@@ -551,26 +569,33 @@ def square(args):
     E = np.linspace(-4.0, 4.0, 161)
     T = np.round(np.arange(0.0, 10.0 + 0.25, 0.5), 12)
     samples = random_phases(n, SEED, range(nb))
-    cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method="stepwise",
+    method = args.method
+    cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method=method,
                     energy_eV=E, time_fs=T, axes=("x",), spectral_bounds=(-5.1, 5.1), n_moments=M, nsamples=nb,
                     sample_batch=nb, statistics_block=8, include_vac=False, precision="fp64", autotune=False)
     ops = build_operators(ModelBundle(model.H, model.S, velocity={"x": model.velocity["x"]}), cfg, ctx)
     report = certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower, spectral_bounds=cfg.spectral_bounds)
-    list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=samples, bounds_report=report))  # warm-up
+    list(iter_stochastic_blocks(ops, cfg, method, samples=samples, bounds_report=report))  # warm-up
     stream = torch.cuda.current_stream()
     times = []
     for _ in range(max(1, args.steps)):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        blocks = list(iter_stochastic_blocks(ops, cfg, "stepwise", samples=samples, bounds_report=report))
+        blocks = list(iter_stochastic_blocks(ops, cfg, method, samples=samples, bounds_report=report))
         e1.record(stream)
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1))
     ms = float(np.median(times))
     diag = blocks[0].metadata["sample_batch_diagnostics"][0]
     K, nt = diag["max_propagation_order"], len(T)
-    # column-steps: DOS chain, then per interval the augmented series (width 2 nb) and the axis chain
-    vsteps = (M - 1) * nb + (nt - 1) * (K * 2 * nb + (M - 1) * nb)
+    if method == "stepwise":
+        # column-steps: DOS chain, then per interval the augmented series (width 2 nb) and the axis chain
+        vsteps = (M - 1) * nb + (nt - 1) * (K * 2 * nb + (M - 1) * nb)
+    else:
+        # column-steps: DOS chain; per segment the left chain (M + K orders), the velocity chain (K + 1)
+        # and, from the second segment on, the anchors' propagation (width 2 nb, K orders)
+        seg = diag["segments"]
+        vsteps = (M - 1) * nb + seg * ((M + K - 1) * nb + K * nb) + (seg - 1) * K * 2 * nb
     print(f"# GPU: whole estimator {ms:.1f} ms (median of {len(times)}), order {K}", file=sys.stderr, flush=True)
 
     # Reference CPU kernel on the first cpu_vectors samples, and the GPU's
@@ -586,13 +611,16 @@ def square(args):
         cops, kind = _RefStepwiseOps(ref, model, threads), "reference"
     else:
         cops, kind = kpm.Ops(model.H, model.S, {"x": model.velocity["x"]}, rtol=1e-10, threads=threads), "port"
+    run_cpu = kpm.stepwise_samples if method == "stepwise" else kpm.spectral_samples
+    run_gpu = stepwise_samples if method == "stepwise" else spectral_samples
+    key = "msd_A2/commutator" if method == "stepwise" else "msd_A2/vac_integral"
     t0 = time.perf_counter()
-    cdos, cnums = kpm.stepwise_samples(cops, kcfg, samples[:, :cb])
+    cdos, cnums = run_cpu(cops, kcfg, samples[:, :cb])
     cpu_s = time.perf_counter() - t0
     cpu_vsteps = vsteps * cb / nb
-    gdos, gnums, _ = stepwise_samples(ops, cfg, ops.asarray(samples[:, :cb]))
-    g = gnums["msd_A2/commutator"].cpu().numpy()
-    c = cnums["msd_A2/commutator"]
+    gdos, gnums, _ = run_gpu(ops, cfg, ops.asarray(samples[:, :cb]))
+    g = gnums[key].cpu().numpy()
+    c = cnums[key]
     parity = {"msd_numerators_max_rel": float(np.max(np.abs(g - c)) / np.max(np.abs(c))),
               "dos_max_rel": float(np.max(np.abs(gdos.cpu().numpy() - cdos)) / np.max(np.abs(cdos))),
               "samples_compared": cb}
@@ -604,15 +632,17 @@ def square(args):
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic (seeded square lattice, NumPy PCG64 random phases)",
         "config": {"workload": f"square {side}x{side} periodic (N={n}), t=-1 eV, W=2 eV, {nb} vectors, M={M}, "
-                               f"stepwise MSD along x, t=0..10 fs every 0.5 fs, 161 energies, FP64, one whole "
-                               f"estimator run per step", "n": n, "vectors": nb, "moments": M, "times": nt,
-                   "precision": "fp64", "parallelism": "single GPU"},
+                               + ("stepwise MSD" if method == "stepwise" else
+                                  "spectral VAC (segments of 1 fs, integration step 0.01 fs) MSD")
+                               + " along x, t=0..10 fs every 0.5 fs, 161 energies, FP64, one whole estimator run "
+                                 "per step", "n": n, "vectors": nb, "moments": M, "times": nt,
+                   "method": method, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                      "traffic": None, "kernel": "launch-bound at this size (N = 10^4: about 2.5 MB per block)"},
         "parity_vs_reference_cpu": parity,
         "cpu_baseline": {"value": cpu_vsteps / cpu_s, "unit": "vector-steps/s", "cores": threads, "kind": kind,
-                         "sample": f"the whole stepwise estimator (DOS chain, {nt - 1} series steps and MSD "
-                                   f"chains) on the first {cb} of the {nb} vectors, one run, OMP threads={threads}"},
+                         "sample": f"the whole {method} estimator on the first {cb} of the {nb} vectors, one run, "
+                                   f"OMP threads={threads}"},
     }
     line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line))

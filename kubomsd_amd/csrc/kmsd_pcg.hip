@@ -272,9 +272,12 @@ struct Outcome {
 
 // PCG in the solver's own precision.  B (ldb) is any block of that dtype;
 // the solution (times the column scale) lands in OUT (ldo).  Never throws on
-// numerical failure: the caller inspects the outcome.
+// numerical failure: the caller inspects the outcome.  final_check: the
+// closing true-residual acceptance (report fields tn, convergence); the
+// inner solves of the FP64 refinement skip it, their acceptance is the
+// refinement's FP64 residual.
 Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, double rtol, double atol,
-                int maxiter) {
+                int maxiter, bool final_check = true) {
     kmsd_metric *m = S.m;
     kmsd_ctx *c = S.c;
     const int32_t w = S.w;
@@ -298,7 +301,9 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     // z = M^-1 R and both reductions in one pass, P update recomputing z).
     const bool fusable = !m->adaptive && !m->ic && (m->nblocks == 0 || m->uniform4);
     for (int k = 0; k < maxiter; ++k) {
-        if (k % poll == 0) {
+        // (iterations of columns that are done, or of a broken solve, are
+        // gated off on the device; the poll only ends the loop)
+        if (k % poll == 0 && (k > 0 || poll == 1)) {
             const PcgFlags h = S.read_flags();
             if (h.broke || h.any_active == 0) break;
         }
@@ -332,9 +337,11 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
         S.row(PS_PUPDATE, nullptr, 0, nullptr, 0, true);
     }
     // final acceptance on the original (scaled) equation
-    S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
-    S.row(PS_TRUEGAP);
-    S.col(PC_FINAL, S.part, S.rs.gx, rtol, atol);
+    if (final_check) {
+        S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
+        S.row(PS_TRUEGAP);
+        S.col(PC_FINAL, S.part, S.rs.gx, rtol, atol);
+    }
     S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
     Outcome o;
     o.cs.resize(w);
@@ -591,7 +598,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
                            int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
         HIPCHECK(hipGetLastError());
         Solver S = make_solver(m, KMSD_C64, nb);
-        const Outcome o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter);
+        const Outcome o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter, false);
         for (int32_t j = 0; j < nb; ++j) {
             agg[j].iters += o.cs[j].iters;
             agg[j].breakdown |= o.cs[j].breakdown;
