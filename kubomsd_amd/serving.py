@@ -315,14 +315,18 @@ class TransportServer:
         with self.cv:
             self.closed = False
         self.t0 = time.perf_counter()
+        ordered = sorted(jobs, key=lambda j: j.arrival_s)
+        if not realtime:  # every job is waiting before the first slot takes work
+            for job in ordered:
+                self.submit(job)
+            ordered = []
         threads = [threading.Thread(target=self._dispatch, args=(s,), daemon=True) for s in range(self.slots)]
         for t in threads:
             t.start()
-        for job in sorted(jobs, key=lambda j: j.arrival_s):
-            if realtime:
-                delay = job.arrival_s - self.clock()
-                if delay > 0:
-                    time.sleep(delay)
+        for job in ordered:
+            delay = job.arrival_s - self.clock()
+            if delay > 0:
+                time.sleep(delay)
             self.submit(job)
         with self.cv:
             self.closed = True
