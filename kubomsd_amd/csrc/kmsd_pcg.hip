@@ -540,8 +540,8 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     ColState *cs2 = static_cast<ColState *>(wk.cs2.ensure(c, sizeof(ColState) * size_t(nb)));
     double *nrm = static_cast<double *>(wk.part2.ensure(c, size_t(std::max<int64_t>(c->groups, 8)) * nb * 16));
     launch_convert(c, KMSD_C64, B, ldb, KMSD_C128, bb, nb, n, nb);
-    launch_scale(c, KMSD_C128, bb, nb, yy, nb, n, nb, Cplx{0, 0});
-    launch_scale(c, KMSD_C128, bb, nb, rr, nb, n, nb, Cplx{1, 0});
+    // y = 0 and r = b are implicit in the first round: it reads b as its
+    // residual and assigns y (zeroed here only if no round runs).
     // ||b|| per column (FP64) -> threshold
     auto colnorm = [&](const void *X) {
         Step s;
@@ -572,6 +572,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.safe = 1.0;
     }
     int64_t checks = 1;
+    int rounds = 0;
     for (int outer = 0; outer < 12; ++outer) {
         bool done = true;
         // The inner solve reduces every column's residual by at least
@@ -594,8 +595,8 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         }
         copy_h2d(c, cs2, sc.data(), sizeof(ColState) * nb);
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
-        hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream, rr,
-                           int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
+        hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream,
+                           rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
         HIPCHECK(hipGetLastError());
         Solver S = make_solver(m, KMSD_C64, nb);
         const Outcome o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter, false);
@@ -607,7 +608,8 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         account(m, o, 1);
         --m->calls;
         hipLaunchKernelGGL((scaled_convert_kernel<float, double>), dim3(unsigned(g)), dim3(256), 0, c->stream, d64,
-                           int64_t(nb), yy, int64_t(nb), n, nb, cs2, 1);
+                           int64_t(nb), yy, int64_t(nb), n, nb, cs2, rounds ? 1 : 2);
+        ++rounds;
         HIPCHECK(hipGetLastError());
         // r = b - S y in FP64
         Step s;
@@ -631,6 +633,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         if (brk) break;
     }
     ++m->calls;
+    if (!rounds) launch_scale(c, KMSD_C128, bb, nb, yy, nb, n, nb, Cplx{0, 0});
     launch_convert(c, KMSD_C128, yy, nb, KMSD_C64, Y, ldy, n, nb);
     // finiteness of the stored solution per column
     bool fin_all = !launch_nonfinite(c, KMSD_C64, Y, ldy, n, nb);

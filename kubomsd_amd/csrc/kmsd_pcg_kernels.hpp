@@ -506,6 +506,8 @@ __global__ __launch_bounds__(256) void scaled_convert_kernel(const void *Xv, int
         if (mode == 0) {  // O = X / safe
             const double inv = 1.0 / cs[c].safe;
             O[r * ldo + c] = to<VB>(cx<double>{x.re * inv, x.im * inv});
+        } else if (mode == 2) {  // O = X * safe
+            O[r * ldo + c] = to<VB>(cx<double>{x.re * cs[c].safe, x.im * cs[c].safe});
         } else {  // O += X * safe
             cx<double> o = to<double>(O[r * ldo + c]);
             o.re += x.re * cs[c].safe;

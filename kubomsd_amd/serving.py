@@ -32,12 +32,15 @@ per-job budget check here generalizes to many jobs.
 """
 from collections import deque
 from dataclasses import dataclass, field
+import logging
 import threading
 import time
 
 import numpy as np
 
 from .errors import ResourceBlocked
+
+log = logging.getLogger(__name__)
 
 
 @dataclass
@@ -377,8 +380,9 @@ def run_jobs(group, ctx, keep_blocks=False):
     if len(group) > 1:
         try:
             return run_job_group(group, ctx, keep_blocks)
-        except Exception:  # noqa: BLE001 -- the jobs are then run one by one, each reporting its own error
-            pass
+        except Exception as exc:  # noqa: BLE001 -- the jobs are then run one by one, each reporting its own error
+            log.warning("shared launches for jobs %s failed (%s: %s); running them one by one",
+                        [j.job_id for j in group], type(exc).__name__, exc)
     outs = []
     for job in group:
         try:
