@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--tile-bytes", type=int, default=None,
                    help="nonorthogonal: gathered-block bytes per tile of the plane-tiled 4x4-block walk "
                         "(0: natural row order; default: the library's)")
+    p.add_argument("--no-bsr-mfma", action="store_true",
+                   help="nonorthogonal: FP32-valued 4x4-block products on the VALU kernel instead of the "
+                        "matrix cores")
     p.add_argument("--moment-doubling", action="store_true",
                    help="serving/stepwise: orthogonal jobs take moments from the product identities "
                         "(errors are still reported against the FP64 three-term recursion)")
@@ -209,6 +212,8 @@ def nonorthogonal(args):
         ctx.set_tuning("pcg_poll", args.pcg_poll)
     if args.tile_bytes is not None:
         ctx.set_tuning("tile_bytes", args.tile_bytes)
+    if args.no_bsr_mfma:
+        ctx.set_tuning("bsr_mfma", 0)
     t_build = time.perf_counter()
     model = cubic_orbital_model(args.atoms)
     n, nb, M = model.n, args.vectors, args.moments

@@ -101,7 +101,12 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * that prefetches the next row's indices; 0: the generic row kernel; same
  * results either way), "fuse_velocity" (1, the default: kmsd_chain_v forms
  * V T_k inside the chain's steps where it can; 0: always the regrouping path;
- * same results either way). */
+ * same results either way), "tile_bytes" (4x4-block products of matrices
+ * with a plane stride walk the block rows in tiles of this many gathered
+ * bytes across planes; 0: natural order; same results either way),
+ * "bsr_mfma" (1, the default: FP32-valued 4x4-block products of complex64
+ * blocks, the metric products of complex64 inner solves, run on the matrix
+ * cores with FP32 row sums; 0: the VALU kernel with FP64 row sums). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an

@@ -95,6 +95,9 @@ struct StepArgs {
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
     uint32_t tile_plane;  // BSR4: > 0 walks block rows plane-tiled (tile_row), block rows per plane
     uint32_t tile_rows;   // block rows per tile within a plane
+    // bsr4_mfma_kernel: byte extents of X, the block values and the block
+    // columns (buffer descriptors; reads past them return zero)
+    uint32_t xbytes, vbytes, ibytes;
 };
 
 // Block row at walk position pos of the plane-tiled walk: the nq block rows
@@ -702,6 +705,168 @@ __global__ __launch_bounds__(256) void bsr4_step_kernel(const StepArgs a) {
 
     if (!(fl & (F_MOM | F_MOM2))) return;
     step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, blockIdx.x);
+}
+
+// FP32-valued 4x4-block product on complex64 blocks with the block products
+// on the matrix cores (the PCG metric products of complex64 inner solves).
+// Flags: F_SPMM with F_STORE, F_MOM and F_P only; real (or imaginary-only)
+// values; w a multiple of 4.
+//
+// Lane layout: lanes per block row lpr = 2^lpr_log2 in {4, 8, 16}, lane g of
+// a block row owns complex columns 4g..4g+3 of its four rows (grid.y walks
+// column chunks of 4*lpr).  Each lane quad (one block row, 16 columns) is
+// one block of v_mfma_f32_4x4x1_16b: for block column cc, lane 4b+i supplies
+// S[i][cc] as the A operand and its own X[4j+cc][col].re/.im as the B
+// operand (one instruction per real column t of the lane), and D[i][n]
+// lands in lane 4b+n, register i: the lane's own output rows.  Sums are
+// FP32 over the whole block row.
+//
+// A block row's column indices and values are staged in LDS per chunk of
+// KC = 10*lpr/4 blocks by buffer-to-LDS loads issued together (one memory
+// latency per chunk), so a block waits only for its X rows, which are read
+// by buffer loads with 32-bit offsets.  LDS per wavefront: 10 x 1 KB of
+// values (instruction i holds blocks kc + i*lpr/4 .. of every block row,
+// 16 bytes per lane) and 3 x 256 B of indices.
+// The same XCD-aware, optionally plane-tiled, block row walk as
+// bsr4_step_kernel; partials [gridDim.x][w] in the same format.
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, int(bytes), 0x00020000);
+}
+
+static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a) {
+    using V = cx<float>;
+    constexpr int CPL = 4, NV = 10, NI = 3;
+    using PK = Pack<float, CPL>;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (a.gate && *a.gate == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lg = a.lpr_log2, lpr = 1 << lg;
+    const int qpw = 64 >> lg, qpb = 4 * qpw;
+    const int qw = lane >> lg;
+    const int cgrp = lane & (lpr - 1);
+    const int col = (int(blockIdx.y) * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+    const int ri = lane & 3;
+    const int bpi = lpr >> 2, kc_max = NV * bpi;  // blocks per value instruction and row; chunk length
+    const V *P = static_cast<const V *>(a.P);
+    const V *W = static_cast<const V *>(a.W);
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(a.X, a.xbytes), vr = buffer_rsrc(a.val, a.vbytes),
+                                 ir = buffer_rsrc(a.idx, a.ibytes);
+    const uint32_t fl = a.flags;
+    __shared__ f4 svals[4][NV][64];
+    __shared__ int32_t sidx[4][NI][64];
+    cx<double> mom[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) mom[c] = {0.0, 0.0};
+    const int64_t nq = a.n >> 2;
+    const int64_t nblk = (nq + qpb - 1) / qpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+    const uint32_t rowb = uint32_t(a.ldx) * 8u, colb = uint32_t(col) * 8u;
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t pos = blk * qpb + wave * qpw + qw;
+        const bool valid = pos < nq;
+        const int64_t q =
+            !valid ? 0 : (a.tile_plane ? int64_t(tile_row(uint32_t(pos), uint32_t(nq), a.tile_plane, a.tile_rows)) : pos);
+        int32_t k0 = 0, cnt = 0;
+        if (valid) {
+            k0 = a.ptr[q];
+            cnt = a.ptr[q + 1] - k0;
+        }
+        int kmax = cnt;  // longest block row of the wavefront: the MFMAs need every lane
+        for (int off = lpr; off < 64; off <<= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+        f4 d[2 * CPL];
+#pragma unroll
+        for (int t = 0; t < 2 * CPL; ++t) d[t] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int kc = 0; kc < kmax; kc += kc_max) {
+            // The previous chunk's LDS reads are done before it is overwritten.
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (lds_void *)&sidx[wave][i][0], 4,
+                                                         uint32_t(k0 + kc + i * lpr + cgrp) * 4u, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(vr, (lds_void *)&svals[wave][i][0], 16,
+                                                         uint32_t(k0 + kc + i * bpi) * 64u + uint32_t(cgrp) * 16u, 0,
+                                                         0, 0);
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            const int kend = kmax - kc < kc_max ? kmax - kc : kc_max;
+            for (int kq = 0; kq < kend; ++kq) {
+                const bool on = kc + kq < cnt;
+                const int32_t j = sidx[wave][kq >> lg][qw * lpr + (kq & (lpr - 1))];
+                const uint32_t xo = uint32_t(j) * 4u * rowb + colb;
+                PK x[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    x[cc].v = {};
+                    if (on && colok) {
+                        const f4 lo = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(xr, xo + cc * rowb, 0, 0));
+                        const f4 hi =
+                            __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(xr, xo + cc * rowb + 16, 0, 0));
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            x[cc].v[e] = lo[e];
+                            x[cc].v[4 + e] = hi[e];
+                        }
+                    }
+                }
+                const int bi = kq / bpi, bb = kq - bi * bpi;
+                f4 s = svals[wave][bi][qw * lpr + bb * 4 + ri];
+                if (!on) s = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                    for (int t = 0; t < 2 * CPL; ++t)
+                        d[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(s[cc], x[cc].v[t], d[t], 0, 0, 0);
+            }
+        }
+        if (!valid || !colok) continue;
+        const int64_t row0 = q * 4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            PK xi, pv, wv;
+            cx<double> y[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) y[c] = {double(d[2 * c][r]), double(d[2 * c + 1][r])};
+            if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
+            if (fl & F_MOM) wv.load_nt(W + (row0 + r) * a.ldw + col);
+            step_epilogue<float, double, CPL>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
+        }
+    }
+    if (!(fl & F_MOM)) return;
+    // Lanes of one column (lane bits >= lpr_log2) by shuffles, the four
+    // wavefronts through LDS, one partial per (workgroup, column).
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+        for (int off = lpr; off < 64; off <<= 1) {
+            mom[c].re += __shfl_xor(mom[c].re, off);
+            mom[c].im += __shfl_xor(mom[c].im, off);
+        }
+    __shared__ cx<double> red[4][16 * CPL];
+    if (lane < lpr) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = mom[c];
+    }
+    __syncthreads();
+    if (wave == 0 && lane < lpr && colok) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (col + c >= a.w) continue;
+            cx<double> s = red[0][lane * CPL + c];
+            for (int w2 = 1; w2 < 4; ++w2) {
+                s.re += red[w2][lane * CPL + c].re;
+                s.im += red[w2][lane * CPL + c].im;
+            }
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
+        }
+    }
 }
 
 // out[s][c] = sum_g part[s][g][c] in a fixed tree order (deterministic).

@@ -294,6 +294,7 @@ struct KernelKey {
     int ell = 0;         // ELL width with the prefetching kernel, 0 = generic kernel
     int lpr_log2 = 6;
     bool vout = false;   // the ELL kernel also forms V X (FP64 values and row sums, real A)
+    bool mfma = false;   // bsr4_mfma_kernel (FP32-valued 4x4 blocks on the matrix cores)
 };
 
 template <class VT, class MR, bool MC, class AT, int CPL>
@@ -435,6 +436,38 @@ int cpl_cap_for(int dtype, std::initializer_list<std::pair<const void *, int64_t
 
 namespace {
 
+// The matrix-core 4x4-block kernel for an FP32-valued BSR4 product of a
+// complex64 block when its flags, widths and byte extents allow it (buffer
+// offsets are 32-bit); otherwise the launch keeps the VALU kernel.
+void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, StepArgs &a, Shape &sh, KernelFn &fn,
+                KernelKey &key) {
+    if (fl & ~(F_SPMM | F_STORE | F_MOM | F_P)) return;
+    if (s.w % 4 || cpl_limit(s, KMSD_C64) < 4) return;
+    const int64_t lim = (int64_t(1) << 31) - (int64_t(1) << 16);
+    const int64_t nnzb = A->noff / 16;
+    const int64_t xbytes = s.n * s.ldx * 8;
+    if (xbytes > lim || nnzb * 64 > lim) return;
+    const int lg = std::max(2, std::min(4, ilog2_ceil((s.w + 3) / 4)));
+    const int64_t lpr = int64_t(1) << lg;
+    fn = bsr4_mfma_kernel;
+    key.mfma = true;
+    sh.cpl = 4;
+    sh.lpr_log2 = lg;
+    sh.gy = int((s.w + 4 * lpr - 1) / (4 * lpr));
+    if (!s.groups) {
+        const int64_t qpb = 4 * (64 / lpr);
+        const int64_t nblk = (s.n / 4 + qpb - 1) / qpb;
+        const int64_t want_g = ((nblk + 7) / 8) * 8;
+        int64_t g = c->groups;
+        if (c->auto_groups) g = std::max<int64_t>(8, (int64_t(c->cus) * resident_per_cu(fn) / sh.gy) / 8 * 8);
+        sh.gx = std::max<int64_t>(8, std::min<int64_t>(g, want_g));
+    }
+    a.lpr_log2 = lg;
+    a.xbytes = uint32_t(xbytes);
+    a.vbytes = uint32_t(nnzb * 64);
+    a.ibytes = uint32_t(nnzb * 4);
+}
+
 // Kernel arguments, launch shape and kernel of one fused step.
 void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh, KernelFn &fn, KernelKey &key) {
     const kmsd_matrix *A = s.A;
@@ -486,6 +519,7 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.W) fl |= F_MOM;
     if (s.doubling) fl |= F_MOM2;
     if (s.ACC) fl |= F_ACC;
+    if (bsr && f32 && A->vkind != VK_COMPLEX && c->bsr_mfma) mfma_shape(c, s, A, fl, a, sh, fn, key);
     a.X = s.X;
     a.Z = s.Z;
     a.P = s.P;
