@@ -58,6 +58,11 @@ def parse():
     p.add_argument("--tile-bytes", type=int, default=None,
                    help="nonorthogonal: gathered-block bytes per tile of the plane-tiled 4x4-block walk "
                         "(0: natural row order; default: the library's)")
+    p.add_argument("--sample-batch", type=int, default=0,
+                   help="square: samples per batch (0: all vectors in one batch)")
+    p.add_argument("--block-width-autotune", action="store_true",
+                   help="square: let the estimator pick the batch width from measured step times, starting "
+                        "from --sample-batch")
     p.add_argument("--no-bsr-mfma", action="store_true",
                    help="nonorthogonal: FP32-valued 4x4-block products on the VALU kernel instead of the "
                         "matrix cores")
@@ -577,7 +582,8 @@ def square(args):
     method = args.method
     cfg = RunConfig(solver="stochastic-kpm", kernel="jackson", backend="hip", stochastic_method=method,
                     energy_eV=E, time_fs=T, axes=("x",), spectral_bounds=(-5.1, 5.1), n_moments=M, nsamples=nb,
-                    sample_batch=nb, statistics_block=8, include_vac=False, precision="fp64", autotune=False)
+                    sample_batch=args.sample_batch or nb, statistics_block=8, include_vac=False, precision="fp64",
+                    autotune=False, block_width_autotune=args.block_width_autotune)
     ops = build_operators(ModelBundle(model.H, model.S, velocity={"x": model.velocity["x"]}), cfg, ctx)
     report = certified_bounds(ops.host_H, ops.host_S, ops.metric.s_lower, spectral_bounds=cfg.spectral_bounds)
     list(iter_stochastic_blocks(ops, cfg, method, samples=samples, bounds_report=report))  # warm-up
@@ -592,6 +598,8 @@ def square(args):
         times.append(e0.elapsed_time(e1))
     ms = float(np.median(times))
     diag = blocks[0].metadata["sample_batch_diagnostics"][0]
+    batch_info = {"sample_batch": cfg.sample_batch, "admitted": blocks[0].metadata["admitted_sample_batch"],
+                  "width_tuning": blocks[0].metadata.get("block_width_tuning")}
     K, nt = diag["max_propagation_order"], len(T)
     if method == "stepwise":
         # column-steps: DOS chain, then per interval the augmented series (width 2 nb) and the axis chain
@@ -641,7 +649,7 @@ def square(args):
                                   "spectral VAC (segments of 1 fs, integration step 0.01 fs) MSD")
                                + " along x, t=0..10 fs every 0.5 fs, 161 energies, FP64, one whole estimator run "
                                  "per step", "n": n, "vectors": nb, "moments": M, "times": nt,
-                   "method": method, "precision": "fp64", "parallelism": "single GPU"},
+                   "method": method, "precision": "fp64", "parallelism": "single GPU", "batch": batch_info},
         "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                      "traffic": None, "kernel": "launch-bound at this size (N = 10^4: about 2.5 MB per block)"},
         "parity_vs_reference_cpu": parity,

@@ -118,6 +118,13 @@ int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
  * products of complex64 solves).  Uses three scratch blocks of n*nb elements
  * from the budget. */
 int kmsd_autotune(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice);
+/* No reference counterpart (the reference's sample batch is a fixed
+ * setting).  Times one fused Chebyshev step of a moment chain with product A
+ * on an nb-column block of dtype at the context's current launch shape
+ * (median of three after a warm-up) and writes microseconds per step to *us;
+ * the vector-block width tuner compares these per column across widths.
+ * Uses three scratch blocks of n*nb elements from the budget. */
+int kmsd_time_step(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, double *us);
 /* Message of the last failure in ctx (or of the last failed create when ctx
  * is NULL).  Valid until the next call on ctx. */
 const char *kmsd_last_error(const kmsd_ctx *ctx);

@@ -40,6 +40,7 @@ SIGNATURES = {
     "kmsd_ctx_transfers": [_P, _P],
     "kmsd_ctx_set_tuning": [_P, C.c_char_p, _I64],
     "kmsd_autotune": [_P, _P, _I64, _I, _P],
+    "kmsd_time_step": [_P, _P, _I64, _I, _P],
     "kmsd_last_error": [_P],
     "kmsd_build_info": [_P, _I64],
     "kmsd_matrix_create": [_P, _I, _I64, _I64, _P, _P, _P, _P],
@@ -252,6 +253,40 @@ class HipContext:
         self.set_tuning("cpl_c64" if dtype == torch.complex64 else "cpl_c128", choice["cpl"])
         self.set_tuning("groups", choice["groups"])
         return dict(choice)
+
+    def time_step(self, matrix, nb, dtype):
+        """Microseconds of one fused Chebyshev step with `matrix` on an
+        nb-column block of `dtype` at the current launch shape
+        (kmsd_time_step)."""
+        torch = _torch()
+        code = KMSD_C64 if dtype == torch.complex64 else KMSD_C128
+        self.bind_stream()
+        us = C.c_double()
+        self.check(self.lib.kmsd_time_step(self.h, matrix.h, int(nb), code, C.byref(us)))
+        return us.value
+
+    def tune_block_width(self, matrix, widths, dtype, margin=0.03):
+        """Vector-block width among `widths` (ascending) with the least
+        measured step time per column: a wider block is taken only when it is
+        more than `margin` faster per column than the best narrower one, so
+        ties keep the smaller workspace.  Remembered per (device, storage, n,
+        nnz, widths, dtype) in this process.  Returns (width, {width: us per
+        step})."""
+        torch = _torch()
+        widths = sorted({int(w) for w in widths})
+        info = matrix.info()
+        key = "|".join(str(v) for v in ("width", torch.cuda.get_device_name(self.device), info["format"],
+                                         info["n"], info["nnz"], ",".join(map(str, widths)), str(dtype)))
+        hit = _TUNED.get(key)
+        if hit is not None:
+            return hit["width"], dict(hit["us"])
+        us = {w: self.time_step(matrix, w, dtype) for w in widths}
+        best = widths[0]
+        for w in widths[1:]:
+            if us[w] / w < (1.0 - margin) * us[best] / best:
+                best = w
+        _TUNED[key] = {"width": best, "us": us}
+        return best, dict(us)
 
     def upload(self, array, dtype=None):
         """Host array -> device tensor on this context (counted as H2D)."""

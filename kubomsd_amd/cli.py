@@ -67,6 +67,9 @@ def parser():
     run.add_argument("--efermi-source")
     run.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     run.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    run.add_argument("--block-width-autotune", action="store_true",
+                     help="pick the sample batch width from measured step times (sample-batch and its "
+                          "doublings up to nsamples that fit the memory budget)")
     run.add_argument("--moment-doubling", action="store_true",
                      help="S = I: moments from the Chebyshev product identities (half the orders; rounding "
                           "differs from the default recursion)")

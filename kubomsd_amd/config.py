@@ -115,6 +115,7 @@ class RunConfig:
     device_memory_limit_gb: float = 0.
     autotune: bool = True
     moment_doubling: bool = False
+    block_width_autotune: bool = False
 
     def __post_init__(self):
         object.__setattr__(self, "energy_eV", grid(self.energy_eV, "energy"))
@@ -196,6 +197,8 @@ class RunConfig:
             raise KuboMSDError("device_memory_limit_gb must be nonnegative and finite")
         if not isinstance(self.autotune, (bool, np.bool_)):
             raise KuboMSDError("autotune must be a boolean")
+        if not isinstance(self.block_width_autotune, (bool, np.bool_)):
+            raise KuboMSDError("block_width_autotune must be a boolean")
         if not isinstance(self.moment_doubling, (bool, np.bool_)):
             raise KuboMSDError("moment_doubling must be a boolean")
         if self.spectral_bounds is not None:

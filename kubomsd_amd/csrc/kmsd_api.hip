@@ -288,6 +288,75 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
     });
 }
 
+extern "C" int kmsd_time_step(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int dtype, double *us) {
+    if (!c || !A || nb < 1 || nb > (int64_t(1) << 20) || !us) return KMSD_EINVAL;
+    return guard(c, [&] {
+        if (dtype != KMSD_C64 && dtype != KMSD_C128) throw Error(KMSD_EINVAL, "dtype must be KMSD_C64 or KMSD_C128");
+        // One fused Chebyshev step of a moment chain with the context's
+        // launch shape: 2 A X - P into P, partials <W, P>; median of three
+        // after a warm-up.
+        const size_t bytes = size_t(A->n) * size_t(nb) * elem_bytes(dtype);
+        const int64_t groups = std::max<int64_t>(step_groups(c, A->n, int32_t(nb), dtype, 4, A), 8);
+        const size_t pbytes = size_t(groups) * size_t(nb) * 2 * sizeof(double);
+        void *X = nullptr, *P = nullptr, *W = nullptr, *part = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            e0 = e1 = nullptr;
+            (void)hipStreamSynchronize(c->stream);
+            dfree(c, part, pbytes);
+            dfree(c, W, bytes);
+            dfree(c, P, bytes);
+            dfree(c, X, bytes);
+        };
+        try {
+            X = dalloc(c, bytes);
+            P = dalloc(c, bytes);
+            W = dalloc(c, bytes);
+            part = dalloc(c, pbytes);
+            HIPCHECK(hipMemsetAsync(X, 0, bytes, c->stream));
+            HIPCHECK(hipMemsetAsync(P, 0, bytes, c->stream));
+            HIPCHECK(hipMemsetAsync(W, 0, bytes, c->stream));
+            HIPCHECK(hipEventCreate(&e0));
+            HIPCHECK(hipEventCreate(&e1));
+            Step s;
+            s.A = A;
+            s.X = X;
+            s.ldx = nb;
+            s.P = P;
+            s.ldp = nb;
+            s.OUT = P;
+            s.ldo = nb;
+            s.W = W;
+            s.ldw = nb;
+            s.part = static_cast<double *>(part);
+            s.alpha = Cplx{1.0, 0};
+            s.beta = Cplx{-0.5, 0};
+            s.gamma = -1.0;
+            s.use_beta = s.use_p = true;
+            s.n = A->n;
+            s.w = int32_t(nb);
+            s.groups = groups;
+            launch_step(c, dtype, s);  // warm-up
+            float t[3];
+            for (float &v : t) {
+                HIPCHECK(hipEventRecord(e0, c->stream));
+                launch_step(c, dtype, s);
+                HIPCHECK(hipEventRecord(e1, c->stream));
+                HIPCHECK(hipEventSynchronize(e1));
+                HIPCHECK(hipEventElapsedTime(&v, e0, e1));
+            }
+            std::sort(t, t + 3);
+            *us = 1000.0 * double(t[1]);
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+    });
+}
+
 extern "C" const char *kmsd_last_error(const kmsd_ctx *c) {
     return c ? c->err.c_str() : g_create_error.c_str();
 }

@@ -98,6 +98,28 @@ def admitted_batch(ops, cfg, method):
     raise ResourceBlocked("RESOURCE_BLOCKED: one sample's device workspace exceeds the HBM budget")
 
 
+def tuned_batch(ops, cfg, method):
+    """Sample batch width from measured step times (cfg.block_width_autotune):
+    the admitted cfg.sample_batch and its doublings up to the sample count
+    (at most 256) whose workspace fits the headroom, timed as one fused
+    Chebyshev step of H on a block of that width (kmsd_time_step); the width
+    with the least time per column wins (a wider one only when more than 3 %
+    faster).  Per-sample results do not depend on the width beyond the
+    rounding of the moment reductions.  Returns (width, {width: us})."""
+    import torch
+    base = admitted_batch(ops, cfg, method)
+    free = device_headroom(ops.ctx)
+    widths = [base]
+    w = 2 * base
+    while w <= min(cfg.nsamples, 256) and batch_workspace_bytes(ops, cfg, method, w) <= free:
+        widths.append(w)
+        w *= 2
+    if len(widths) == 1:
+        return base, {}
+    dtype = torch.complex64 if cfg.precision == "mixed" else torch.complex128
+    return ops.ctx.tune_block_width(ops.H, widths, dtype)
+
+
 def _sample_batches(ops, cfg, method, samples, accum, dos_blocks, batch, stats, batch_results=None):
     """Runs the samples in global order, `batch` at a time, and adds each
     sample's DOS and numerators to its statistics block in sample order
@@ -223,7 +245,13 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
     dos_blocks = torch.zeros((nbk, len(cfg.energy_eV)), dtype=torch.float64, device=dev)
     stats = DeviceStatistics(ops.ctx)
     # precomputed batches were admitted by their runner
-    batch = max(b - a for a, b in batch_results) if batch_results else admitted_batch(ops, cfg, method)
+    width_times = {}
+    if batch_results:
+        batch = max(b - a for a, b in batch_results)
+    elif getattr(cfg, "block_width_autotune", False):
+        batch, width_times = tuned_batch(ops, cfg, method)
+    else:
+        batch = admitted_batch(ops, cfg, method)
     # The reported kernel resolution is host root-finding only; it runs on a
     # helper thread while the samples' device work (inside library calls
     # that release the GIL) proceeds.
@@ -236,6 +264,9 @@ def iter_stochastic_blocks(ops, config, method, samples=None, bounds_report=None
         valid, reason, static, metadata = _normalize_dos(ops, cfg, ns, counts, dos_blocks, report, method,
                                                          diagnostics, batch, stats, resolution.result())
     metadata["sample_seconds"] = seconds
+    if width_times:
+        metadata["block_width_tuning"] = {"chosen": int(batch),
+                                          "us_per_step": {str(k): float(v) for k, v in width_times.items()}}
     valid_host = ops.ctx.to_host(valid)
     for start in range(0, len(cfg.time_fs), cfg.time_block):
         stop = min(start + cfg.time_block, len(cfg.time_fs))

@@ -51,6 +51,17 @@ def test_parser_maps_reference_flags():
     assert cfg.device_memory_limit_gb == 2.0
 
 
+def test_block_width_autotune_flag_and_validation():
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.errors import KuboMSDError
+    base = ["run", "--input", "m.json", "--out", "o", "--emin", "-1", "--emax", "1", "--n-energy", "3",
+            "--tmax-fs", ".1", "--spectral-min", "-4", "--spectral-max", "4"]
+    assert not config_from_args(parser().parse_args(base)).block_width_autotune
+    assert config_from_args(parser().parse_args(base + ["--block-width-autotune"])).block_width_autotune
+    with pytest.raises(KuboMSDError, match="block_width_autotune"):
+        RunConfig(energy_eV=[0.0], time_fs=[0.0], block_width_autotune="yes")
+
+
 @pytest.mark.parametrize("extra, match", [(["--spectral-min", "-4"], "spectral"), ([], "emin")])
 def test_parser_rejects_incomplete(extra, match, tmp_path):
     a = ["run", "--input", "m.json", "--out", "o"] + extra
