@@ -313,7 +313,7 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
         S.col(PC_ALPHA, S.part2, S.g2, rtol, atol, false, any);
         if (fusable && !refresh) {
             S.fused(PS_UPD_PRE);
-            S.clear_flag(offsetof(PcgFlags, any_active), sizeof(int32_t));
+            S.clear_flag(offsetof(PcgFlags, any_active), 2 * sizeof(int32_t));  // any_active, updated
             S.col(PC_RN_BETA, S.part, S.rs.gx, rtol, atol);
             S.fused(PS_PUPD_PRE);
             continue;

@@ -17,7 +17,9 @@ struct ColState {
 };
 
 struct PcgFlags {
-    int32_t any_active, broke, need_refresh, nonfinite;
+    // updated: columns active when the fused iteration's PC_RN_BETA began,
+    // i.e. whose alpha this iteration's P update still has to apply to Y
+    int32_t any_active, updated, broke, need_refresh, nonfinite;
     int64_t refreshes, restarts, checks;
 };
 
@@ -34,8 +36,11 @@ enum : int32_t {
     PS_RESCALE = 9,  // OUT = Y * safe ; flags nonfinite columns
     // Fused iteration (fixed residual policy, no refresh this iteration);
     // z = M^-1 R is recomputed where needed instead of stored.
-    PS_UPD_PRE = 10,   // Y += alpha P, R -= alpha SP ; partial |R|^2 (re), Re<R,z> (im)
-    PS_PUPD_PRE = 11,  // P = active ? z + (restart ? 0 : beta P) : 0
+    // Y += alpha P waits for the P update, which reads P anyway: the two
+    // passes move 3 + 5 block streams instead of 6 + 3, and Y gets the same
+    // values (same operands, same expression).
+    PS_UPD_PRE = 10,   // R -= alpha SP ; partial |R|^2 (re), Re<R,z> (im)
+    PS_PUPD_PRE = 11,  // Y += alpha P ; P = active ? z + (restart ? 0 : beta P) : 0
 };
 
 struct PcgArgs {
@@ -59,7 +64,10 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
     using PK = Pack<VT, CPL>;
     // In-loop stages stop after a curvature breakdown (the reference leaves
     // the loop); set-up and final-acceptance stages always run.
-    if (a.require_active && (a.fl->broke || !a.fl->any_active)) return;
+    // (the fused P update also finishes the iteration's Y update, so it runs
+    // when any column was active at the iteration's start)
+    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated : a.fl->any_active;
+    if (a.require_active && (a.fl->broke || !live)) return;
     if (a.gate && *a.gate == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lpr = 1 << a.lpr_log2;
@@ -87,6 +95,7 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
                     : (a.stage == PS_PUPDATE || a.stage == PS_PUPD_PRE) ? s.beta
                     : (a.stage == PS_RESCALE) ? s.safe
                                               : 0.0;
+            if (a.stage == PS_PUPD_PRE) cf2[c] = s.alpha;
         }
     }
     const int64_t nblk = (a.n + rpb - 1) / rpb;
@@ -175,36 +184,40 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
                 break;
             }
             case PS_UPD_PRE: {
-                PK p, sp, y, r;
-                p.load(P + o);
+                PK sp, r;
                 sp.load(SP + o);
-                y.load(Y + o);
                 r.load(R + o);
                 const double d = a.diag[row];
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    const cx<double> pv = p.get(c), sv = sp.get(c);
-                    cx<double> yv = y.get(c), rv = r.get(c);
-                    yv.re += pv.re * cf[c];
-                    yv.im += pv.im * cf[c];
+                    const cx<double> sv = sp.get(c);
+                    cx<double> rv = r.get(c);
                     rv.re -= sv.re * cf[c];
                     rv.im -= sv.im * cf[c];
-                    y.set(c, yv);
                     r.set(c, rv);
                     const cx<double> rs = r.get(c);
                     const double rr = rs.re * rs.re + rs.im * rs.im;
                     acc_re[c] += rr;
                     acc_im[c] += rr / d;
                 }
-                y.store(Y + o);
                 r.store(R + o);
                 break;
             }
             case PS_PUPD_PRE: {
-                PK r, p;
+                PK r, p, y;
                 r.load(R + o);
                 p.load(P + o);
+                y.load(Y + o);
                 const double d = a.diag[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> pv = p.get(c);
+                    cx<double> yv = y.get(c);
+                    yv.re += pv.re * cf2[c];
+                    yv.im += pv.im * cf2[c];
+                    y.set(c, yv);
+                }
+                y.store(Y + o);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     cx<double> v{0.0, 0.0};
@@ -478,6 +491,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             s.rn = sqrt(tre);
             s.iters += s.active;
             s.restart = 0;
+            if (s.active) atomicAdd(&f->updated, 1);
             s.active = s.active && s.rn > s.thr;
             s.beta = s.active ? tim / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
             s.rz = tim;
@@ -556,7 +570,8 @@ template <class VT, int CPL>
 __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
     using V = cx<VT>;
     using PK = Pack<VT, CPL>;
-    if (a.require_active && (a.fl->broke || !a.fl->any_active)) return;
+    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated : a.fl->any_active;
+    if (a.require_active && (a.fl->broke || !live)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lpr = 1 << a.lpr_log2;
     const int qpw = 64 >> a.lpr_log2, qpb = 4 * qpw;
@@ -566,17 +581,18 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
     const bool colok = col < a.w;
     const int64_t w = a.w;
     const bool upd = a.stage == PS_UPD_PRE;
-    double acc_re[CPL], acc_im[CPL], cf[CPL];
+    double acc_re[CPL], acc_im[CPL], cf[CPL], al[CPL];
     int32_t act[CPL], rst[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-        acc_re[c] = acc_im[c] = cf[c] = 0.0;
+        acc_re[c] = acc_im[c] = cf[c] = al[c] = 0.0;
         act[c] = rst[c] = 0;
         if (colok) {
             const ColState &s = a.cs[col + c];
             act[c] = s.active;
             rst[c] = s.restart;
             cf[c] = upd ? s.alpha : s.beta;
+            al[c] = s.alpha;
         }
     }
     const int64_t nq = a.n >> 2;
@@ -601,22 +617,16 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
         if (upd) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                PK p, sp, y;
-                p.load(P + o0 + i * w);
+                PK sp;
                 sp.load(SP + o0 + i * w);
-                y.load(Y + o0 + i * w);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    const cx<double> pv = p.get(c), sv = sp.get(c);
-                    cx<double> yv = y.get(c), rv = r[i].get(c);
-                    yv.re += pv.re * cf[c];
-                    yv.im += pv.im * cf[c];
+                    const cx<double> sv = sp.get(c);
+                    cx<double> rv = r[i].get(c);
                     rv.re -= sv.re * cf[c];
                     rv.im -= sv.im * cf[c];
-                    y.set(c, yv);
                     r[i].set(c, rv);
                 }
-                y.store(Y + o0 + i * w);
                 r[i].store(R + o0 + i * w);
             }
         }
@@ -641,8 +651,18 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
         if (!upd) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                PK p;
+                PK p, y;
                 p.load(P + o0 + i * w);
+                y.load(Y + o0 + i * w);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> pv = p.get(c);
+                    cx<double> yv = y.get(c);
+                    yv.re += pv.re * al[c];
+                    yv.im += pv.im * al[c];
+                    y.set(c, yv);
+                }
+                y.store(Y + o0 + i * w);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     cx<double> v{0.0, 0.0};
