@@ -287,6 +287,9 @@ struct kmsd_metric {
     size_t block_bytes = 0;          // device bytes of the block arrays
     std::vector<int64_t> host_block_ptr;
     bool uniform4 = false;           // every block is 4x4 aligned to multiples of 4
+    // uniform 4x4 blocks with real factors: float[16] per block, L below the
+    // diagonal and 1/L_ii on it (the fused FP32 inner iterations' copy)
+    float *factors32 = nullptr;
     struct IcData;
     IcData *ic = nullptr;            // IC(0) preconditioner (takes precedence over blocks)
     bool direct = false;             // direct solve with the one full-size block plus refinement

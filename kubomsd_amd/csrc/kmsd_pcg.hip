@@ -114,6 +114,7 @@ struct Solver {
         a.P = P;
         a.SP = SP;
         a.fac = m->factors;
+        a.fac32 = dtype == KMSD_C64 ? m->factors32 : nullptr;
         a.cs = cs;
         a.fl = fl;
         a.part = part;

@@ -6,6 +6,7 @@
 // fused step kernel; column reductions go through [groups][w] partials that
 // the per-column kernels reduce in a fixed order (deterministic).
 #pragma once
+#include <type_traits>
 #include "kmsd_kernels.hpp"
 
 namespace kmsd {
@@ -50,6 +51,7 @@ struct PcgArgs {
     void *OUT;
     const double *diag;
     const double *fac;  // uniform 4x4 block factors, 16 complex entries per block (quad kernel)
+    const float *fac32; // the same factors when real, FP32 with 1/L_ii on the diagonal (complex64 quad kernel)
     ColState *cs;
     PcgFlags *fl;
     double *part;
@@ -562,6 +564,46 @@ __device__ __forceinline__ void blk4_solve(const cx<double> *L, cx<double> (&v)[
     }
 }
 
+// The same solve with a real factor in FP32: L below the diagonal, 1/L_ii on
+// it (products instead of divisions), FP32 arithmetic (complex64 blocks).
+__device__ __forceinline__ void blk4_solve_r32(const float *L, cx<double> (&v)[4]) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 l0 = *reinterpret_cast<const f4 *>(L), l1 = *reinterpret_cast<const f4 *>(L + 4),
+             l2 = *reinterpret_cast<const f4 *>(L + 8), l3 = *reinterpret_cast<const f4 *>(L + 12);
+    const float m[16] = {l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3],
+                         l2[0], l2[1], l2[2], l2[3], l3[0], l3[1], l3[2], l3[3]};
+    float re[4], im[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        re[i] = float(v[i].re);
+        im[i] = float(v[i].im);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float sr = re[i], si = im[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            sr = __builtin_fmaf(-m[i * 4 + k], re[k], sr);
+            si = __builtin_fmaf(-m[i * 4 + k], im[k], si);
+        }
+        re[i] = sr * m[i * 5];
+        im[i] = si * m[i * 5];
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        float sr = re[i], si = im[i];
+#pragma unroll
+        for (int k = i + 1; k < 4; ++k) {
+            sr = __builtin_fmaf(-m[k * 4 + i], re[k], sr);
+            si = __builtin_fmaf(-m[k * 4 + i], im[k], si);
+        }
+        re[i] = sr * m[i * 5];
+        im[i] = si * m[i * 5];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = {double(re[i]), double(im[i])};
+}
+
 // Fused iteration stages PS_UPD_PRE / PS_PUPD_PRE for a uniform 4x4
 // block-Jacobi preconditioner: each lane owns CPL columns of the four rows of
 // one block, so z = L^-H L^-1 r is formed in registers.  Partials as in
@@ -631,12 +673,16 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
             }
         }
         const cx<double> *L = F + q * 16;
+        const bool r32 = std::is_same<VT, float>::value && a.fac32 != nullptr;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             cx<double> v[4], rv[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) v[i] = rv[i] = r[i].get(c);
-            blk4_solve(L, v);
+            if (r32)
+                blk4_solve_r32(a.fac32 + q * 16, v);
+            else
+                blk4_solve(L, v);
             if (upd) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {

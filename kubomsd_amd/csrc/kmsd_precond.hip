@@ -287,12 +287,13 @@ void metric_release_blocks(kmsd_metric *m) {
     kmsd_ctx *c = m->ctx;
     for (void *p : {static_cast<void *>(m->block_ptr), static_cast<void *>(m->factor_off),
                     static_cast<void *>(m->factors), static_cast<void *>(m->block_rows),
-                    static_cast<void *>(m->large)})
+                    static_cast<void *>(m->large), static_cast<void *>(m->factors32)})
         if (p) (void)hipFree(p);
     c->used -= std::min<uint64_t>(c->used, m->block_bytes);
     m->block_ptr = m->block_rows = m->large = nullptr;
     m->factor_off = nullptr;
     m->factors = nullptr;
+    m->factors32 = nullptr;
     m->block_bytes = 0;
     m->nblocks = m->nlarge = 0;
     m->max_block = 0;
@@ -330,12 +331,29 @@ void metric_upload_blocks(kmsd_metric *m, int64_t nblocks, const int64_t *off, c
             m->block_rows = upload(c, r32.data(), r32.size(), &track);
         }
         if (!big.empty()) m->large = upload(c, big.data(), big.size(), &track);
+        if (uniform4) {
+            // real factors (the usual case: real S): an FP32 copy with the
+            // diagonal inverted for the fused complex64 iterations
+            bool real = true;
+            for (size_t e = 0; e < nfac && real; ++e) real = factors[2 * e + 1] == 0.0;
+            if (real) {
+                std::vector<float> f32(nfac, 0.0f);
+                for (int64_t q = 0; q < nblocks; ++q)
+                    for (int i = 0; i < 4; ++i)
+                        for (int k = 0; k <= i; ++k) {
+                            const double v = factors[2 * (size_t(q) * 16 + size_t(i * 4 + k))];
+                            f32[size_t(q) * 16 + size_t(i * 4 + k)] = float(k == i ? 1.0 / v : v);
+                        }
+                m->factors32 = upload(c, f32.data(), f32.size(), &track);
+            }
+        }
         HIPCHECK(hipStreamSynchronize(c->stream));
     } catch (...) {
         for (auto &a : track) dfree(c, a.first, a.second);
         m->block_ptr = m->block_rows = m->large = nullptr;
         m->factor_off = nullptr;
         m->factors = nullptr;
+        m->factors32 = nullptr;
         throw;
     }
     for (auto &a : track) m->block_bytes += a.second;
