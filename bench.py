@@ -232,7 +232,7 @@ def nonorthogonal(args):
         print(f"# IC(0) factor: shift {ic.report['shift']}, {ic.report['seconds']:.1f}s", file=sys.stderr, flush=True)
     eng = HipEngine(ctx, model.H, metric, [(-1, []), (-1, []), (-1, [])], 1e-10, 0.0, 1000)
     # the chain's dominant products: S in the FP32 inner solves
-    tuning = apply_tuning(ctx, args, metric.S, nb, torch.complex64, f32_values=True)
+    tuning = apply_tuning(ctx, args, metric.S, nb, torch.complex64, f32_values=True, tile=True)
     bound = CUBIC64_BOUND if args.atoms == 64 and CUBIC64_BOUND else None
     if bound is None:
         import scipy.sparse.linalg as sla
@@ -695,15 +695,17 @@ def plain_gpu_fp64(H, Bh, c, a, orders, device):
     return nb * orders / (ms / 1e3), mu
 
 
-def apply_tuning(ctx, args, matrix, nb, dtype, f32_values=False):
-    """Explicit --cpl/--groups, else the library's autotuner (cached)."""
+def apply_tuning(ctx, args, matrix, nb, dtype, f32_values=False, tile=False):
+    """Explicit --cpl/--groups, else the library's autotuner (cached); tile:
+    also the 4x4-block tile size unless --tile-bytes gives it."""
     if args.cpl or args.groups:
         if args.cpl:
             ctx.set_tuning("cpl", args.cpl)
         if args.groups:
             ctx.set_tuning("groups", args.groups)
         return {"cpl": args.cpl or None, "groups": args.groups or None, "source": "command line"}
-    choice = ctx.autotune(matrix, nb, dtype, f32_values=f32_values)
+    choice = ctx.autotune(matrix, nb, dtype, f32_values=f32_values,
+                          tile=tile and getattr(args, "tile_bytes", None) is None)
     choice["source"] = "kmsd_autotune"
     return choice
 

@@ -38,6 +38,11 @@ enum { KMSD_C64 = 1, KMSD_C128 = 2 };
 /* Or-ed into kmsd_autotune's dtype: time the product with FP32 matrix values,
  * the form the inner solves of a complex64 metric solve use. */
 enum { KMSD_TUNE_F32_VALUES = 0x100 };
+/* Or-ed into kmsd_autotune's dtype: choice has three entries, and for a
+ * 4x4-block matrix with a plane stride the tuner also times the plane-tiled
+ * walk's tile size ("tile_bytes": 0 = natural order, 512 KiB, 1 MiB, 2 MiB)
+ * with the chosen shape and writes it to choice[2]. */
+enum { KMSD_TUNE_TILE = 0x200 };
 enum { KMSD_CSR = 0, KMSD_BSR4 = 1, KMSD_AUTO = 2 };
 enum { KMSD_PRECOND_DIAGONAL = 0, KMSD_PRECOND_BLOCK_JACOBI = 1 };
 enum { KMSD_MU_FP32 = 0, KMSD_MU_BF16X3 = 1, KMSD_MU_FP64 = 2 };
@@ -114,9 +119,10 @@ int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
  * workgroup count (automatic, 1024, 2048, 4096), keeps the default shape
  * (automatic workgroups) unless another is more than 3 % faster, applies
  * the choice to ctx ("cpl_c64" or "cpl_c128" for the tuned precision, and
- * "groups") and writes it to choice[2] (groups 0 = automatic).  dtype may carry KMSD_TUNE_F32_VALUES (tune for the metric
- * products of complex64 solves).  Uses three scratch blocks of n*nb elements
- * from the budget. */
+ * "groups") and writes it to choice[0..1] (groups 0 = automatic).  dtype may carry KMSD_TUNE_F32_VALUES (tune for the metric
+ * products of complex64 solves) and KMSD_TUNE_TILE (also the 4x4-block tile
+ * size, into choice[2]; the default 1 MiB stays unless another is more than
+ * 3 % faster).  Uses three scratch blocks of n*nb elements from the budget. */
 int kmsd_autotune(kmsd_ctx *ctx, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice);
 /* No reference counterpart (the reference's sample batch is a fixed
  * setting).  Times one fused Chebyshev step of a moment chain with product A

@@ -24,6 +24,7 @@ LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libkmsd.so"
 KMSD_OK, KMSD_EINVAL, KMSD_ENUMERIC, KMSD_ERESOURCE, KMSD_EDEVICE = range(5)
 KMSD_C64, KMSD_C128 = 1, 2
 KMSD_TUNE_F32_VALUES = 0x100
+KMSD_TUNE_TILE = 0x200
 KMSD_CSR, KMSD_BSR4, KMSD_AUTO = 0, 1, 2
 KMSD_MU_FP32, KMSD_MU_BF16X3, KMSD_MU_FP64 = 0, 1, 2
 
@@ -203,7 +204,7 @@ class HipContext:
     def set_tuning(self, key, value):
         self.check(self.lib.kmsd_ctx_set_tuning(self.h, key.encode(), int(value)))
 
-    def autotune(self, matrix, nb, dtype, cache_path=None, f32_values=False):
+    def autotune(self, matrix, nb, dtype, cache_path=None, f32_values=False, tile=False):
         """Launch shape (columns per lane, workgroups) for products of `matrix`
         on nb-column blocks of `dtype` (torch dtype), timed on the device by
         kmsd_autotune and remembered per (device, storage, n, nnz, nb, dtype)
@@ -211,13 +212,17 @@ class HipContext:
         ~/.cache/kubomsd_amd/tuning.json), so later runs reuse the same shape
         and produce the same results.  f32_values: tune the product with FP32
         matrix values (the metric products of complex64 solves, which dominate
-        nonorthogonal chains).  Applies the shape to this context."""
+        nonorthogonal chains).  tile: also the tile size of the plane-tiled
+        4x4-block walk (matrices with a plane stride).  Applies the shape to
+        this context."""
         import json
         import os
         torch = _torch()
         code = KMSD_C64 if dtype == torch.complex64 else KMSD_C128
         if f32_values and code == KMSD_C64:
             code |= KMSD_TUNE_F32_VALUES
+        if tile:
+            code |= KMSD_TUNE_TILE
         info = matrix.info()
         key = "|".join(str(v) for v in (torch.cuda.get_device_name(self.device), info["format"], info["n"],
                                          info["nnz"], int(nb), code))
@@ -232,9 +237,11 @@ class HipContext:
                 choice = None
         if choice is None:
             self.bind_stream()
-            out = np.zeros(2, np.int64)
+            out = np.zeros(3, np.int64)
             self.check(self.lib.kmsd_autotune(self.h, matrix.h, int(nb), code, _np_ptr(out)))
             choice = {"cpl": int(out[0]), "groups": int(out[1])}
+            if tile:
+                choice["tile_bytes"] = int(out[2])
             try:
                 os.makedirs(os.path.dirname(path), exist_ok=True)
                 try:
@@ -252,6 +259,8 @@ class HipContext:
         _TUNED[key] = choice
         self.set_tuning("cpl_c64" if dtype == torch.complex64 else "cpl_c128", choice["cpl"])
         self.set_tuning("groups", choice["groups"])
+        if "tile_bytes" in choice:
+            self.set_tuning("tile_bytes", choice["tile_bytes"])
         return dict(choice)
 
     def time_step(self, matrix, nb, dtype):

@@ -174,7 +174,8 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
 extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int dtype, int64_t *choice) {
     if (!c || !A || nb < 1 || nb > (int64_t(1) << 20) || !choice) return KMSD_EINVAL;
     const bool f32_values = (dtype & KMSD_TUNE_F32_VALUES) != 0;
-    dtype &= ~int(KMSD_TUNE_F32_VALUES);
+    const bool tune_tile = (dtype & KMSD_TUNE_TILE) != 0;
+    dtype &= ~int(KMSD_TUNE_F32_VALUES | KMSD_TUNE_TILE);
     return guard(c, [&] {
         if (dtype != KMSD_C64 && dtype != KMSD_C128) throw Error(KMSD_EINVAL, "dtype must be KMSD_C64 or KMSD_C128");
         if (f32_values && dtype != KMSD_C64) throw Error(KMSD_EINVAL, "FP32 matrix values need complex64 blocks");
@@ -189,6 +190,7 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
         const int saved_cpl = c->cpl[slot];
         const int64_t saved_groups = c->groups;
         const bool saved_auto = c->auto_groups;
+        const int64_t saved_tile = c->tile_bytes;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto release = [&] {
             if (e0) (void)hipEventDestroy(e0);
@@ -213,6 +215,44 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
             const int def_cpl = default_cpl(dtype, A);
             const int cpls[3] = {1, 2, 4};
             const int64_t groups[4] = {0, 1024, 2048, 4096};  // 0 = resident capacity
+            auto time_shape = [&]() -> double {
+                Step s;
+                s.A = A;
+                s.X = X;
+                s.ldx = nb;
+                s.P = P;
+                s.ldp = nb;
+                s.OUT = P;
+                s.ldo = nb;
+                s.part = static_cast<double *>(part);
+                s.alpha = Cplx{1.0, 0};
+                if (f32_values) {
+                    // the inner solves' product: S p stored, <p, S p> partials
+                    s.W = X;
+                    s.ldw = nb;
+                    s.f32_matrix = true;
+                } else {
+                    // a moment-chain order: 2 H T_n - T_(n-1) in place, <W, T_(n+1)> partials
+                    s.W = W;
+                    s.ldw = nb;
+                    s.beta = Cplx{-0.5, 0};
+                    s.gamma = -1.0;
+                    s.use_beta = s.use_p = true;
+                }
+                s.n = A->n;
+                s.w = int32_t(nb);
+                launch_step(c, dtype, s);  // warm-up
+                float t[3];
+                for (float &v : t) {
+                    HIPCHECK(hipEventRecord(e0, c->stream));
+                    launch_step(c, dtype, s);
+                    HIPCHECK(hipEventRecord(e1, c->stream));
+                    HIPCHECK(hipEventSynchronize(e1));
+                    HIPCHECK(hipEventElapsedTime(&v, e0, e1));
+                }
+                std::sort(t, t + 3);
+                return double(t[1]);
+            };
             double best = 0, def_time = 0;
             int best_cpl = def_cpl;
             int64_t best_groups = 0;
@@ -223,42 +263,7 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
                     c->cpl[slot] = cp;
                     c->auto_groups = g == 0;
                     c->groups = g ? g : int64_t(std::max(8, c->cus * 8));
-                    Step s;
-                    s.A = A;
-                    s.X = X;
-                    s.ldx = nb;
-                    s.P = P;
-                    s.ldp = nb;
-                    s.OUT = P;
-                    s.ldo = nb;
-                    s.part = static_cast<double *>(part);
-                    s.alpha = Cplx{1.0, 0};
-                    if (f32_values) {
-                        // the inner solves' product: S p stored, <p, S p> partials
-                        s.W = X;
-                        s.ldw = nb;
-                        s.f32_matrix = true;
-                    } else {
-                        // a moment-chain order: 2 H T_n - T_(n-1) in place, <W, T_(n+1)> partials
-                        s.W = W;
-                        s.ldw = nb;
-                        s.beta = Cplx{-0.5, 0};
-                        s.gamma = -1.0;
-                        s.use_beta = s.use_p = true;
-                    }
-                    s.n = A->n;
-                    s.w = int32_t(nb);
-                    launch_step(c, dtype, s);  // warm-up
-                    float t[3];
-                    for (float &v : t) {
-                        HIPCHECK(hipEventRecord(e0, c->stream));
-                        launch_step(c, dtype, s);
-                        HIPCHECK(hipEventRecord(e1, c->stream));
-                        HIPCHECK(hipEventSynchronize(e1));
-                        HIPCHECK(hipEventElapsedTime(&v, e0, e1));
-                    }
-                    std::sort(t, t + 3);
-                    const double med = t[1];
+                    const double med = time_shape();
                     if (cp == def_cpl && g == 0) def_time = med;
                     if (best == 0 || med < best) {
                         best = med;
@@ -277,10 +282,31 @@ extern "C" int kmsd_autotune(kmsd_ctx *c, const kmsd_matrix *A, int64_t nb, int 
             c->groups = best_groups ? best_groups : int64_t(std::max(8, c->cus * 8));
             choice[0] = best_cpl;
             choice[1] = best_groups;
+            if (tune_tile) {
+                // tile size of the plane-tiled 4x4-block walk, with the chosen shape
+                int64_t best_tile = c->tile_bytes;
+                if (A->fmt == KMSD_BSR4 && A->plane) {
+                    const int64_t tiles[4] = {int64_t(1) << 20, 0, int64_t(1) << 19, int64_t(1) << 21};
+                    double tbest = 0, tdef = 0;
+                    for (int64_t tb : tiles) {
+                        c->tile_bytes = tb;
+                        const double med = time_shape();
+                        if (tb == (int64_t(1) << 20)) tdef = med;
+                        if (tbest == 0 || med < tbest) {
+                            tbest = med;
+                            best_tile = tb;
+                        }
+                    }
+                    if (tbest > 0.97 * tdef) best_tile = int64_t(1) << 20;
+                }
+                c->tile_bytes = best_tile;
+                choice[2] = best_tile;
+            }
         } catch (...) {
             c->cpl[slot] = saved_cpl;
             c->groups = saved_groups;
             c->auto_groups = saved_auto;
+            c->tile_bytes = saved_tile;
             release();
             throw;
         }

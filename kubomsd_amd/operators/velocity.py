@@ -189,7 +189,7 @@ class OperatorBundle:
                     self.tuning = self.ctx.autotune(self.H, config.sample_batch, self.dtype)
                 else:
                     self.tuning = self.ctx.autotune(self.metric.device.S, config.sample_batch, self.dtype,
-                                                    f32_values=True)
+                                                    f32_values=True, tile=True)
             except ResourceBlocked:
                 self.tuning = None
         self.native = HipEngine(self.ctx, self.H, self.metric.device, definitions, config.metric_rtol,

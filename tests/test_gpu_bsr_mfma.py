@@ -104,3 +104,33 @@ def test_mfma_metric_solve_reaches_fp64_residual(hip_ctx):
     assert rel_err(out[1][0], out[0][0]) < 1e-6
     assert not np.array_equal(out[1][0], out[0][0])
     assert abs(out[1][1] - out[0][1]) <= 2
+
+
+def test_autotune_picks_tile_size_for_plane_matrices(hip_ctx, tmp_path):
+    """kmsd_autotune with KMSD_TUNE_TILE also times the plane-tiled walk's tile
+    size for a 4x4-block matrix with a plane stride (simple cubic 16^3: 256
+    block rows per plane), applies it, and products with it still match
+    SciPy; without a plane stride the context's tile size is reported."""
+    import torch
+    from kubomsd_amd.backends import hip as H
+    m = cubic_orbital_model(16, seed=4411)
+    S = H.HipMatrix(hip_ctx, m.S)
+    assert S.info()["format"] == "bsr4" and S.info()["plane_stride"] == 256
+    H._TUNED.clear()
+    try:
+        choice = hip_ctx.autotune(S, 16, torch.complex64, cache_path=str(tmp_path / "t.json"), f32_values=True,
+                                  tile=True)
+        assert choice["tile_bytes"] in (0, 1 << 19, 1 << 20, 1 << 21)
+        B = random_block(m.n, 16, 3)
+        got = host(S.matmul(to_dev(hip_ctx, B, "c64")))
+        assert rel_err(got, m.S @ B.astype(np.complex64).astype(np.complex128)) < 2e-6
+        flat = H.HipMatrix(hip_ctx, ragged_bsr(), fmt=H.KMSD_BSR4)
+        assert flat.info()["plane_stride"] == 0
+        H._TUNED.clear()
+        other = hip_ctx.autotune(flat, 16, torch.complex64, cache_path=str(tmp_path / "t.json"), tile=True)
+        assert other["tile_bytes"] == choice["tile_bytes"]  # unchanged: no plane to tile
+    finally:
+        H._TUNED.clear()
+        hip_ctx.set_tuning("cpl", 0)
+        hip_ctx.set_tuning("groups", 0)
+        hip_ctx.set_tuning("tile_bytes", 1 << 20)
