@@ -111,7 +111,10 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * bytes across planes; 0: natural order; same results either way),
  * "bsr_mfma" (1, the default: FP32-valued 4x4-block products of complex64
  * blocks, the metric products of complex64 inner solves, run on the matrix
- * cores with FP32 row sums; 0: the VALU kernel with FP64 row sums). */
+ * cores with FP32 row sums; 0: the VALU kernel with FP64 row sums),
+ * "bsr_lds" (1, the default: FP64-valued 4x4-block products stage each
+ * chunk's block columns and values in LDS; 0: the kernel that loads them
+ * block by block; same results either way). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an

@@ -156,6 +156,8 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             c->ell_kernel = value != 0;
         } else if (k == "bsr_mfma") {
             c->bsr_mfma = value != 0;
+        } else if (k == "bsr_lds") {
+            c->bsr_lds = value != 0;
         } else if (k == "fuse_velocity") {
             c->fuse_velocity = value != 0;
         } else if (k == "tile_bytes") {

@@ -30,6 +30,7 @@ struct kmsd_ctx {
     bool graphs = true;     // replay moment-chain chunks as HIP graphs (identity metric)
     bool ell_kernel = true; // fixed-width ELL products (widths 3, 4, 6) use the prefetching kernel
     bool bsr_mfma = true;   // FP32-valued 4x4-block products of complex64 blocks on the matrix cores
+    bool bsr_lds = true;    // FP64-valued 4x4-block products stage block columns and values in LDS
     int64_t tile_bytes = 1 << 20; // BSR4 plane-tiled walk: gathered-block bytes per tile (0: natural order)
     bool fuse_velocity = true; // kmsd_chain_v forms V T_k inside the chain's steps where V fits H's ELL slots
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)

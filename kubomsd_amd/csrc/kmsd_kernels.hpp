@@ -869,6 +869,168 @@ static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a)
     }
 }
 
+// FP64-valued 4x4-block product (real or imaginary-only values) with the
+// block columns and values of each chunk of blocks staged in LDS, as in
+// bsr4_mfma_kernel, and FP64 row sums on the VALU in bsr4_step_kernel's
+// order (so its products are bitwise those of bsr4_step_kernel): the
+// chain's H products of complex64 or complex128 blocks and the FP64
+// residuals of the metric refinement.  Flags: F_SPMM with F_BETA, F_P,
+// F_STORE and F_MOM.  Lane layout of bsr4_step_kernel (lpr = 2^lpr_log2 >= 8
+// lanes per block row, CPL columns each); a wavefront stages KC = lpr blocks
+// of each of its block rows per chunk: 8 buffer-to-LDS loads of 16 bytes per
+// lane (instruction i holds blocks kc + i*lpr/8 .. of every block row, 1 KB)
+// and one of the block columns.  DPP: lanes per block row >= 16, each lane
+// reads one of the block's 16 values and the row shares them by DPP
+// broadcasts; otherwise each lane reads all 16.
+template <class VT, int CPL, bool DPP>
+__global__ __launch_bounds__(256) void bsr4_lds_kernel(const StepArgs a) {
+    using V = cx<VT>;
+    using MV = MatVal<double, false>;
+    using PK = Pack<VT, CPL>;
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (a.gate && *a.gate == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lg = a.lpr_log2, lpr = 1 << lg;
+    const int qpw = 64 >> lg, qpb = 4 * qpw;
+    const int qw = lane >> lg;
+    const int cgrp = lane & (lpr - 1);
+    const int col = (int(blockIdx.y) * lpr + cgrp) * CPL;
+    const bool colok = col < a.w;
+    const int bpi = lpr >> 3, kc_max = lpr;  // blocks per value load and row; chunk length
+    const V *P = static_cast<const V *>(a.P);
+    const V *W = static_cast<const V *>(a.W);
+    const V *X = static_cast<const V *>(a.X);
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(a.X, a.xbytes), vr = buffer_rsrc(a.val, a.vbytes),
+                                 ir = buffer_rsrc(a.idx, a.ibytes);
+    const uint32_t fl = a.flags;
+    const bool need_xi = (fl & F_BETA) != 0;
+    __shared__ d2 svals[4][8][64];
+    __shared__ int32_t sidx[4][64];
+    cx<double> mom[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) mom[c] = {0.0, 0.0};
+    const int64_t nq = a.n >> 2;
+    const int64_t nblk = (nq + qpb - 1) / qpb;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const int64_t span = (nblk + 7) / 8;
+    const int64_t b0 = int64_t(xcd) * span;
+    const int64_t b1 = b0 + span < nblk ? b0 + span : nblk;
+    const uint32_t rowb = uint32_t(a.ldx) * uint32_t(sizeof(V)), colb = uint32_t(col) * uint32_t(sizeof(V));
+    for (int64_t blk = b0 + slot; blk < b1; blk += per) {
+        const int64_t pos = blk * qpb + wave * qpw + qw;
+        const bool valid = pos < nq;
+        const int64_t q =
+            !valid ? 0 : (a.tile_plane ? int64_t(tile_row(uint32_t(pos), uint32_t(nq), a.tile_plane, a.tile_rows)) : pos);
+        int32_t k0 = 0, cnt = 0;
+        if (valid) {
+            k0 = a.ptr[q];
+            cnt = a.ptr[q + 1] - k0;
+        }
+        int kmax = cnt;  // the wavefront stages chunks together
+        for (int off = lpr; off < 64; off <<= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+        cx<double> y[4][CPL];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) y[r][c] = {0.0, 0.0};
+        for (int kc = 0; kc < kmax; kc += kc_max) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (lds_void *)&sidx[wave][0], 4, uint32_t(k0 + kc + cgrp) * 4u, 0,
+                                                     0, 0);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(vr, (lds_void *)&svals[wave][i][0], 16,
+                                                         uint32_t(k0 + kc + i * bpi) * 128u + uint32_t(cgrp) * 16u, 0,
+                                                         0, 0);
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            const int kend = min(kmax - kc, kc_max);
+            for (int kq = 0; kq < kend; ++kq) {
+                if (kc + kq >= cnt) continue;  // this block row is done (its lanes idle to the chunk's end)
+                const int32_t j = sidx[wave][qw * lpr + kq];
+                const uint32_t xo = uint32_t(j) * 4u * rowb + colb;
+                PK x0[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    x0[cc].v = {};
+                    if (colok) {
+                        if constexpr (sizeof(VT) * 2 * CPL == 32) {
+                            const f4 lo = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(xr, xo + cc * rowb, 0, 0));
+                            const f4 hi =
+                                __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(xr, xo + cc * rowb + 16, 0, 0));
+                            typedef VT vx __attribute__((ext_vector_type(16 / sizeof(VT))));
+                            const vx l2 = __builtin_bit_cast(vx, lo), h2 = __builtin_bit_cast(vx, hi);
+#pragma unroll
+                            for (int e = 0; e < int(16 / sizeof(VT)); ++e) {
+                                x0[cc].v[e] = l2[e];
+                                x0[cc].v[int(16 / sizeof(VT)) + e] = h2[e];
+                            }
+                        } else {
+                            x0[cc].load(X + (int64_t(j) * 4 + cc) * a.ldx + col);
+                        }
+                    }
+                }
+                const int bi = kq / bpi, bb = kq - bi * bpi;
+                const double *vb = reinterpret_cast<const double *>(&svals[wave][bi][qw * lpr + bb * 8]);
+                double v[16];
+                if constexpr (DPP) {
+                    row_bcast16(vb[lane & 15], v);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const d2 t = reinterpret_cast<const d2 *>(vb)[e];
+                        v[2 * e] = t[0];
+                        v[2 * e + 1] = t[1];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                        for (int c = 0; c < CPL; ++c) MV::fma(y[r][c], v[r * 4 + cc], x0[cc].template geta<double>(c));
+            }
+        }
+        if (!valid || !colok) continue;
+        const int64_t row0 = q * 4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            PK xi, pv, wv;
+            if (need_xi) xi.load(X + (row0 + r) * a.ldx + col);
+            if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
+            if (fl & F_MOM) wv.load_nt(W + (row0 + r) * a.ldw + col);
+            step_epilogue<VT, double, CPL>(a, fl, row0 + r, col, y[r], xi, pv, wv, mom);
+        }
+    }
+    if (!(fl & F_MOM)) return;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+        for (int off = lpr; off < 64; off <<= 1) {
+            mom[c].re += __shfl_xor(mom[c].re, off);
+            mom[c].im += __shfl_xor(mom[c].im, off);
+        }
+    __shared__ cx<double> red[4][32 * CPL];
+    if (lane < lpr) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) red[wave][lane * CPL + c] = mom[c];
+    }
+    __syncthreads();
+    if (wave == 0 && lane < lpr && colok) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (col + c >= a.w) continue;
+            cx<double> s = red[0][lane * CPL + c];
+            for (int w2 = 1; w2 < 4; ++w2) {
+                s.re += red[w2][lane * CPL + c].re;
+                s.im += red[w2][lane * CPL + c].im;
+            }
+            reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
+        }
+    }
+}
+
 // out[s][c] = sum_g part[s][g][c] in a fixed tree order (deterministic).
 // grid (w, nsteps), 256 threads.  mode 0: real part to out_re[s*ldo + c];
 // mode 1: complex to out_cx[s*ldo + c].

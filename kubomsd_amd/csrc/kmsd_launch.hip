@@ -295,6 +295,7 @@ struct KernelKey {
     int lpr_log2 = 6;
     bool vout = false;   // the ELL kernel also forms V X (FP64 values and row sums, real A)
     bool mfma = false;   // bsr4_mfma_kernel (FP32-valued 4x4 blocks on the matrix cores)
+    bool lds = false;    // bsr4_lds_kernel (FP64-valued 4x4 blocks, LDS-staged)
 };
 
 template <class VT, class MR, bool MC, class AT, int CPL>
@@ -468,6 +469,42 @@ void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, S
     a.ibytes = uint32_t(nnzb * 4);
 }
 
+// The LDS-staged FP64-valued 4x4-block kernel (bsr4_lds_kernel) when the
+// flags, width and byte extents allow it; otherwise bsr4_step_kernel.
+void lds_shape(kmsd_ctx *c, int dtype, const Step &s, const kmsd_matrix *A, uint32_t fl, StepArgs &a, Shape &sh,
+               KernelFn &fn, KernelKey &key) {
+    if (fl & ~(F_SPMM | F_BETA | F_P | F_STORE | F_MOM)) return;
+    const int cpl = dtype == KMSD_C64 ? 4 : 2;
+    if (s.w % cpl || cpl_limit(s, dtype) < cpl) return;
+    const int64_t lanes = s.w / cpl;
+    const int lg = std::max(3, std::min(5, ilog2_ceil(lanes)));
+    const int64_t lpr = int64_t(1) << lg;
+    const int64_t lim = (int64_t(1) << 31) - (int64_t(1) << 16);
+    const int64_t nnzb = A->noff / 16;
+    const int64_t xbytes = s.n * s.ldx * int64_t(elem_bytes(dtype));
+    if (xbytes > lim || nnzb * 128 > lim) return;
+    if (dtype == KMSD_C64)
+        fn = lg >= 4 ? bsr4_lds_kernel<float, 4, true> : bsr4_lds_kernel<float, 4, false>;
+    else
+        fn = lg >= 4 ? bsr4_lds_kernel<double, 2, true> : bsr4_lds_kernel<double, 2, false>;
+    key.lds = true;
+    sh.cpl = cpl;
+    sh.lpr_log2 = lg;
+    sh.gy = int((lanes + lpr - 1) / lpr);
+    if (!s.groups) {
+        const int64_t qpb = 4 * (64 / lpr);
+        const int64_t nblk = (s.n / 4 + qpb - 1) / qpb;
+        const int64_t want_g = ((nblk + 7) / 8) * 8;
+        int64_t g = c->groups;
+        if (c->auto_groups) g = std::max<int64_t>(8, (int64_t(c->cus) * resident_per_cu(fn) / sh.gy) / 8 * 8);
+        sh.gx = std::max<int64_t>(8, std::min<int64_t>(g, want_g));
+    }
+    a.lpr_log2 = lg;
+    a.xbytes = uint32_t(xbytes);
+    a.vbytes = uint32_t(nnzb * 128);
+    a.ibytes = uint32_t(nnzb * 4);
+}
+
 // Kernel arguments, launch shape and kernel of one fused step.
 void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh, KernelFn &fn, KernelKey &key) {
     const kmsd_matrix *A = s.A;
@@ -520,6 +557,7 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.doubling) fl |= F_MOM2;
     if (s.ACC) fl |= F_ACC;
     if (bsr && f32 && A->vkind != VK_COMPLEX && c->bsr_mfma) mfma_shape(c, s, A, fl, a, sh, fn, key);
+    if (bsr && !f32 && A->vkind != VK_COMPLEX && c->bsr_lds) lds_shape(c, dtype, s, A, fl, a, sh, fn, key);
     a.X = s.X;
     a.Z = s.Z;
     a.P = s.P;

@@ -134,3 +134,50 @@ def test_autotune_picks_tile_size_for_plane_matrices(hip_ctx, tmp_path):
         hip_ctx.set_tuning("cpl", 0)
         hip_ctx.set_tuning("groups", 0)
         hip_ctx.set_tuning("tile_bytes", 1 << 20)
+
+
+@pytest.mark.parametrize("dtype, w", [("c64", 8), ("c64", 32), ("c64", 48), ("c64", 64), ("c64", 128),
+                                      ("c128", 16), ("c128", 24), ("c128", 32), ("c128", 64)])
+def test_lds_staged_fp64_products_bitwise(hip_ctx, dtype, w):
+    """FP64-valued 4x4-block products (bsr4_lds_kernel: block columns and
+    values staged in LDS per chunk) sum in bsr4_step_kernel's order, so the
+    two kernels ("bsr_lds" 1 / 0) agree bit for bit: plain products on ragged
+    rows (0..45 blocks: several chunks of 8, 16 or 32 blocks) and real or
+    imaginary-only values, and against SciPy."""
+    from kubomsd_amd.backends.hip import KMSD_BSR4, HipMatrix
+    try:
+        for A in (ragged_bsr(), 1j * ragged_bsr(seed=29), cubic_orbital_model(3, seed=4411).H):
+            A = sp.csr_matrix(A)
+            m = HipMatrix(hip_ctx, A, fmt=KMSD_BSR4)
+            B = random_block(A.shape[0], w, 300 + w)
+            out = []
+            for flag in (1, 0):
+                hip_ctx.set_tuning("bsr_lds", flag)
+                out.append(host(m.matmul(to_dev(hip_ctx, B, dtype))))
+            np.testing.assert_array_equal(out[0], out[1])
+            Bq = B.astype(np.complex64).astype(np.complex128) if dtype == "c64" else B
+            assert rel_err(out[0], A @ Bq) < (1e-13 if dtype == "c128" else 2e-6)
+    finally:
+        hip_ctx.set_tuning("bsr_lds", 1)
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_lds_staged_chain_and_solve_bitwise(hip_ctx, dtype):
+    """Moment chains of S^-1 H (H products with the recursion's BETA/P terms
+    and moment partials, FP64 refinement residuals of the metric solves) are
+    bit for bit the same with either FP64-valued 4x4-block kernel."""
+    import scipy.linalg as la
+    from kubomsd_amd.backends.hip import HipEngine, HipMetric
+    m = cubic_orbital_model(4, seed=4411)
+    blocks = [(a, b, la.cholesky(m.S[a:b, a:b].toarray(), lower=True)) for a, b in m.atom_slices]
+    B = random_block(m.n, 32, 9)
+    outs = []
+    try:
+        for flag in (1, 0):
+            hip_ctx.set_tuning("bsr_lds", flag)
+            met = HipMetric(hip_ctx, m.S, m.S.diagonal().real, blocks)
+            eng = HipEngine(hip_ctx, m.H, met, [(-1, [])] * 3, 1e-10, 0.0, 1000)
+            outs.append(host(eng.moments(to_dev(hip_ctx, B, dtype), 24, 0.0, 40.0)))
+    finally:
+        hip_ctx.set_tuning("bsr_lds", 1)
+    np.testing.assert_array_equal(outs[0], outs[1])
