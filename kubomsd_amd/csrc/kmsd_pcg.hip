@@ -199,6 +199,30 @@ struct Solver {
             row(PS_DIAG, nullptr, 0, nullptr, 0, require_active, gate);
             return;
         }
+        if (m->uniform4 && m->factors32 && dtype == KMSD_C64 && m->nlarge == 0) {
+            // z and <r, z> in one pass of the 4x4-block kernel (FP32 factors)
+            PcgArgs a{};
+            a.R = R;
+            a.Z = Z;
+            a.fac = m->factors;
+            a.fac32 = m->factors32;
+            a.cs = cs;
+            a.fl = fl;
+            a.part = part;
+            a.n = n;
+            a.w = w;
+            a.lpr_log2 = rs.lpr_log2;
+            a.stage = PS_PRE_DOT;
+            a.require_active = require_active ? 1 : 0;
+            a.gate = gate;
+            const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
+            if (rs.cpl == 2)
+                hipLaunchKernelGGL((pcg_quad_kernel<float, 2>), grid, dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((pcg_quad_kernel<float, 1>), grid, dim3(256), 0, c->stream, a);
+            HIPCHECK(hipGetLastError());
+            return;
+        }
         if (m->nlarge < m->nblocks) {
             const int64_t total = m->nblocks * w;
             const int64_t g = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 16384));

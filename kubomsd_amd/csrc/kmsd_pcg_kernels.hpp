@@ -42,6 +42,7 @@ enum : int32_t {
     // values (same operands, same expression).
     PS_UPD_PRE = 10,   // R -= alpha SP ; partial |R|^2 (re), Re<R,z> (im)
     PS_PUPD_PRE = 11,  // Y += alpha P ; P = active ? z + (restart ? 0 : beta P) : 0
+    PS_PRE_DOT = 12,   // uniform 4x4 blocks: Z = M^-1 R stored ; partial Re<R,Z> (one pass for z and the dot)
 };
 
 struct PcgArgs {
@@ -622,7 +623,8 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
     const int col = (blockIdx.y * lpr + cgrp) * CPL;
     const bool colok = col < a.w;
     const int64_t w = a.w;
-    const bool upd = a.stage == PS_UPD_PRE;
+    const bool upd = a.stage == PS_UPD_PRE, pre = a.stage == PS_PRE_DOT;
+    if (a.gate && *a.gate == 0) return;
     double acc_re[CPL], acc_im[CPL], cf[CPL], al[CPL];
     int32_t act[CPL], rst[CPL];
 #pragma unroll
@@ -646,6 +648,7 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
     V *Y = static_cast<V *>(a.Y);
     V *R = static_cast<V *>(a.R);
     V *P = static_cast<V *>(a.P);
+    V *Z = static_cast<V *>(a.Z);
     const V *SP = static_cast<const V *>(a.SP);
     const cx<double> *F = reinterpret_cast<const cx<double> *>(a.fac);
 
@@ -694,6 +697,21 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
                 for (int i = 0; i < 4; ++i) r[i].set(c, v[i]);  // r now holds z
             }
         }
+        if (pre) {
+            // Z is stored rounded, and <R, Z> is taken of the stored values
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                r[i].store(Z + o0 + i * w);
+                PK rr;
+                rr.load(R + o0 + i * w);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const cx<double> rv = rr.get(c), zv = r[i].get(c);
+                    acc_re[c] += rv.re * zv.re + rv.im * zv.im;
+                }
+            }
+            continue;
+        }
         if (!upd) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -726,7 +744,7 @@ __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
             }
         }
     }
-    if (!upd) return;
+    if (!upd && !pre) return;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         for (int off = lpr; off < 64; off <<= 1) {
