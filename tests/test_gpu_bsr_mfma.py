@@ -181,3 +181,25 @@ def test_lds_staged_chain_and_solve_bitwise(hip_ctx, dtype):
     finally:
         hip_ctx.set_tuning("bsr_lds", 1)
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("policy, interval", [("fixed", 3), ("adaptive", 4)])
+def test_mfma_products_in_refresh_and_adaptive_iterations(hip_ctx, policy, interval):
+    """complex64 solves whose inner iterations take the staged path (true-
+    residual refreshes every `interval` iterations, or the adaptive policy):
+    the refresh's T = Q - S Y product also runs on the matrix cores (F_P
+    epilogue), and the solves still reach the FP64 residual."""
+    import scipy.linalg as la
+    from kubomsd_amd.backends.hip import HipMetric
+    m = cubic_orbital_model(4, seed=4411)
+    S = sp.csr_matrix(m.S)
+    blocks = [(a, b, la.cholesky(m.S[a:b, a:b].toarray(), lower=True)) for a, b in m.atom_slices]
+    B = random_block(S.shape[0], 12, 21)
+    met = HipMetric(hip_ctx, S, S.diagonal().real, blocks)
+    met.set_residual_policy(policy, interval, 0.1)
+    Y, rep = met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 1000)
+    Bq = B.astype(np.complex64).astype(np.complex128)
+    res = np.linalg.norm(Bq - S @ host(Y), axis=0) / np.linalg.norm(Bq, axis=0)
+    assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+    assert res.max() < 1e-6  # the solution is rounded once to complex64
+    assert met.diagnostics()["true_residual_checks"] >= 1
