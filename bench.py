@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--groups", type=int, default=0, help="override workgroups per launch")
     p.add_argument("--cpl", type=int, default=0)
     p.add_argument("--pcg-poll", type=int, default=0, help="override PCG iterations between host polls")
+    p.add_argument("--min-blocks", type=int, default=0,
+                   help="override the least row blocks per workgroup of row-walking launches")
     p.add_argument("--tile-bytes", type=int, default=None,
                    help="nonorthogonal: gathered-block bytes per tile of the plane-tiled 4x4-block walk "
                         "(0: natural row order; default: the library's)")
@@ -213,6 +215,8 @@ def nonorthogonal(args):
     from kubomsd_amd.synthetic import cubic_orbital_model
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
     if args.pcg_poll:
         ctx.set_tuning("pcg_poll", args.pcg_poll)
     if args.tile_bytes is not None:
@@ -337,6 +341,8 @@ def double_moments(args):
     from kubomsd_amd.synthetic import square_lattice
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
     t_build = time.perf_counter()
     model = square_lattice(args.side, t=-1.0, W=1.5)
     n, M = model.n, args.moments
@@ -433,6 +439,8 @@ def stepwise(args):
     import logging
     logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="# %(message)s")
     ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
     t_build = time.perf_counter()
     model = honeycomb_lattice(args.cells, t=-2.7, W=1.0, a=1.42)
     n, nb, M = model.n, args.vectors, args.moments
@@ -573,6 +581,8 @@ def square(args):
     from kubomsd_amd.synthetic import square_lattice
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
     side = args.cells if args.cells != 1000 else 100
     model = square_lattice(side, t=-1.0, W=2.0)
     n, nb, M = model.n, args.vectors, args.moments
@@ -831,6 +841,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
     t_build = time.perf_counter()
     model = honeycomb_lattice(args.cells, t=-2.7, W=1.0, a=1.42)
     n, nb, M = model.n, args.vectors, args.moments

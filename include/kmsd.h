@@ -114,7 +114,12 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * cores with FP32 row sums; 0: the VALU kernel with FP64 row sums),
  * "bsr_lds" (1, the default: FP64-valued 4x4-block products stage each
  * chunk's block columns and values in LDS; 0: the kernel that loads them
- * block by block; same results either way). */
+ * block by block; same results either way), "min_blocks" (1..64, default 1:
+ * row-walking launches use at most one workgroup per this many row blocks,
+ * so small operators get fewer, fuller workgroups and fewer partial sums),
+ * "pcg_min_blocks" (1..64, default 4: the same for the vector passes of PCG
+ * solves, whose S products then use no more workgroups than the passes;
+ * both change the partial-sum partition, so sums round differently). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an

@@ -60,10 +60,13 @@ extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_c
         c->budget = budget ? budget : uint64_t(0.85 * double(fr));
         c->groups = int64_t(std::max(8, c->cus * 8));
         c->flag = static_cast<int32_t *>(dalloc(c, 256));
+        HIPCHECK(hipHostMalloc(&c->pin, kPinBytes, hipHostMallocDefault));
         noalloc_setup(device);
     });
     if (rc) {
         g_create_error = c->err;
+        if (c->flag) (void)hipFree(c->flag);
+        if (c->pin) (void)hipHostFree(c->pin);
         delete c;
         std::lock_guard<std::mutex> lock(g_owner_mu);
         --g_live;
@@ -74,10 +77,36 @@ extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_c
 }
 
 namespace kmsd {
+void read_status(kmsd_ctx *c, std::initializer_list<StatusPart> parts) {
+    size_t off = 0, staged[4] = {0, 0, 0, 0};
+    bool pinned[4] = {false, false, false, false};
+    int i = 0;
+    for (const StatusPart &p : parts) {
+        if (i < 4 && c->pin && off + p.bytes <= kPinBytes) {
+            HIPCHECK(hipMemcpyAsync(static_cast<char *>(c->pin) + off, p.src, p.bytes, hipMemcpyDeviceToHost,
+                                    c->stream));
+            staged[i] = off;
+            pinned[i] = true;
+            off += (p.bytes + 63) & ~size_t(63);
+        } else {
+            HIPCHECK(hipMemcpyAsync(p.dst, p.src, p.bytes, hipMemcpyDeviceToHost, c->stream));
+        }
+        c->d2h_status_bytes += p.bytes;
+        ++i;
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    i = 0;
+    for (const StatusPart &p : parts) {
+        if (i < 4 && pinned[i]) std::memcpy(p.dst, static_cast<const char *>(c->pin) + staged[i], p.bytes);
+        ++i;
+    }
+}
+
 void ctx_free(kmsd_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->flag) (void)hipFree(c->flag);
+    if (c->pin) (void)hipHostFree(c->pin);
     if (c->mu_part) (void)hipFree(c->mu_part);
     if (c->stats_tab) (void)hipFree(c->stats_tab);
     if (c->capture) (void)hipStreamDestroy(c->capture);
@@ -166,6 +195,9 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
+        } else if (k == "min_blocks" || k == "pcg_min_blocks") {
+            if (value < 1 || value > 64) throw Error(KMSD_EINVAL, "min_blocks must be in [1, 64]");
+            (k == "min_blocks" ? c->min_blocks : c->pcg_min_blocks) = int(value);
 
         } else {
             throw Error(KMSD_EINVAL, "unknown tuning key");

@@ -430,8 +430,7 @@ bool build_slot_values(kmsd_engine *e, const kmsd_matrix *H, const kmsd_matrix *
     hipLaunchKernelGGL(slot_values_kernel, dim3(g), dim3(256), 0, c->stream, H->idx, n, w, V->ptr, V->ell, V->idx,
                        static_cast<const double *>(V->val64), d, c->flag);
     HIPCHECK(hipGetLastError());
-    copy_d2h(c, &fail, c->flag, sizeof(fail), true);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    read_status(c, {{&fail, c->flag, sizeof(fail)}});
     if (fail) out.release();
     return fail == 0;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -23,6 +24,8 @@ struct kmsd_ctx {
     std::string err;
     int64_t groups = 2048;  // workgroups per row-walking launch (multiple of 8)
     bool auto_groups = true; // fused steps: one workgroup per resident slot of the kernel (else `groups`)
+    int min_blocks = 1;      // row blocks per workgroup at least (small operators: fewer, fuller workgroups)
+    int pcg_min_blocks = 4;  // the same for the vector passes and products of PCG solves
     int cpl[2] = {0, 0};    // columns per lane of complex64 [0] and complex128 [1] blocks; 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
@@ -35,6 +38,7 @@ struct kmsd_ctx {
     bool fuse_velocity = true; // kmsd_chain_v forms V T_k inside the chain's steps where V fits H's ELL slots
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
+    void *pin = nullptr;      // pinned host staging for status reads (kmsd::kPinBytes, allocated at creation)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
     size_t mu_part_bytes = 0;
     void *stats_tab = nullptr;  // coefficient table of kmsd_time_gradient (grow-only)
@@ -116,6 +120,19 @@ inline void copy_d2h(kmsd_ctx *c, void *dst, const void *src, size_t bytes, bool
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     (status ? c->d2h_status_bytes : c->d2h_data_bytes) += bytes;
 }
+
+// Status bytes (flags, per-column solver state, norms) to the host in one
+// stream synchronization: each part is copied asynchronously into the
+// context's pinned staging buffer, then to dst after the synchronization.  A
+// part that does not fit the staging buffer takes a pageable copy.  The
+// staging buffer exists from context creation, so no call allocates for it.
+constexpr size_t kPinBytes = size_t(256) << 10;
+struct StatusPart {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+void read_status(kmsd_ctx *c, std::initializer_list<StatusPart> parts);
 
 extern thread_local std::string g_create_error;
 
@@ -204,6 +221,7 @@ struct Step {
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
     int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
+    int64_t max_groups = 0;     // if nonzero (a multiple of 8) and `groups` is not set: at most this many
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
     // Fused second product (real width-3/4/6 ELL A with FP64 values): VOUT =
     // valpha * V X, V's real values in A's ELL slots.
@@ -239,6 +257,9 @@ void group_launch(kmsd_ctx *c, const GroupPlan &plan, int parity, int slot);
 // Number of workgroups launch_step would choose for (n, w, dtype) with
 // product operator A (null: no product).  Chains pass it as Step::groups to
 // every launch that writes partial rows of one reduction.
+// Workgroups wanted for nblk row blocks: one per `per` blocks (default: the
+// context's min_blocks), rounded up to a multiple of 8 (the XCD count).
+int64_t groups_for_blocks(const kmsd_ctx *c, int64_t nblk, int per = 0);
 int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap, const kmsd_matrix *A,
                     bool f32_matrix = false);
 // Automatic columns per lane of products with A (null: no product) on dtype blocks.

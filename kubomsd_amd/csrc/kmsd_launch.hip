@@ -370,7 +370,7 @@ Shape shape_for(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_max, const
     s.gy = int((lanes + lpr - 1) / lpr);
     const int64_t rpb = 4 * (64 / lpr);
     const int64_t nblk = (n + rpb - 1) / rpb;
-    const int64_t want_g = ((nblk + 7) / 8) * 8;
+    const int64_t want_g = groups_for_blocks(c, nblk);
     KernelKey k;
     k.dtype = dtype;
     k.f32 = f32;
@@ -421,6 +421,12 @@ int64_t blocks_for(int64_t total) { return std::max<int64_t>(1, std::min<int64_t
 
 }  // namespace
 
+int64_t groups_for_blocks(const kmsd_ctx *c, int64_t nblk, int per) {
+    const int64_t k = per > 0 ? per : c->min_blocks;
+    const int64_t g = (nblk + k - 1) / k;
+    return ((g + 7) / 8) * 8;
+}
+
 int64_t step_groups(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cpl_cap, const kmsd_matrix *A, bool f32_matrix) {
     return shape_for(c, n, w, dtype, cpl_cap, A, f32_values(c, dtype, A, f32_matrix)).gx;
 }
@@ -458,7 +464,7 @@ void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, S
     if (!s.groups) {
         const int64_t qpb = 4 * (64 / lpr);
         const int64_t nblk = (s.n / 4 + qpb - 1) / qpb;
-        const int64_t want_g = ((nblk + 7) / 8) * 8;
+        const int64_t want_g = groups_for_blocks(c, nblk);
         int64_t g = c->groups;
         if (c->auto_groups) g = std::max<int64_t>(8, (int64_t(c->cus) * resident_per_cu(fn) / sh.gy) / 8 * 8);
         sh.gx = std::max<int64_t>(8, std::min<int64_t>(g, want_g));
@@ -494,7 +500,7 @@ void lds_shape(kmsd_ctx *c, int dtype, const Step &s, const kmsd_matrix *A, uint
     if (!s.groups) {
         const int64_t qpb = 4 * (64 / lpr);
         const int64_t nblk = (s.n / 4 + qpb - 1) / qpb;
-        const int64_t want_g = ((nblk + 7) / 8) * 8;
+        const int64_t want_g = groups_for_blocks(c, nblk);
         int64_t g = c->groups;
         if (c->auto_groups) g = std::max<int64_t>(8, (int64_t(c->cus) * resident_per_cu(fn) / sh.gy) / 8 * 8);
         sh.gx = std::max<int64_t>(8, std::min<int64_t>(g, want_g));
@@ -558,6 +564,10 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.ACC) fl |= F_ACC;
     if (bsr && f32 && A->vkind != VK_COMPLEX && c->bsr_mfma) mfma_shape(c, s, A, fl, a, sh, fn, key);
     if (bsr && !f32 && A->vkind != VK_COMPLEX && c->bsr_lds) lds_shape(c, dtype, s, A, fl, a, sh, fn, key);
+    if (!s.groups && s.max_groups) {
+        if (s.max_groups < 8 || s.max_groups % 8) throw Error(KMSD_EDEVICE, "internal error: workgroup cap");
+        sh.gx = std::min<int64_t>(sh.gx, s.max_groups);
+    }
     a.X = s.X;
     a.Z = s.Z;
     a.P = s.P;
@@ -709,8 +719,7 @@ bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_
         hipLaunchKernelGGL(nonfinite_kernel<float>, dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx, n, w,
                            c->flag);
     HIPCHECK(hipGetLastError());
-    copy_d2h(c, &got, c->flag, sizeof(got), true);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    read_status(c, {{&got, c->flag, sizeof(got)}});
     return got != 0;
 }
 

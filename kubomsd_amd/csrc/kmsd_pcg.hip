@@ -20,7 +20,7 @@ using namespace kmsd;
 
 struct kmsd_metric::Work {
     DBuf q, y, r, z, p, sp, t, cs, fl, part, part2;
-    DBuf b128, y128, r128, r64, d64, cs2;
+    DBuf b128, y128, r128, r64, d64, cs2, sums;
 };
 
 namespace kmsd {
@@ -48,7 +48,7 @@ RowShape row_shape(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cap) {
     s.gy = int((lanes + lpr - 1) / lpr);
     const int64_t rpb = 4 * (64 / lpr);
     const int64_t nblk = (n + rpb - 1) / rpb;
-    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, ((nblk + 7) / 8) * 8));
+    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, groups_for_blocks(c, nblk, c->pcg_min_blocks)));
     return s;
 }
 
@@ -66,6 +66,7 @@ struct Solver {
     double *part;   // row-kernel partials [rs.gx][w]
     double *part2;  // step-kernel partials
     int64_t g2 = 0;
+    int32_t slot = 0;  // flag slot of the current iteration (k & 1): gates and the liveness reads
 
     void row(int stage, const void *B = nullptr, int64_t ldb = 0, void *OUT = nullptr, int64_t ldo = 0,
              bool require_active = false, const int32_t *gate = nullptr) {
@@ -90,6 +91,7 @@ struct Solver {
         a.lpr_log2 = rs.lpr_log2;
         a.stage = stage;
         a.require_active = require_active;
+        a.slot = slot;
         a.gate = gate;
         const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
         if (dtype == KMSD_C128)
@@ -123,6 +125,7 @@ struct Solver {
         a.lpr_log2 = rs.lpr_log2;
         a.stage = stage;
         a.require_active = 1;
+        a.slot = slot;
         const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
         if (dtype == KMSD_C128)
             hipLaunchKernelGGL((pcg_quad_kernel<double, 1>), grid, dim3(256), 0, c->stream, a);
@@ -151,6 +154,7 @@ struct Solver {
         a.stage = stage;
         a.adaptive = m->adaptive;
         a.refresh = refresh;
+        a.slot = slot;
         a.gate = gate;
         hipLaunchKernelGGL(pcg_col_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, a);
         HIPCHECK(hipGetLastError());
@@ -180,6 +184,9 @@ struct Solver {
         s.n = n;
         s.w = w;
         s.gate = gate;
+        // small systems: no more workgroups than the vector passes use, so
+        // the curvature partials stay as few as the passes' partials
+        s.max_groups = rs.gx;
         // complex64 solves are the inner solves of an FP64 refinement (or a
         // caller's FP32 request): FP32 matrix values halve the S stream.
         s.f32_matrix = dtype == KMSD_C64;
@@ -214,6 +221,7 @@ struct Solver {
             a.lpr_log2 = rs.lpr_log2;
             a.stage = PS_PRE_DOT;
             a.require_active = require_active ? 1 : 0;
+            a.slot = slot;
             a.gate = gate;
             const dim3 grid(unsigned(rs.gx), unsigned(rs.gy));
             if (rs.cpl == 2)
@@ -256,8 +264,7 @@ struct Solver {
 
     PcgFlags read_flags() {
         PcgFlags h{};
-        copy_d2h(c, &h, fl, sizeof(h), true);
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        read_status(c, {{&h, fl, sizeof(h)}});
         return h;
     }
 };
@@ -318,7 +325,6 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     S.col(PC_RZ0, S.part, S.rs.gx, rtol, atol);
     // 4. p = active ? z : 0
     S.row(PS_PINIT);
-    const int32_t *any = &S.fl->any_active;
     const int32_t *cand = &S.fl->need_refresh;
     const int poll = std::max(1, c->pcg_poll);
     // Fixed residual policy with a diagonal or uniform 4x4 preconditioner:
@@ -326,11 +332,13 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     // z = M^-1 R and both reductions in one pass, P update recomputing z).
     const bool fusable = !m->adaptive && !m->ic && (m->nblocks == 0 || m->uniform4);
     for (int k = 0; k < maxiter; ++k) {
+        S.slot = k & 1;
+        const int32_t *any = &S.fl->any_active[S.slot];
         // (iterations of columns that are done, or of a broken solve, are
         // gated off on the device; the poll only ends the loop)
         if (k % poll == 0 && (k > 0 || poll == 1)) {
             const PcgFlags h = S.read_flags();
-            if (h.broke || h.any_active == 0) break;
+            if (h.broke || h.any_active[S.slot] == 0) break;
         }
         const bool refresh = (k + 1) % m->check_interval == 0;
         // sp = S p ; curvature <p, sp>
@@ -338,7 +346,6 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
         S.col(PC_ALPHA, S.part2, S.g2, rtol, atol, false, any);
         if (fusable && !refresh) {
             S.fused(PS_UPD_PRE);
-            S.clear_flag(offsetof(PcgFlags, any_active), 2 * sizeof(int32_t));  // any_active, updated
             S.col(PC_RN_BETA, S.part, S.rs.gx, rtol, atol);
             S.fused(PS_PUPD_PRE);
             continue;
@@ -355,12 +362,15 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
             S.col(PC_TRUE, S.part, S.rs.gx, rtol, atol, refresh, gate);
             S.row(PS_RESTART, nullptr, 0, nullptr, 0, true, gate);
         }
-        // z = M^-1 r ; beta
+        // z = M^-1 r ; beta (counts the next iteration's active columns into
+        // slot ^ 1, which the P update then reads)
         S.precondition(true);
-        S.clear_flag(offsetof(PcgFlags, any_active), sizeof(int32_t));
         S.col(PC_BETA, S.part, S.rs.gx, rtol, atol);
+        S.slot ^= 1;
         S.row(PS_PUPDATE, nullptr, 0, nullptr, 0, true);
+        S.slot ^= 1;
     }
+    S.slot = 0;
     // final acceptance on the original (scaled) equation
     if (final_check) {
         S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);
@@ -370,9 +380,7 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
     Outcome o;
     o.cs.resize(w);
-    copy_d2h(c, o.cs.data(), S.cs, sizeof(ColState) * w, true);
-    copy_d2h(c, &o.fl, S.fl, sizeof(PcgFlags), true);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    read_status(c, {{o.cs.data(), S.cs, sizeof(ColState) * w}, {&o.fl, S.fl, sizeof(PcgFlags)}});
     return o;
 }
 
@@ -409,16 +417,23 @@ void account(kmsd_metric *m, const Outcome &o, int64_t extra_checks) {
     m->restarts += o.fl.restarts;
 }
 
+// Per-column sqrt(sum over groups of the partials' real parts) in a fixed
+// order (colsum_kernel, one workgroup per column), read as w doubles.
+std::vector<double> column_norms(kmsd_metric *m, const double *part, int64_t groups, int32_t w) {
+    kmsd_ctx *c = m->ctx;
+    double *sums = static_cast<double *>(m->work->sums.ensure(c, sizeof(double) * size_t(w)));
+    hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, part, groups, w, sums);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> out(static_cast<size_t>(w));
+    read_status(c, {{out.data(), sums, sizeof(double) * size_t(w)}});
+    for (double &v : out) v = std::sqrt(v);
+    return out;
+}
+
 // Per-column |T| from PS_TRUEGAP partials, summed in a fixed order.
 std::vector<double> true_norms(Solver &S) {
     S.row(PS_TRUEGAP);
-    std::vector<double> h(size_t(S.rs.gx) * size_t(S.w) * 2), out(size_t(S.w), 0.0);
-    copy_d2h(S.c, h.data(), S.part, h.size() * sizeof(double), true);
-    HIPCHECK(hipStreamSynchronize(S.c->stream));
-    for (int64_t g = 0; g < S.rs.gx; ++g)
-        for (int32_t j = 0; j < S.w; ++j) out[size_t(j)] += h[(size_t(g) * S.w + j) * 2];
-    for (double &v : out) v = std::sqrt(v);
-    return out;
+    return column_norms(S.m, S.part, S.rs.gx, S.w);
 }
 
 // Direct solve with the one full-size factor (reference Metric::solve with
@@ -456,8 +471,7 @@ Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo
     S.precondition(false);  // Z = M^-1 q
     copy_into_y(false);
     std::vector<ColState> cs(static_cast<size_t>(w));
-    copy_d2h(c, cs.data(), S.cs, sizeof(ColState) * w, true);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    read_status(c, {{cs.data(), S.cs, sizeof(ColState) * w}});
     for (int step = 0; step < m->refine_maxiter; ++step) {
         S.spmv(S.Y, S.T, S.Q, -1.0, 1.0, nullptr, nullptr);  // T = q - S y
         ++checks;
@@ -484,9 +498,7 @@ Outcome direct_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo
     S.row(PS_RESCALE, nullptr, 0, OUT, ldo);
     Outcome o;
     o.cs.resize(w);
-    copy_d2h(c, o.cs.data(), S.cs, sizeof(ColState) * w, true);
-    copy_d2h(c, &o.fl, S.fl, sizeof(PcgFlags), true);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    read_status(c, {{o.cs.data(), S.cs, sizeof(ColState) * w}, {&o.fl, S.fl, sizeof(PcgFlags)}});
     return o;
 }
 
@@ -578,13 +590,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.n = n;
         s.w = nb;
         const int64_t g = launch_step(c, KMSD_C128, s);
-        std::vector<double> h(size_t(g) * nb * 2), out(nb, 0.0);
-        copy_d2h(c, h.data(), nrm, h.size() * sizeof(double), true);
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        for (int64_t q = 0; q < g; ++q)
-            for (int32_t j = 0; j < nb; ++j) out[j] += h[(size_t(q) * nb + j) * 2];
-        for (double &v : out) v = std::sqrt(v);
-        return out;
+        return column_norms(m, nrm, g, nb);
     };
     const std::vector<double> bn = colnorm(bb);
     std::vector<double> thr(nb);

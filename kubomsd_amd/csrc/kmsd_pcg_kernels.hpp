@@ -18,9 +18,14 @@ struct ColState {
 };
 
 struct PcgFlags {
-    // updated: columns active when the fused iteration's PC_RN_BETA began,
-    // i.e. whose alpha this iteration's P update still has to apply to Y
-    int32_t any_active, updated, broke, need_refresh, nonfinite;
+    // Activity counts by iteration parity s = k & 1: any_active[s] counts the
+    // columns active at the start of iteration k (the set-up writes slot 0;
+    // iteration k's PC_BETA / PC_RN_BETA count into slot s ^ 1); updated[s]
+    // counts the columns active when iteration k's PC_RN_BETA began, i.e.
+    // whose alpha the iteration's P update still has to apply to Y.  Iteration
+    // k's PC_ALPHA launch clears any_active[s ^ 1] and updated[s] (their last
+    // readers ran in iteration k - 1 or k - 2), so the loop needs no memset.
+    int32_t any_active[2], updated[2], broke, need_refresh, nonfinite;
     int64_t refreshes, restarts, checks;
 };
 
@@ -58,6 +63,7 @@ struct PcgArgs {
     double *part;
     int64_t n, ldb, ldo;  // internal blocks have ld = w
     int32_t w, lpr_log2, stage, require_active;  // require_active: skip after breakdown or with no active column
+    int32_t slot;         // flag slot read by require_active (any_active[slot], or updated[slot] for PS_PUPD_PRE)
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
 
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
     // the loop); set-up and final-acceptance stages always run.
     // (the fused P update also finishes the iteration's Y update, so it runs
     // when any column was active at the iteration's start)
-    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated : a.fl->any_active;
+    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated[a.slot] : a.fl->any_active[a.slot];
     if (a.require_active && (a.fl->broke || !live)) return;
     if (a.gate && *a.gate == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -393,20 +399,37 @@ struct ColArgs {
     double rtol, atol, gap_ratio;
     double imag_tol;      // breakdown: |Im <p,Sp>| > imag_tol |Re <p,Sp>|
     int32_t w, stage, adaptive, refresh;
+    int32_t slot;         // iteration parity k & 1 (flag slots of PcgFlags)
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
 };
 
 static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
     const int j = blockIdx.x;
     if (j >= a.w) return;
+    if (a.stage == PC_ALPHA && j == 0 && threadIdx.x == 0) {
+        a.fl->any_active[a.slot ^ 1] = 0;
+        a.fl->updated[a.slot] = 0;
+    }
     if (a.gate && *a.gate == 0) return;
     if (a.stage != PC_SAFE && a.stage != PC_FINAL && a.stage != PC_INIT && a.stage != PC_RZ0 && a.fl->broke) return;
     __shared__ double sre[256], sim[256];
     const bool is_max = a.stage == PC_SAFE;
     double re = 0.0, im = 0.0;
     if (a.part) {
+        // groups t, t + 256, ... in ascending order; eight loads in flight
         const cx<double> *p = reinterpret_cast<const cx<double> *>(a.part);
-        for (int64_t g = threadIdx.x; g < a.groups; g += 256) {
+        int64_t g = threadIdx.x;
+        for (; g + 7 * 256 < a.groups; g += 8 * 256) {
+            cx<double> v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = p[(g + u * 256) * a.w + j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                re = is_max ? fmax(re, v[u].re) : re + v[u].re;
+                im += v[u].im;
+            }
+        }
+        for (; g < a.groups; g += 256) {
             const cx<double> v = p[g * a.w + j];
             re = is_max ? fmax(re, v.re) : re + v.re;
             im += v.im;
@@ -439,7 +462,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             s.qn = sqrt(tre);
             s.thr = fmax(a.rtol * s.qn, a.atol / s.safe);
             s.active = s.qn > s.thr;
-            if (s.active) atomicAdd(&f->any_active, 1);
+            if (s.active) atomicAdd(&f->any_active[0], 1);
             break;
         }
         case PC_RZ0:
@@ -464,7 +487,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             break;
         }
         case PC_TRUE: {
-            if (j == 0 && f->any_active > 0) {
+            if (j == 0 && f->any_active[a.slot] > 0) {
                 atomicAdd(reinterpret_cast<unsigned long long *>(&f->refreshes), 1ull);
                 atomicAdd(reinterpret_cast<unsigned long long *>(&f->checks), 1ull);
             }
@@ -484,7 +507,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         case PC_BETA: {
             s.beta = s.active ? tre / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
             s.rz = tre;
-            if (s.active) atomicAdd(&f->any_active, 1);
+            if (s.active) atomicAdd(&f->any_active[a.slot ^ 1], 1);
             break;
         }
         case PC_RN_BETA: {
@@ -494,11 +517,11 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             s.rn = sqrt(tre);
             s.iters += s.active;
             s.restart = 0;
-            if (s.active) atomicAdd(&f->updated, 1);
+            if (s.active) atomicAdd(&f->updated[a.slot], 1);
             s.active = s.active && s.rn > s.thr;
             s.beta = s.active ? tim / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
             s.rz = tim;
-            if (s.active) atomicAdd(&f->any_active, 1);
+            if (s.active) atomicAdd(&f->any_active[a.slot ^ 1], 1);
             break;
         }
         case PC_FINAL: {
@@ -508,6 +531,33 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         case PC_CLEAR:
             break;
     }
+}
+
+// out[j] = sum of the real parts of partials [groups][w] (complex) in column
+// j: one workgroup per column, each lane adding groups t, t + 256, ... in
+// ascending order, then a fixed LDS tree (the reduction of pcg_col_kernel),
+// so the sums do not depend on timing.
+static __global__ __launch_bounds__(256) void colsum_kernel(const double *part, int64_t groups, int32_t w,
+                                                            double *out) {
+    __shared__ double red[256];
+    const int64_t j = blockIdx.x;
+    double acc = 0.0;
+    int64_t q = threadIdx.x;
+    for (; q + 7 * 256 < groups; q += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[((q + u * 256) * w + j) * 2];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; q < groups; q += 256) acc += part[(q * w + j) * 2];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
 }
 
 // y128 += scale_j * d64 (refinement update), and r64 = r128 / scale_j.
@@ -613,7 +663,7 @@ template <class VT, int CPL>
 __global__ __launch_bounds__(256) void pcg_quad_kernel(const PcgArgs a) {
     using V = cx<VT>;
     using PK = Pack<VT, CPL>;
-    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated : a.fl->any_active;
+    const int32_t live = a.stage == PS_PUPD_PRE ? a.fl->updated[a.slot] : a.fl->any_active[a.slot];
     if (a.require_active && (a.fl->broke || !live)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lpr = 1 << a.lpr_log2;
