@@ -88,11 +88,14 @@ def parse():
     p.add_argument("--method", choices=["stepwise", "spectral"], default="stepwise",
                    help="square: the stochastic estimator (spectral: VAC by segmented Chebyshev products, "
                         "the mu tables on the FP64 matrix cores)")
-    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise", "square"],
+    p.add_argument("--workload", choices=["honeycomb", "nonorthogonal", "mu", "stepwise", "square", "flake"],
                    default="honeycomb",
                    help="honeycomb: orthogonal moment chain (default); nonorthogonal: simple-cubic 64^3 x 4 "
                         "orbitals, S^-1 H chain with batched block-Jacobi PCG; mu: square 2048^2 double-moment "
-                        "tables mu_mn on the matrix cores")
+                        "tables mu_mn on the matrix cores; flake: one nonorthogonal serving job alone")
+    p.add_argument("--flake-n", type=float, default=17672, help="flake: orbitals (about)")
+    p.add_argument("--pcg-min-blocks", type=int, default=0,
+                   help="override the least row blocks per workgroup of PCG vector passes")
     p.add_argument("--side", type=int, default=2048, help="mu: square lattice side")
     p.add_argument("--samples", type=int, default=32, help="mu: random vectors (one table each)")
     p.add_argument("--atoms", type=int, default=64, help="nonorthogonal: atoms per cube side")
@@ -103,7 +106,7 @@ def parse():
     a = p.parse_args()
     nonorth = a.workload == "nonorthogonal"
     a.vectors = a.vectors or (16 if a.workload == "square" else 32 if nonorth else 64)
-    a.moments = a.moments or {"honeycomb": 2048, "stepwise": 2048, "square": 512}.get(a.workload, 1024)
+    a.moments = a.moments or {"honeycomb": 2048, "stepwise": 2048, "square": 512, "flake": 256}.get(a.workload, 1024)
     return a
 
 
@@ -175,6 +178,52 @@ def serving(args):
                         "vector_orders_total": work, "vector_orders_per_s": work / wall,
                         "diffusion_rel_err_vs_fp64": errs}}
     print(json.dumps(line))
+
+
+def flake(args):
+    """One serving job alone: a nonorthogonal honeycomb flake of about
+    --flake-n orbitals (S = I + 0.08 U[-1,1] on the bonds, diagonal-
+    preconditioned PCG to 1e-10 per order), --vectors vectors, --moments
+    moments, stepwise DOS + MSD + D(E) at t = 0, 0.5, 1 fs in mixed precision,
+    the job shape of the serving trace.  Reports seconds per job (a first run
+    warms up; the median of --steps timed runs)."""
+    import torch
+    from kubomsd_amd.backends.hip import HipContext
+    from kubomsd_amd.config import RunConfig
+    from kubomsd_amd.serving import FlakeModel, TransportJob, estimate_job_bytes, run_job
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    ctx = HipContext(0)
+    if args.min_blocks:
+        ctx.set_tuning("min_blocks", args.min_blocks)
+    if args.pcg_min_blocks:
+        ctx.set_tuning("pcg_min_blocks", args.pcg_min_blocks)
+    if args.pcg_poll:
+        ctx.set_tuning("pcg_poll", args.pcg_poll)
+    L = max(2, int(round(np.sqrt(args.flake_n / 2))))
+    n, nb, M = 2 * L * L, args.vectors, args.moments
+    cfg = RunConfig(energy_eV=np.linspace(-7.5, 7.5, 61), time_fs=[0.0, 0.5, 1.0], solver="stochastic-kpm",
+                    kernel="jackson", backend="hip", n_moments=M, axes=("x",), nsamples=nb, sample_batch=nb,
+                    statistics_block=max(1, nb // 4), include_vac=False, precision="mixed", metric_rtol=1e-10,
+                    seed=SEED, autotune=False)
+    job = TransportJob(0, FlakeModel(n, 1.0, True, 77), cfg, "stepwise", None, 0.0,
+                       estimate_job_bytes(n, 4 * n, nb, 1, "mixed", True))
+    run_job(job, ctx)
+    times = []
+    for _ in range(max(1, args.steps)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_job(job, ctx)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    print(json.dumps({"metric": "seconds per serving job (nonorthogonal flake, alone)", "value": sec, "unit": "s",
+                      "n_gpus": 1, "steps": len(times), "warmup": 1, "ms_per_step": 1e3 * sec,
+                      "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c64+f64",
+                      "data": "synthetic (seeded honeycomb flake, nonorthogonal overlap)",
+                      "config": {"workload": f"nonorthogonal honeycomb flake N={n}, {nb} vectors, M={M}, "
+                                             "stepwise t = 0, 0.5, 1 fs, diagonal-preconditioned PCG to 1e-10",
+                                 "n": n, "vectors": nb, "moments": M},
+                      "runs_s": times}))
 
 
 # |lambda(S^-1 H)| <= |H|_2 / lambda_min(S) for the seeded simple-cubic model
@@ -829,6 +878,11 @@ def main():
         if args.gpus != 1:
             sys.exit("this build runs on one GPU only")
         square(args)
+        return
+    if args.workload == "flake":
+        if args.gpus != 1:
+            sys.exit("this build runs on one GPU only")
+        flake(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sample sharding is a design "

@@ -118,7 +118,8 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * row-walking launches use at most one workgroup per this many row blocks,
  * so small operators get fewer, fuller workgroups and fewer partial sums),
  * "pcg_min_blocks" (1..64, default 4: the same for the vector passes of PCG
- * solves, whose S products then use no more workgroups than the passes;
+ * solves, keeping at least 512 workgroups where there are that many row
+ * blocks; their S products then use no more workgroups than the passes;
  * both change the partial-sum partition, so sums round differently). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).

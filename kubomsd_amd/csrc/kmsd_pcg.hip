@@ -48,7 +48,10 @@ RowShape row_shape(kmsd_ctx *c, int64_t n, int32_t w, int dtype, int cap) {
     s.gy = int((lanes + lpr - 1) / lpr);
     const int64_t rpb = 4 * (64 / lpr);
     const int64_t nblk = (n + rpb - 1) / rpb;
-    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, groups_for_blocks(c, nblk, c->pcg_min_blocks)));
+    // at least pcg_min_blocks row blocks per workgroup, but no fewer than
+    // 512 workgroups (two per CU) where there are that many row blocks
+    const int64_t floor_g = std::min<int64_t>(512, groups_for_blocks(c, nblk, 1));
+    s.gx = std::max<int64_t>(8, std::min<int64_t>(c->groups, std::max(floor_g, groups_for_blocks(c, nblk, c->pcg_min_blocks))));
     return s;
 }
 
