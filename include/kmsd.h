@@ -120,7 +120,13 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * "pcg_min_blocks" (1..64, default 4: the same for the vector passes of PCG
  * solves, keeping at least 512 workgroups where there are that many row
  * blocks; their S products then use no more workgroups than the passes;
- * both change the partial-sum partition, so sums round differently). */
+ * both change the partial-sum partition, so sums round differently),
+ * "cheb_inner" (1, the default: complex64 metric solves whose S is CSR/ELL
+ * with the diagonal or no preconditioner and has a Gershgorin interval
+ * [lo, hi] with lo > 0 and hi <= 4 lo take Chebyshev-series inner solves
+ * inside the FP64 refinement, one fused product per term and no
+ * reductions; the acceptance stays the FP64 true residual; 0: PCG inner
+ * solves). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an

@@ -195,6 +195,8 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
         } else if (k == "pcg_poll") {
             if (value < 1) throw Error(KMSD_EINVAL, "pcg_poll must be positive");
             c->pcg_poll = int(value);
+        } else if (k == "cheb_inner") {
+            c->cheb_inner = value != 0;
         } else if (k == "min_blocks" || k == "pcg_min_blocks") {
             if (value < 1 || value > 64) throw Error(KMSD_EINVAL, "min_blocks must be in [1, 64]");
             (k == "min_blocks" ? c->min_blocks : c->pcg_min_blocks) = int(value);

@@ -26,6 +26,7 @@ struct kmsd_ctx {
     bool auto_groups = true; // fused steps: one workgroup per resident slot of the kernel (else `groups`)
     int min_blocks = 1;      // row blocks per workgroup at least (small operators: fewer, fuller workgroups)
     int pcg_min_blocks = 4;  // the same for the vector passes and products of PCG solves
+    bool cheb_inner = true;  // complex64 refinement: Chebyshev inner solves where S's Gershgorin interval allows
     int cpl[2] = {0, 0};    // columns per lane of complex64 [0] and complex128 [1] blocks; 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
@@ -329,6 +330,11 @@ struct kmsd_metric {
     std::vector<double> rep_res, rep_rel;
     struct Work;
     Work *work = nullptr;
+    // Chebyshev inner solves of the FP64 refinement: S's Gershgorin interval
+    // [cheb_lo, cheb_hi]; cheb_state 0 not yet examined, 1 positive and
+    // narrow enough, 2 not usable.
+    int cheb_state = 0;
+    double cheb_lo = 0, cheb_hi = 0;
     kmsd_matrix *W = nullptr;  // Z S Z for the sampler (built on first use)
     kmsd::DBuf sb1, sb2;       // sampler Clenshaw buffers
     kmsd::DBuf pinner;         // partial sums of kmsd_metric_inner

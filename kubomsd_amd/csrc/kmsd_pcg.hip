@@ -321,11 +321,16 @@ Outcome pcg_run(Solver &S, const void *B, int64_t ldb, void *OUT, int64_t ldo, d
     S.row(PS_ABSMAX, B, ldb);
     S.col(PC_SAFE, S.part, S.rs.gx, rtol, atol);
     // 2. q = b/safe, r = q, y = 0 ; |q| -> thr, activity
-    S.row(PS_SCALE, B, ldb);
-    S.col(PC_INIT, S.part, S.rs.gx, rtol, atol);
-    // 3. z = M^-1 r, rz
-    S.precondition(false);
-    S.col(PC_RZ0, S.part, S.rs.gx, rtol, atol);
+    // 3. z = M^-1 r, rz (with the diagonal preconditioner in the same pass)
+    if (!m->ic && m->nblocks == 0) {
+        S.row(PS_SCALE_DIAG, B, ldb);
+        S.col(PC_INIT_RZ0, S.part, S.rs.gx, rtol, atol);
+    } else {
+        S.row(PS_SCALE, B, ldb);
+        S.col(PC_INIT, S.part, S.rs.gx, rtol, atol);
+        S.precondition(false);
+        S.col(PC_RZ0, S.part, S.rs.gx, rtol, atol);
+    }
     // 4. p = active ? z : 0
     S.row(PS_PINIT);
     const int32_t *cand = &S.fl->need_refresh;
@@ -540,6 +545,114 @@ void direct_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y
                                         : "S solve failed true-residual acceptance");
 }
 
+// Whether the complex64 refinement may take Chebyshev inner solves: S in
+// CSR/ELL form with real or complex values, no block, IC(0) or direct
+// preconditioner, and a Gershgorin interval [lo, hi] with lo > 0 and
+// hi <= kChebKappa lo (then a series of about a dozen terms reduces the
+// residual by 1e-6).  The interval is computed once per metric.
+constexpr double kChebKappa = 4.0;
+
+bool cheb_usable(kmsd_metric *m) {
+    kmsd_ctx *c = m->ctx;
+    if (!c->cheb_inner || m->ic || m->nblocks || m->direct || m->identity) return false;
+    if (m->cheb_state) return m->cheb_state == 1;
+    m->cheb_state = 2;
+    const kmsd_matrix *S = m->S;
+    if (!S || S->fmt == KMSD_BSR4 || S->vkind == VK_IMAG || !S->val64) return false;
+    const int64_t n = S->n;
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 256));
+    double *out = static_cast<double *>(m->work->sums.ensure(c, sizeof(double) * 2 * size_t(g)));
+    const void *dg = S->has_diag ? S->dval64 : nullptr;
+    if (S->vkind == VK_COMPLEX)
+        hipLaunchKernelGGL((gershgorin_kernel<double, true>), dim3(unsigned(g)), dim3(256), 0, c->stream, S->val64, dg,
+                           S->ptr, S->ell, n, out);
+    else
+        hipLaunchKernelGGL((gershgorin_kernel<double, false>), dim3(unsigned(g)), dim3(256), 0, c->stream, S->val64,
+                           dg, S->ptr, S->ell, n, out);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> h(2 * size_t(g));
+    read_status(c, {{h.data(), out, h.size() * sizeof(double)}});
+    double lo = h[0], hi = h[1];
+    for (int64_t q = 1; q < g; ++q) {
+        lo = std::min(lo, h[2 * q]);
+        hi = std::max(hi, h[2 * q + 1]);
+    }
+    if (std::isfinite(lo) && std::isfinite(hi) && lo > 0 && hi <= kChebKappa * lo) {
+        m->cheb_state = 1;
+        m->cheb_lo = lo;
+        m->cheb_hi = hi;
+    }
+    return m->cheb_state == 1;
+}
+
+// Inner solve of the FP64 refinement by a Chebyshev series of S^-1 on S's
+// Gershgorin interval [lo, hi] (center c, half width h): d = sum_k g_k
+// T_k((S - c)/h) r with g_0 = 1/(h s), g_k = 2 (-rho)^k/(h s), z = c/h,
+// s = sqrt(z^2 - 1), rho = z - s (the expansion of 1/(z + t)).  Each term
+// is one fused step: T_(k+1) = (2/h) S T_k - (2c/h) T_k - T_(k-1) with
+// d += g_(k+1) T_(k+1) accumulated in the same launch; no reductions and no
+// host round trips.  The tail bound |r - S d_K| <= hi (2/(h s))
+// rho^(K+1)/(1 - rho) |r| sets the term count for the requested residual
+// reduction.  r is overwritten (it holds T_(k-1) in turn).  Returns the
+// number of products.
+int cheb_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction) {
+    kmsd_ctx *c = m->ctx;
+    const int64_t n = m->n;
+    const double lo = m->cheb_lo, hi = m->cheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
+    if (!(h > 1e-14 * cc)) {
+        launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{1.0 / cc, 0.0});  // S = c I
+        return 0;
+    }
+    const double z = cc / h, sq = std::sqrt(z * z - 1.0), rho = z - sq, amp = 2.0 / (h * sq);
+    int K = 1;
+    while (K < 400 && hi * amp * std::pow(rho, double(K + 1)) / (1.0 - rho) > reduction) ++K;
+    void *t1 = m->work->t.ensure(c, size_t(n) * size_t(nb) * 8);
+    launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{0.5 * amp, 0.0});
+    double g = -amp * rho;
+    Step s;
+    s.A = m->S;
+    s.X = r;
+    s.ldx = nb;
+    s.alpha = Cplx{1.0 / h, 0.0};
+    s.use_beta = true;
+    s.beta = Cplx{-cc / h, 0.0};
+    s.OUT = t1;
+    s.ldo = nb;
+    s.ACC = d;
+    s.lda = nb;
+    s.coef = Cplx{g, 0.0};
+    s.n = n;
+    s.w = nb;
+    s.f32_matrix = true;
+    launch_step(c, KMSD_C64, s);
+    void *prev = r, *cur = t1;
+    for (int k = 1; k < K; ++k) {
+        g *= -rho;
+        Step u;
+        u.A = m->S;
+        u.X = cur;
+        u.ldx = nb;
+        u.alpha = Cplx{2.0 / h, 0.0};
+        u.use_beta = true;
+        u.beta = Cplx{-2.0 * cc / h, 0.0};
+        u.P = prev;
+        u.ldp = nb;
+        u.use_p = true;
+        u.gamma = -1.0;
+        u.OUT = prev;  // T_(k+1) replaces T_(k-1) row by row (each row read before it is written)
+        u.ldo = nb;
+        u.ACC = d;
+        u.lda = nb;
+        u.coef = Cplx{g, 0.0};
+        u.n = n;
+        u.w = nb;
+        u.f32_matrix = true;
+        launch_step(c, KMSD_C64, u);
+        std::swap(prev, cur);
+    }
+    return K;
+}
+
 }  // namespace
 
 void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb, double rtol,
@@ -632,8 +745,19 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream,
                            rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
         HIPCHECK(hipGetLastError());
-        Solver S = make_solver(m, KMSD_C64, nb);
-        const Outcome o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter, false);
+        Outcome o;
+        if (cheb_usable(m)) {
+            const int K = cheb_run(m, r64, d64, nb, inner_rtol);
+            o.cs.assign(size_t(nb), ColState{});
+            for (ColState &x : o.cs) {
+                x.iters = K;
+                x.fin = 1;
+            }
+            o.fl = PcgFlags{};
+        } else {
+            Solver S = make_solver(m, KMSD_C64, nb);
+            o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter, false);
+        }
         for (int32_t j = 0; j < nb; ++j) {
             agg[j].iters += o.cs[j].iters;
             agg[j].breakdown |= o.cs[j].breakdown;

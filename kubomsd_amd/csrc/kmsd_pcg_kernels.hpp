@@ -48,6 +48,9 @@ enum : int32_t {
     PS_UPD_PRE = 10,   // R -= alpha SP ; partial |R|^2 (re), Re<R,z> (im)
     PS_PUPD_PRE = 11,  // Y += alpha P ; P = active ? z + (restart ? 0 : beta P) : 0
     PS_PRE_DOT = 12,   // uniform 4x4 blocks: Z = M^-1 R stored ; partial Re<R,Z> (one pass for z and the dot)
+    // Set-up with the diagonal preconditioner: PS_SCALE and PS_DIAG in one
+    // pass (Q = R = B/safe, Y = 0, Z = R/d; partials |Q|^2 (re), Re<R,Z> (im)).
+    PS_SCALE_DIAG = 13,
 };
 
 struct PcgArgs {
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
             const ColState &s = a.cs[col + c];
             act[c] = s.active;
             rst[c] = s.restart;
-            cf[c] = (a.stage == PS_SCALE) ? 1.0 / s.safe
+            cf[c] = (a.stage == PS_SCALE || a.stage == PS_SCALE_DIAG) ? 1.0 / s.safe
                     : (a.stage == PS_UPDATE || a.stage == PS_UPD_PRE) ? s.alpha
                     : (a.stage == PS_PUPDATE || a.stage == PS_PUPD_PRE) ? s.beta
                     : (a.stage == PS_RESCALE) ? s.safe
@@ -154,6 +157,31 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
                 }
                 q.store(Q + o);
                 q.store(R + o);
+                PK zero;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) zero.set(c, cx<double>{0.0, 0.0});
+                zero.store(Y + o);
+                break;
+            }
+            case PS_SCALE_DIAG: {
+                PK b, q, z;
+                b.load(B + row * a.ldb + col);
+                const double d = a.diag[row];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    cx<double> v = b.get(c);
+                    v.re *= cf[c];
+                    v.im *= cf[c];
+                    q.set(c, v);
+                    const cx<double> qv = q.get(c);
+                    acc_re[c] += qv.re * qv.re + qv.im * qv.im;
+                    z.set(c, cx<double>{qv.re / d, qv.im / d});
+                    const cx<double> zv = z.get(c);
+                    acc_im[c] += qv.re * zv.re + qv.im * zv.im;
+                }
+                q.store(Q + o);
+                q.store(R + o);
+                z.store(Z + o);
                 PK zero;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) zero.set(c, cx<double>{0.0, 0.0});
@@ -336,7 +364,8 @@ __global__ __launch_bounds__(256) void pcg_row_kernel(const PcgArgs a) {
     }
 
     const bool reduces = a.stage == PS_ABSMAX || a.stage == PS_SCALE || a.stage == PS_UPDATE ||
-                         a.stage == PS_DIAG || a.stage == PS_DOT || a.stage == PS_TRUEGAP || a.stage == PS_UPD_PRE;
+                         a.stage == PS_DIAG || a.stage == PS_DOT || a.stage == PS_TRUEGAP || a.stage == PS_UPD_PRE ||
+                         a.stage == PS_SCALE_DIAG;
     if (a.stage == PS_RESCALE) {
         // nonfinite columns: flag per column (vector atomics on global memory)
 #pragma unroll
@@ -389,6 +418,7 @@ enum : int32_t {
     PC_FINAL = 7,   // final true residual -> report fields
     PC_CLEAR = 8,   // zero flag block + restart marks
     PC_RN_BETA = 9, // fused iteration: rn, iterations, activity (re = |r|^2), then beta, rz (im = Re<r,z>)
+    PC_INIT_RZ0 = 10, // PC_INIT from re, then PC_RZ0 from im (after PS_SCALE_DIAG)
 };
 
 struct ColArgs {
@@ -411,7 +441,9 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         a.fl->updated[a.slot] = 0;
     }
     if (a.gate && *a.gate == 0) return;
-    if (a.stage != PC_SAFE && a.stage != PC_FINAL && a.stage != PC_INIT && a.stage != PC_RZ0 && a.fl->broke) return;
+    if (a.stage != PC_SAFE && a.stage != PC_FINAL && a.stage != PC_INIT && a.stage != PC_RZ0 &&
+        a.stage != PC_INIT_RZ0 && a.fl->broke)
+        return;
     __shared__ double sre[256], sim[256];
     const bool is_max = a.stage == PC_SAFE;
     double re = 0.0, im = 0.0;
@@ -467,6 +499,13 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         }
         case PC_RZ0:
             s.rz = tre;
+            break;
+        case PC_INIT_RZ0:
+            s.qn = sqrt(tre);
+            s.thr = fmax(a.rtol * s.qn, a.atol / s.safe);
+            s.active = s.qn > s.thr;
+            if (s.active) atomicAdd(&f->any_active[0], 1);
+            s.rz = tim;
             break;
         case PC_ALPHA: {
             // curvature = <p, S p>: re = tre, im = tim
@@ -530,6 +569,52 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         }
         case PC_CLEAR:
             break;
+    }
+}
+
+// Gershgorin interval of a CSR/ELL matrix with a split diagonal: per
+// workgroup, min over rows of Re(S_ii) - r_i and max of Re(S_ii) + r_i with
+// r_i = sum over j != i of |S_ij| (ELL padding slots hold zero); out[2g],
+// out[2g + 1].
+template <class R, bool MC>
+__global__ __launch_bounds__(256) void gershgorin_kernel(const void *offv, const void *diagv, const int32_t *ptr,
+                                                         int32_t ell, int64_t n, double *out) {
+    using MT = typename MatVal<R, MC>::type;
+    const MT *off = static_cast<const MT *>(offv);
+    const MT *dg = static_cast<const MT *>(diagv);
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) {
+        const int64_t k0 = ell ? r * ell : ptr[r], k1 = ell ? k0 + ell : ptr[r + 1];
+        double rad = 0.0, cen = 0.0;
+        for (int64_t k = k0; k < k1; ++k) {
+            if constexpr (MC)
+                rad += sqrt(double(off[k].re) * double(off[k].re) + double(off[k].im) * double(off[k].im));
+            else
+                rad += fabs(double(off[k]));
+        }
+        if (dg) {
+            if constexpr (MC)
+                cen = double(dg[r].re);
+            else
+                cen = double(dg[r]);
+        }
+        lo = fmin(lo, cen - rad);
+        hi = fmax(hi, cen + rad);
+    }
+    __shared__ double slo[256], shi[256];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off2 = 128; off2 > 0; off2 >>= 1) {
+        if (int(threadIdx.x) < off2) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + off2]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + off2]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = slo[0];
+        out[2 * blockIdx.x + 1] = shi[0];
     }
 }
 

@@ -57,6 +57,7 @@ class TransportJob:
     size_class: int = 0
     meta: dict = field(default_factory=dict)
     group_key: object = None  # jobs with equal keys may share launches (None: always alone)
+    est_work: float = 0.0  # relative service-time estimate (orders jobs that arrive together)
 
 
 @dataclass
@@ -132,11 +133,24 @@ class TransportServer:
             self.pending += 1
             self.cv.notify_all()
 
-    def _pick(self, now):
+    def _pick(self, now, longest=False):
         heads = [q[0] for q in self.queues if q]
         if not heads:
             return None
-        oldest = min(heads, key=lambda j: j.arrival_s)
+        # The oldest arrival first.  Among jobs that arrived together, slot 0
+        # (longest=True) takes the one with the most estimated work and the
+        # other slots the one with the least: the longest jobs start at once
+        # on one slot while the short ones stream through the others, so a
+        # batch arriving at once neither waits behind its longest jobs nor
+        # finishes with them (first in queue order on equal estimates).
+        t_old = min(j.arrival_s for j in heads)
+        oldest = None
+        for q in self.queues:
+            for j in q:
+                if j.arrival_s != t_old:
+                    continue
+                if oldest is None or (j.est_work > oldest.est_work if longest else j.est_work < oldest.est_work):
+                    oldest = j
         free = self.budget - self.used
         if oldest.est_bytes <= free:
             return oldest
@@ -156,7 +170,7 @@ class TransportServer:
         self.peak_used = max(self.peak_used, self.used)
         self.peak_running = max(self.peak_running, self.running)
 
-    def _take(self):
+    def _take(self, slot=1):
         """The next admitted work for a slot: a list of one job, or of up to
         group_max waiting jobs with the first one's group key (in arrival
         order, each within the budget left)."""
@@ -164,7 +178,7 @@ class TransportServer:
             while True:
                 if self.closed and self.pending == 0:
                     return None
-                job = self._pick(self.clock())
+                job = self._pick(self.clock(), longest=slot == 0 and self.slots > 1)
                 if job is not None:
                     self._admit(job)
                     group = [job]
@@ -273,7 +287,7 @@ class TransportServer:
 
     def _dispatch(self, slot):
         while True:
-            group = self._take()
+            group = self._take(slot)
             if group is None:
                 return
             start = self.clock()
@@ -514,6 +528,10 @@ def synthetic_trace(n_jobs=256, rate_per_s=4.0, n_range=(1e4, 4e6), moments=(256
         # launch-bound jobs (a Chebyshev order moves under ~32 MB) share launches
         # with others of the same shape
         key = (M, nb, tuple(times), precision) if not nonorth and n_act * nb <= group_elements else None
+        # service-time estimate: orbitals x orders x vectors, a PCG solve per order costing
+        # about ten products when the overlap is nonorthogonal
+        work = float(n_act) * M * nb * (10.0 if nonorth else 1.0)
         jobs.append(TransportJob(i, build, cfg, "stepwise", None, float(arrivals[i]), est,
-                                 meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth), group_key=key))
+                                 meta=dict(n=n_act, M=M, vectors=nb, W=W, nonorthogonal=nonorth), group_key=key,
+                                 est_work=work))
     return jobs

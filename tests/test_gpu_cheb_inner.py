@@ -1,0 +1,105 @@
+"""Chebyshev inner solves of the complex64 metric refinement (tuning key
+"cheb_inner", on by default).  Where S is stored as CSR/ELL with the
+diagonal or no preconditioner and its Gershgorin interval [lo, hi] has
+lo > 0 and hi <= 4 lo, each FP32 inner solve is a Chebyshev series of S^-1
+on that interval, one fused step per term with no reductions; the FP64
+refinement accepts on the FP64 true residual exactly as with PCG inner
+solves.  Checked against SciPy's solve of the complex64-rounded right-hand
+side (one complex64 rounding of the output), against the PCG inner solves
+(cheb_inner = 0), with real and complex Hermitian S, and through the
+engine's moments of S^-1 H against the FP64 oracle.  A matrix whose
+interval is not positive keeps the PCG inner solves, bit for bit."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from helpers import host, random_block, rel_err, to_dev
+from kubomsd_amd.synthetic import honeycomb_flake
+from oracle import native as O
+
+pytestmark = pytest.mark.gpu
+
+
+def flake_S(n=700, seed=11, complex_values=False):
+    """S = I + 0.08 U on the bonds of a honeycomb flake (Gershgorin interval
+    within [0.76, 1.24]); complex_values: Hermitian with complex bond values."""
+    m = honeycomb_flake(n, s_scale=0.08, seed=seed)
+    S = sp.csr_matrix(m.S)
+    if complex_values:
+        up = sp.triu(S, k=1).tocoo()
+        rng = np.random.default_rng(seed)
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, up.nnz))
+        U = sp.coo_matrix((up.data * ph, (up.row, up.col)), shape=S.shape)
+        S = (sp.identity(S.shape[0], format="csr") + U + U.conj().T).tocsr()
+    S.sort_indices()
+    return m, S
+
+
+def solve(ctx, S, B, flag):
+    from kubomsd_amd.backends.hip import HipMetric
+    ctx.set_tuning("cheb_inner", flag)
+    try:
+        met = HipMetric(ctx, S, S.diagonal().real)
+        Y, rep = met.solve(to_dev(ctx, B, "c64"), 1e-10, 0.0, 1000)
+        return host(Y), rep
+    finally:
+        ctx.set_tuning("cheb_inner", 1)
+
+
+@pytest.mark.parametrize("complex_values", [False, True])
+def test_cheb_inner_solves_reach_fp64_residual(hip_ctx, complex_values):
+    _, S = flake_S(complex_values=complex_values)
+    B = random_block(S.shape[0], 16, 7)
+    Bq = B.astype(np.complex64).astype(np.complex128)
+    ref = spla.spsolve(S.tocsc(), Bq)
+    out = {}
+    for flag in (1, 0):
+        Y, rep = solve(hip_ctx, S, B, flag)
+        assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+        res = np.linalg.norm(Bq - S @ Y, axis=0) / np.linalg.norm(Bq, axis=0)
+        assert res.max() < 1e-6  # the solution is rounded once to complex64
+        assert rel_err(Y, ref) < 2e-7
+        out[flag] = (Y, rep)
+    # the series runs the same number of terms on every column; the two
+    # inner solvers round differently
+    assert len(set(out[1][1]["iterations"])) == 1 and out[1][1]["iterations"][0] > 0
+    assert not np.array_equal(out[1][0], out[0][0])
+    assert rel_err(out[1][0], out[0][0]) < 2e-7
+
+
+def test_pcg_kept_where_the_interval_is_not_positive(hip_ctx):
+    """A well-conditioned SPD matrix that is not diagonally dominant: its
+    Gershgorin interval reaches below zero, so the inner solves stay PCG and
+    the two settings give the same bits."""
+    rng = np.random.default_rng(5)
+    n = 300
+    A = sp.random(n, n, density=0.03, random_state=7, format="csr")
+    S = (A @ A.T + 0.5 * sp.identity(n)).astype(np.complex128).tocsr()
+    S.sort_indices()
+    d = S.diagonal().real
+    off = np.asarray(abs(S).sum(axis=1)).ravel() - np.abs(S.diagonal())
+    assert (d - off).min() <= 0  # not usable for the series
+    B = rng.normal(size=(n, 8)) + 1j * rng.normal(size=(n, 8))
+    Y1, rep1 = solve(hip_ctx, S, B, 1)
+    Y0, rep0 = solve(hip_ctx, S, B, 0)
+    assert all(rep1["converged"])
+    np.testing.assert_array_equal(Y1, Y0)
+    assert list(rep1["iterations"]) == list(rep0["iterations"])
+
+
+def test_flake_moments_with_cheb_inner_match_oracle(hip_ctx):
+    """Moments of S^-1 H (complex64 blocks, each order one H product and one
+    refined S solve with Chebyshev inner solves) against the FP64 oracle
+    engine, as the complex64 engine tests bound them."""
+    from kubomsd_amd.backends.hip import HipEngine, HipMetric
+    m, S = flake_S(n=400, seed=3)
+    H = sp.csr_matrix(m.H)
+    rt = 2e-13
+    oe = O.Engine(O.Csr(H.toarray()), O.Metric(O.Csr(S.toarray()), S.diagonal().real, ()), [(-1, [])] * 3, rt,
+                  0.0, 1000)
+    he = HipEngine(hip_ctx, H, HipMetric(hip_ctx, S, S.diagonal().real), [(-1, [])] * 3, rt, 0.0, 1000)
+    B = random_block(H.shape[0], 4, 13)
+    got = host(he.moments(to_dev(hip_ctx, B, "c64"), 24, 0.0, 12.0))
+    want = oe.moments(B, 24, 0.0, 12.0)
+    assert rel_err(got, want) < 2e-4

@@ -12,6 +12,7 @@ class _Srv(TransportServer):
         self.pending, self.closed, self.running, self.peak_used, self.peak_running = 0, False, 0, 0, 0
         self.t0 = 0.0
         self.group_max, self.groups_formed = 8, []
+        self.slots = 4
 
     def clock(self):
         return self.now
@@ -35,6 +36,48 @@ def test_oldest_fitting_job_first_and_small_jobs_pass_a_blocked_large_one():
     assert s._pick(1.0) is None
     s._release(job(0, 900, 0.0))
     assert s._take()[0].job_id == 3
+
+
+def test_jobs_arriving_together_split_longest_and_shortest_first():
+    """Among jobs with the same arrival time slot 0 takes the one with the
+    most estimated work and the other slots the one with the least (queue
+    order on equal estimates); distinct arrival times keep first-come order
+    whatever the estimates."""
+    s = _Srv(1000)
+    s.now = 1.0
+    jobs = [TransportJob(i, None, None, arrival_s=0.0, est_bytes=100, est_work=w)
+            for i, w in enumerate((1.0, 5.0, 3.0, 5.0, 1.0))]
+    for j in jobs:
+        s.submit(j)
+    assert [s._take(0)[0].job_id, s._take(1)[0].job_id, s._take(2)[0].job_id, s._take(0)[0].job_id,
+            s._take(1)[0].job_id] == [1, 0, 4, 3, 2]
+    s = _Srv(1000)
+    s.now = 1.0
+    for j in (TransportJob(0, None, None, arrival_s=0.0, est_bytes=100, est_work=9.0),
+              TransportJob(1, None, None, arrival_s=0.1, est_bytes=100, est_work=1.0)):
+        s.submit(j)
+    assert [s._take(1)[0].job_id for _ in range(2)] == [0, 1]
+    # a large job that does not fit does not block smaller ones of the same arrival
+    s = _Srv(1000)
+    s.now = 1.0
+    s.used = 500
+    for j in (TransportJob(0, None, None, arrival_s=0.0, est_bytes=100, est_work=1.0),
+              TransportJob(1, None, None, arrival_s=0.0, est_bytes=900, est_work=9.0)):
+        s.submit(j)
+    assert s._take(0)[0].job_id == 0
+    # one slot: first in queue order among equal arrivals of equal estimates
+    s = _Srv(1000)
+    s.slots = 1
+    s.now = 1.0
+    for j in (TransportJob(0, None, None, arrival_s=0.0, est_bytes=100, est_work=2.0),
+              TransportJob(1, None, None, arrival_s=0.0, est_bytes=100, est_work=1.0)):
+        s.submit(j)
+    assert s._take(0)[0].job_id == 1
+    # the synthetic trace estimates work from the job shape
+    trace = synthetic_trace(8, rate_per_s=8.0, n_range=(1e3, 1e4), seed=3)
+    for j in trace:
+        f = 10.0 if j.meta["nonorthogonal"] else 1.0
+        assert j.est_work == j.meta["n"] * j.meta["M"] * j.meta["vectors"] * f
 
 
 def test_starving_job_blocks_new_admissions():
