@@ -194,6 +194,14 @@ int kmsd_metric_destroy(kmsd_metric *m);
 /* Reference: Metric.set_residual_policy(policy, interval, gap)
  * (cpp/native.cpp:206-210): adaptive 0 = "fixed", 1 = "adaptive". */
 int kmsd_metric_set_residual_policy(kmsd_metric *m, int adaptive, int interval, double gap);
+/* No counterpart in the reference (its solves are FP64 only); for callers
+ * running in mixed precision.  enabled = 1: complex128 right-hand sides are
+ * solved like complex64 ones, by FP32 inner solves inside the FP64
+ * iterative refinement, whose acceptance is the same FP64 true-residual test
+ * (max(rtol |q|, atol/safe) per column) as the FP64 PCG; the solutions then
+ * differ from the FP64 PCG's within that tolerance.  0 (the default): FP64
+ * PCG for complex128 blocks. */
+int kmsd_metric_set_mixed(kmsd_metric *m, int enabled);
 /* Reference: Metric.solve(B, rtol, atol, maxiter) -> (Y, report)
  * (cpp/native.cpp:313-452, 649-654).  Batched PCG, one independent system per
  * column, per-column RHS scaling, true-residual refresh every `interval`

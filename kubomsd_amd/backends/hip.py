@@ -51,6 +51,7 @@ SIGNATURES = {
     "kmsd_metric_create": [_P, _P, _P, _I64, _P, _P, _P],
     "kmsd_metric_destroy": [_P],
     "kmsd_metric_set_residual_policy": [_P, _I, _I, _D],
+    "kmsd_metric_set_mixed": [_P, _I],
     "kmsd_metric_solve": [_P, _P, _P, _I64, _I, _D, _D, _I, _P],
     "kmsd_metric_sample": [_P, _P, _P, _I64, _I, _P, _I64, _D, _D],
     "kmsd_metric_counters": [_P, _P, _P],
@@ -435,6 +436,11 @@ class HipMetric:
             raise KuboMSDError("invalid residual policy")
         self.ctx.check(self.ctx.lib.kmsd_metric_set_residual_policy(self.h, int(policy == "adaptive"),
                                                                    int(interval), float(gap)))
+
+    def set_mixed(self, enabled):
+        """complex128 solves by FP32 inner solves inside the FP64 refinement
+        (the complex64 path), accepted on the same FP64 true residual."""
+        self.ctx.check(self.ctx.lib.kmsd_metric_set_mixed(self.h, int(bool(enabled))))
 
     def solve(self, B, rtol, atol, maxiter):
         ctx = self.ctx

@@ -335,6 +335,9 @@ struct kmsd_metric {
     // narrow enough, 2 not usable.
     int cheb_state = 0;
     double cheb_lo = 0, cheb_hi = 0;
+    // complex128 right-hand sides take the complex64 path's FP64 refinement
+    // (FP32 inner solves) instead of FP64 PCG (kmsd_metric_set_mixed)
+    bool mixed_c128 = false;
     kmsd_matrix *W = nullptr;  // Z S Z for the sampler (built on first use)
     kmsd::DBuf sb1, sb2;       // sampler Clenshaw buffers
     kmsd::DBuf pinner;         // partial sums of kmsd_metric_inner

@@ -709,8 +709,8 @@ void launch_convert(kmsd_ctx *c, int din, const void *X, int64_t ldx, int dout, 
 }
 
 bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w) {
-    int32_t zero = 0, got = 0;
-    copy_h2d(c, c->flag, &zero, sizeof(zero));
+    int32_t got = 0;
+    HIPCHECK(hipMemsetAsync(c->flag, 0, sizeof(int32_t), c->stream));
     const int64_t g = blocks_for(n * w);
     if (dtype == KMSD_C128)
         hipLaunchKernelGGL(nonfinite_kernel<double>, dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx, n, w,

@@ -100,6 +100,11 @@ extern "C" int kmsd_metric_set_residual_policy(kmsd_metric *m, int adaptive, int
     });
 }
 
+extern "C" int kmsd_metric_set_mixed(kmsd_metric *m, int enabled) {
+    if (!m) return KMSD_EINVAL;
+    return guard(m->ctx, [&] { m->mixed_c128 = enabled != 0; });
+}
+
 static void copy_report(const kmsd_metric *m, kmsd_report *r) {
     if (!r) return;
     if (r->converged) std::copy(m->rep_conv.begin(), m->rep_conv.end(), r->converged);

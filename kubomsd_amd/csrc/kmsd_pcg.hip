@@ -662,7 +662,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         direct_solve(m, dtype, B, ldb, Y, ldy, nb, rtol, atol);
         return;
     }
-    if (dtype == KMSD_C128) {
+    if (dtype == KMSD_C128 && !m->mixed_c128) {
         Solver S = make_solver(m, dtype, nb);
         const Outcome o = pcg_run(S, B, ldb, Y, ldy, rtol, atol, maxiter);
         store_report(m, o.cs);
@@ -680,7 +680,8 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
                               : "S solve failed true-residual acceptance");
         return;
     }
-    // complex64: FP32 inner PCG inside FP64 iterative refinement.
+    // complex64 (or complex128 with mixed_c128): FP32 inner solves inside
+    // FP64 iterative refinement.
     if (!m->work) m->work = new kmsd_metric::Work();
     kmsd_metric::Work &wk = *m->work;
     const int64_t n = m->n;
@@ -692,7 +693,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     void *d64 = wk.d64.ensure(c, b64);
     ColState *cs2 = static_cast<ColState *>(wk.cs2.ensure(c, sizeof(ColState) * size_t(nb)));
     double *nrm = static_cast<double *>(wk.part2.ensure(c, size_t(std::max<int64_t>(c->groups, 8)) * nb * 16));
-    launch_convert(c, KMSD_C64, B, ldb, KMSD_C128, bb, nb, n, nb);
+    launch_convert(c, dtype, B, ldb, KMSD_C128, bb, nb, n, nb);
     // y = 0 and r = b are implicit in the first round: it reads b as its
     // residual and assigns y (zeroed here only if no round runs).
     // ||b|| per column (FP64) -> threshold
@@ -792,9 +793,9 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     }
     ++m->calls;
     if (!rounds) launch_scale(c, KMSD_C128, bb, nb, yy, nb, n, nb, Cplx{0, 0});
-    launch_convert(c, KMSD_C128, yy, nb, KMSD_C64, Y, ldy, n, nb);
+    launch_convert(c, KMSD_C128, yy, nb, dtype, Y, ldy, n, nb);
     // finiteness of the stored solution per column
-    bool fin_all = !launch_nonfinite(c, KMSD_C64, Y, ldy, n, nb);
+    bool fin_all = !launch_nonfinite(c, dtype, Y, ldy, n, nb);
     for (auto &s : agg) s.fin = s.fin && fin_all;
     std::vector<double> qn = bn;
     store_report(m, agg, &rn, &qn, &thr);

@@ -113,6 +113,10 @@ class DeviceMetric:
             self.device.set_ic0(d.factor, d.permutation, d.scaling)
         self.device.set_residual_policy(config.metric_residual_policy, config.metric_check_interval,
                                         config.metric_gap_ratio)
+        # mixed precision: the complex128 solves (the time propagation's
+        # state) also take the FP64 refinement with FP32 inner solves
+        if config.precision == "mixed" and not self.direct:
+            self.device.set_mixed(True)
         ctx.set_tuning("pcg_poll", max(1, int(config.metric_gpu_check_batch)) if
                        config.metric_gpu_check_batch > 1 else 4)
         # The device skips solves only for S = I with the diagonal preconditioner.

@@ -103,3 +103,34 @@ def test_flake_moments_with_cheb_inner_match_oracle(hip_ctx):
     got = host(he.moments(to_dev(hip_ctx, B, "c64"), 24, 0.0, 12.0))
     want = oe.moments(B, 24, 0.0, 12.0)
     assert rel_err(got, want) < 2e-4
+
+
+@pytest.mark.parametrize("eligible", [True, False])
+def test_mixed_complex128_solves_take_the_refinement(hip_ctx, eligible):
+    """kmsd_metric_set_mixed: complex128 right-hand sides solved by FP32
+    inner solves (Chebyshev where S allows it, else PCG) inside the FP64
+    refinement reach the same FP64 residual as the FP64 PCG, and the two
+    solutions agree to the solve tolerance."""
+    from kubomsd_amd.backends.hip import HipMetric
+    if eligible:
+        _, S = flake_S(n=500, seed=17)
+    else:
+        n = 300
+        A = sp.random(n, n, density=0.03, random_state=9, format="csr")
+        S = (A @ A.T + 0.5 * sp.identity(n)).astype(np.complex128).tocsr()
+        S.sort_indices()
+    B = random_block(S.shape[0], 8, 23)
+    out = {}
+    for mixed in (True, False):
+        met = HipMetric(hip_ctx, S, S.diagonal().real)
+        met.set_mixed(mixed)
+        Y, rep = met.solve(to_dev(hip_ctx, B, "c128"), 1e-10, 0.0, 1000)
+        Y = host(Y)
+        assert Y.dtype == np.complex128
+        assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+        res = np.linalg.norm(B - S @ Y, axis=0) / np.linalg.norm(B, axis=0)
+        assert res.max() <= 2e-10  # the device's FP64 residual, recomputed on the host
+        out[mixed] = (Y, rep)
+    assert rel_err(out[True][0], out[False][0]) < 1e-9
+    if eligible:
+        assert len(set(out[True][1]["iterations"])) == 1
