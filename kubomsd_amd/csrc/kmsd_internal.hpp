@@ -224,6 +224,10 @@ struct Step {
     int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
     int64_t max_groups = 0;     // if nonzero (a multiple of 8) and `groups` is not set: at most this many
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
+    // 4x4-block products on the matrix cores: block-Jacobi solve of each block
+    // row's result with these FP32 factors before the epilogue (F_BETA and
+    // F_ACC allowed); the launch fails if the matrix-core kernel cannot run it
+    const float *bfac32 = nullptr;
     // Fused second product (real width-3/4/6 ELL A with FP64 values): VOUT =
     // valpha * V X, V's real values in A's ELL slots.
     const double *vval = nullptr;
@@ -335,6 +339,11 @@ struct kmsd_metric {
     // narrow enough, 2 not usable.
     int cheb_state = 0;
     double cheb_lo = 0, cheb_hi = 0;
+    // Block-Jacobi Chebyshev inner solves (uniform 4x4 blocks, 4x4-block S):
+    // interval [bcheb_lo, bcheb_hi] from the Ritz values of the first PCG
+    // inner solve, with margins; bcheb_state 0 not yet measured, 1 usable, 2 not
+    int bcheb_state = 0;
+    double bcheb_lo = 0, bcheb_hi = 0;
     // complex128 right-hand sides take the complex64 path's FP64 refinement
     // (FP32 inner solves) instead of FP64 PCG (kmsd_metric_set_mixed)
     bool mixed_c128 = false;

@@ -98,6 +98,10 @@ struct StepArgs {
     // bsr4_mfma_kernel: byte extents of X, the block values and the block
     // columns (buffer descriptors; reads past them return zero)
     uint32_t xbytes, vbytes, ibytes;
+    // bsr4_mfma_kernel<true>: FP32 block factors (16 floats per block row, L
+    // below the diagonal, 1/L_ii on it); the product of each block row is
+    // replaced by (L L^H)^-1 times it before the epilogue
+    const float *bfac32;
 };
 
 // Block row at walk position pos of the plane-tiled walk: the nq block rows
@@ -729,13 +733,60 @@ __global__ __launch_bounds__(256) void bsr4_step_kernel(const StepArgs a) {
 // 16 bytes per lane) and 3 x 256 B of indices.
 // The same XCD-aware, optionally plane-tiled, block row walk as
 // bsr4_step_kernel; partials [gridDim.x][w] in the same format.
+// z = (L L^H)^-1 v for one 4x4 block with a real factor in FP32: L below the
+// diagonal, 1/L_ii on it (products instead of divisions), FP32 arithmetic
+// (complex64 blocks), in place on the real and imaginary parts.
+__device__ __forceinline__ void blk4_solve_f32(const float *L, float (&re)[4], float (&im)[4]) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 l0 = *reinterpret_cast<const f4 *>(L), l1 = *reinterpret_cast<const f4 *>(L + 4),
+             l2 = *reinterpret_cast<const f4 *>(L + 8), l3 = *reinterpret_cast<const f4 *>(L + 12);
+    const float m[16] = {l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3],
+                         l2[0], l2[1], l2[2], l2[3], l3[0], l3[1], l3[2], l3[3]};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float sr = re[i], si = im[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            sr = __builtin_fmaf(-m[i * 4 + k], re[k], sr);
+            si = __builtin_fmaf(-m[i * 4 + k], im[k], si);
+        }
+        re[i] = sr * m[i * 5];
+        im[i] = si * m[i * 5];
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        float sr = re[i], si = im[i];
+#pragma unroll
+        for (int k = i + 1; k < 4; ++k) {
+            sr = __builtin_fmaf(-m[k * 4 + i], re[k], sr);
+            si = __builtin_fmaf(-m[k * 4 + i], im[k], si);
+        }
+        re[i] = sr * m[i * 5];
+        im[i] = si * m[i * 5];
+    }
+}
+
+// The same on complex values held in FP64 (rounded to FP32 first).
+__device__ __forceinline__ void blk4_solve_r32(const float *L, cx<double> (&v)[4]) {
+    float re[4], im[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        re[i] = float(v[i].re);
+        im[i] = float(v[i].im);
+    }
+    blk4_solve_f32(L, re, im);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = {double(re[i]), double(im[i])};
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, int(bytes), 0x00020000);
 }
 
-static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a) {
+template <bool BLK>
+__device__ __forceinline__ void bsr4_mfma_body(const StepArgs &a) {
     using V = cx<float>;
     constexpr int CPL = 4, NV = 10, NI = 3;
     using PK = Pack<float, CPL>;
@@ -829,12 +880,34 @@ static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a)
         }
         if (!valid || !colok) continue;
         const int64_t row0 = q * 4;
+        if constexpr (BLK) {
+            // block-Jacobi: each column's four rows of this block row times
+            // (L L^H)^-1 of the block row's factor
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                float re[4], im[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    re[r] = d[2 * c][r];
+                    im[r] = d[2 * c + 1][r];
+                }
+                blk4_solve_f32(a.bfac32 + q * 16, re, im);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    d[2 * c][r] = re[r];
+                    d[2 * c + 1][r] = im[r];
+                }
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             PK xi, pv, wv;
             cx<double> y[CPL];
 #pragma unroll
             for (int c = 0; c < CPL; ++c) y[c] = {double(d[2 * c][r]), double(d[2 * c + 1][r])};
+            if constexpr (BLK) {
+                if (fl & F_BETA) xi.load(static_cast<const V *>(a.X) + (row0 + r) * a.ldx + col);
+            }
             if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
             if (fl & F_MOM) wv.load_nt(W + (row0 + r) * a.ldw + col);
             step_epilogue<float, double, CPL>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
@@ -867,6 +940,17 @@ static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a)
             reinterpret_cast<cx<double> *>(a.part)[int64_t(blockIdx.x) * a.w + col + c] = s;
         }
     }
+}
+
+static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a) { bsr4_mfma_body<false>(a); }
+
+// The same product with a 4x4 block-Jacobi solve of each block row's result
+// before the epilogue (a.bfac32): one term of a preconditioned Chebyshev
+// series in one launch.
+// (three wavefronts per SIMD, as the plain product, for its gather latency)
+static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bsr4_mfma_blk_kernel(
+    const StepArgs a) {
+    bsr4_mfma_body<true>(a);
 }
 
 // FP64-valued 4x4-block product (real or imaginary-only values) with the

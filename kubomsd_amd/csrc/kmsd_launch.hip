@@ -295,6 +295,7 @@ struct KernelKey {
     int lpr_log2 = 6;
     bool vout = false;   // the ELL kernel also forms V X (FP64 values and row sums, real A)
     bool mfma = false;   // bsr4_mfma_kernel (FP32-valued 4x4 blocks on the matrix cores)
+    bool blk = false;    // bsr4_mfma_blk_kernel (the same with a block-Jacobi solve of each block row)
     bool lds = false;    // bsr4_lds_kernel (FP64-valued 4x4 blocks, LDS-staged)
 };
 
@@ -448,7 +449,8 @@ namespace {
 // offsets are 32-bit); otherwise the launch keeps the VALU kernel.
 void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, StepArgs &a, Shape &sh, KernelFn &fn,
                 KernelKey &key) {
-    if (fl & ~(F_SPMM | F_STORE | F_MOM | F_P)) return;
+    const uint32_t allowed = F_SPMM | F_STORE | F_MOM | F_P | (s.bfac32 ? (F_BETA | F_ACC) : 0u);
+    if (fl & ~allowed) return;
     if (s.w % 4 || cpl_limit(s, KMSD_C64) < 4) return;
     const int64_t lim = (int64_t(1) << 31) - (int64_t(1) << 16);
     const int64_t nnzb = A->noff / 16;
@@ -456,8 +458,10 @@ void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, S
     if (xbytes > lim || nnzb * 64 > lim) return;
     const int lg = std::max(2, std::min(4, ilog2_ceil((s.w + 3) / 4)));
     const int64_t lpr = int64_t(1) << lg;
-    fn = bsr4_mfma_kernel;
+    fn = s.bfac32 ? bsr4_mfma_blk_kernel : bsr4_mfma_kernel;
     key.mfma = true;
+    key.blk = s.bfac32 != nullptr;
+    a.bfac32 = s.bfac32;
     sh.cpl = 4;
     sh.lpr_log2 = lg;
     sh.gy = int((s.w + 4 * lpr - 1) / (4 * lpr));
@@ -562,7 +566,9 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.W) fl |= F_MOM;
     if (s.doubling) fl |= F_MOM2;
     if (s.ACC) fl |= F_ACC;
-    if (bsr && f32 && A->vkind != VK_COMPLEX && c->bsr_mfma) mfma_shape(c, s, A, fl, a, sh, fn, key);
+    if (bsr && f32 && A->vkind != VK_COMPLEX && (c->bsr_mfma || s.bfac32)) mfma_shape(c, s, A, fl, a, sh, fn, key);
+    if (s.bfac32 && !key.blk)
+        throw Error(KMSD_EDEVICE, "internal error: a block-Jacobi product needs the matrix-core 4x4-block kernel");
     if (bsr && !f32 && A->vkind != VK_COMPLEX && c->bsr_lds) lds_shape(c, dtype, s, A, fl, a, sh, fn, key);
     if (!s.groups && s.max_groups) {
         if (s.max_groups < 8 || s.max_groups % 8) throw Error(KMSD_EDEVICE, "internal error: workgroup cap");

@@ -20,7 +20,7 @@ using namespace kmsd;
 
 struct kmsd_metric::Work {
     DBuf q, y, r, z, p, sp, t, cs, fl, part, part2;
-    DBuf b128, y128, r128, r64, d64, cs2, sums;
+    DBuf b128, y128, r128, r64, d64, cs2, sums, hist;
 };
 
 namespace kmsd {
@@ -70,6 +70,8 @@ struct Solver {
     double *part2;  // step-kernel partials
     int64_t g2 = 0;
     int32_t slot = 0;  // flag slot of the current iteration (k & 1): gates and the liveness reads
+    double *hist = nullptr;  // CG coefficient history [hist_len][w] (alpha, beta), or null
+    int32_t hist_len = 0;
 
     void row(int stage, const void *B = nullptr, int64_t ldb = 0, void *OUT = nullptr, int64_t ldo = 0,
              bool require_active = false, const int32_t *gate = nullptr) {
@@ -159,6 +161,8 @@ struct Solver {
         a.refresh = refresh;
         a.slot = slot;
         a.gate = gate;
+        a.hist = hist;
+        a.hist_len = hist_len;
         hipLaunchKernelGGL(pcg_col_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, a);
         HIPCHECK(hipGetLastError());
     }
@@ -653,6 +657,164 @@ int cheb_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction) {
     return K;
 }
 
+// Whether the complex64 refinement may take block-Jacobi Chebyshev inner
+// solves: uniform 4x4 atom blocks with real FP32 factors, a real-valued
+// 4x4-block S, widths and byte extents the matrix-core kernel accepts, and an
+// interval not yet found unusable.
+bool bcheb_candidate(kmsd_metric *m, int32_t nb) {
+    kmsd_ctx *c = m->ctx;
+    const kmsd_matrix *S = m->S;
+    if (!c->cheb_inner || m->ic || m->direct || m->identity || !m->uniform4 || !m->factors32 || m->nlarge)
+        return false;
+    if (!S || S->fmt != KMSD_BSR4 || S->vkind != VK_REAL || !S->val32 || nb % 4) return false;
+    const int64_t lim = (int64_t(1) << 31) - (int64_t(1) << 16);
+    if (S->n * int64_t(nb) * 8 > lim || (S->noff / 16) * 64 > lim) return false;
+    return m->bcheb_state != 2;
+}
+
+// Number of eigenvalues below x of the symmetric tridiagonal matrix with
+// diagonal a and squared off-diagonals b2 (Sturm count of the LDL^T pivots).
+int tri_count_below(const std::vector<double> &a, const std::vector<double> &b2, double x) {
+    int cnt = 0;
+    double q = a[0] - x;
+    if (q < 0) ++cnt;
+    for (size_t i = 1; i < a.size(); ++i) {
+        if (q == 0) q = 1e-300;
+        q = a[i] - x - b2[i - 1] / q;
+        if (q < 0) ++cnt;
+    }
+    return cnt;
+}
+
+// Smallest and largest eigenvalues of that matrix by bisection.
+void tri_extremes(const std::vector<double> &a, const std::vector<double> &b2, double &lo, double &hi) {
+    const size_t m = a.size();
+    double glo = a[0], ghi = a[0];
+    for (size_t i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::sqrt(b2[i - 1]) : 0.0) + (i + 1 < m ? std::sqrt(b2[i]) : 0.0);
+        glo = std::min(glo, a[i] - r);
+        ghi = std::max(ghi, a[i] + r);
+    }
+    double l = glo, u = ghi;  // smallest: count_below(l) = 0 <= ... < 1 = count_below(u)
+    for (int it = 0; it < 100; ++it) {
+        const double x = 0.5 * (l + u);
+        (tri_count_below(a, b2, x) >= 1 ? u : l) = x;
+    }
+    lo = l;
+    l = glo;
+    u = ghi;  // largest: count_below(l) < m <= count_below(u)
+    for (int it = 0; it < 100; ++it) {
+        const double x = 0.5 * (l + u);
+        (tri_count_below(a, b2, x) >= int(m) ? u : l) = x;
+    }
+    hi = u;
+}
+
+// Interval for the block-Jacobi series from the Ritz values of a PCG inner
+// solve: the CG coefficients (alpha_k, beta_k) of each column give the
+// Lanczos tridiagonal of the preconditioned operator, T_kk = 1/alpha_k +
+// beta_(k-1)/alpha_(k-1), T_k,k+1 = sqrt(beta_k)/alpha_k; the extreme Ritz
+// values over the columns with at least 6 iterations, widened by 15 % below
+// and 3 % above (Ritz values lie inside the spectrum; a series whose interval
+// misses part of it still converges for every positive eigenvalue below
+// twice its center, only more slowly, and the refinement's FP64 residual
+// decides acceptance).
+void bcheb_measure(kmsd_metric *m, const double *hist, int32_t nb, int32_t hist_len,
+                   const std::vector<ColState> &cs) {
+    kmsd_ctx *c = m->ctx;
+    m->bcheb_state = 2;
+    std::vector<double> h(size_t(hist_len) * nb * 2);
+    read_status(c, {{h.data(), hist, h.size() * sizeof(double)}});
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    int used = 0;
+    for (int32_t j = 0; j < nb; ++j) {
+        const int64_t len = std::min<int64_t>(cs[size_t(j)].iters, hist_len);
+        if (len < 6) continue;
+        std::vector<double> a(static_cast<size_t>(len)), b2(static_cast<size_t>(len - 1));
+        bool ok = true;
+        for (int64_t k = 0; k < len; ++k) {
+            const double al = h[(size_t(k) * nb + j) * 2];
+            const double be_prev = k > 0 ? h[(size_t(k - 1) * nb + j) * 2 + 1] : 0.0;
+            const double al_prev = k > 0 ? h[(size_t(k - 1) * nb + j) * 2] : 1.0;
+            if (!(al > 0) || !(be_prev >= 0)) ok = false;
+            a[size_t(k)] = 1.0 / al + (k > 0 ? be_prev / al_prev : 0.0);
+            if (k + 1 < len) {
+                const double be = h[(size_t(k) * nb + j) * 2 + 1];
+                if (!(be >= 0)) ok = false;
+                b2[size_t(k)] = be / (al * al);
+            }
+        }
+        if (!ok) continue;
+        double l, u;
+        tri_extremes(a, b2, l, u);
+        lo = std::min(lo, l);
+        hi = std::max(hi, u);
+        ++used;
+    }
+    if (!used || !std::isfinite(lo) || !std::isfinite(hi) || !(lo > 0)) return;
+    lo *= 0.85;
+    hi *= 1.03;
+    if (hi > 20.0 * lo) return;
+    m->bcheb_lo = lo;
+    m->bcheb_hi = hi;
+    m->bcheb_state = 1;
+}
+
+// Block-Jacobi Chebyshev inner solve: d = sum_k g_k T_k((M^-1 S - c)/h) M^-1 r
+// on [bcheb_lo, bcheb_hi] (coefficients and term count as in cheb_run), each
+// term one matrix-core product with the 4x4 block solve of its result, the
+// recurrence and the accumulation in the epilogue (bsr4_mfma_blk_kernel).
+// r is overwritten.  Returns the number of products.
+int cheb_block_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction) {
+    kmsd_ctx *c = m->ctx;
+    const int64_t n = m->n;
+    const double lo = m->bcheb_lo, hi = m->bcheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
+    const double z = cc / h, sq = std::sqrt(z * z - 1.0), rho = z - sq, amp = 2.0 / (h * sq);
+    int K = 1;
+    while (K < 400 && hi * amp * std::pow(rho, double(K + 1)) / (1.0 - rho) > reduction) ++K;
+    void *t0 = m->work->z.ensure(c, size_t(n) * size_t(nb) * 8);
+    const int64_t nq = n / 4;
+    hipLaunchKernelGGL(block_apply32_kernel, dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((nq * nb + 255) / 256, 16384)))),
+                       dim3(256), 0, c->stream, r, t0, m->factors32, nq, nb);
+    HIPCHECK(hipGetLastError());
+    launch_scale(c, KMSD_C64, t0, nb, d, nb, n, nb, Cplx{0.5 * amp, 0.0});
+    double g = -amp * rho;
+    Step s;
+    s.A = m->S;
+    s.bfac32 = m->factors32;
+    s.X = t0;
+    s.ldx = nb;
+    s.alpha = Cplx{1.0 / h, 0.0};
+    s.use_beta = true;
+    s.beta = Cplx{-cc / h, 0.0};
+    s.OUT = r;
+    s.ldo = nb;
+    s.ACC = d;
+    s.lda = nb;
+    s.coef = Cplx{g, 0.0};
+    s.n = n;
+    s.w = nb;
+    s.f32_matrix = true;
+    launch_step(c, KMSD_C64, s);
+    void *prev = t0, *cur = r;
+    for (int k = 1; k < K; ++k) {
+        g *= -rho;
+        Step u = s;
+        u.X = cur;
+        u.alpha = Cplx{2.0 / h, 0.0};
+        u.beta = Cplx{-2.0 * cc / h, 0.0};
+        u.P = prev;
+        u.ldp = nb;
+        u.use_p = true;
+        u.gamma = -1.0;
+        u.OUT = prev;  // T_(k+1) replaces T_(k-1) row by row
+        u.coef = Cplx{g, 0.0};
+        launch_step(c, KMSD_C64, u);
+        std::swap(prev, cur);
+    }
+    return K;
+}
+
 }  // namespace
 
 void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, int64_t ldy, int32_t nb, double rtol,
@@ -755,9 +917,23 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
                 x.fin = 1;
             }
             o.fl = PcgFlags{};
+        } else if (bcheb_candidate(m, nb) && m->bcheb_state == 1) {
+            const int K = cheb_block_run(m, r64, d64, nb, inner_rtol);
+            o.cs.assign(size_t(nb), ColState{});
+            for (ColState &x : o.cs) {
+                x.iters = K;
+                x.fin = 1;
+            }
+            o.fl = PcgFlags{};
         } else {
             Solver S = make_solver(m, KMSD_C64, nb);
+            const bool record = bcheb_candidate(m, nb) && m->bcheb_state == 0;
+            if (record) {
+                S.hist_len = 64;
+                S.hist = static_cast<double *>(wk.hist.ensure(c, sizeof(double) * 2 * 64 * size_t(nb)));
+            }
             o = pcg_run(S, r64, nb, d64, nb, inner_rtol, 0.0, maxiter, false);
+            if (record) bcheb_measure(m, S.hist, nb, S.hist_len, o.cs);
         }
         for (int32_t j = 0; j < nb; ++j) {
             agg[j].iters += o.cs[j].iters;

@@ -431,6 +431,11 @@ struct ColArgs {
     int32_t w, stage, adaptive, refresh;
     int32_t slot;         // iteration parity k & 1 (flag slots of PcgFlags)
     const int32_t *gate;  // if set and *gate == 0 the launch does nothing
+    // if set: alpha (re) and beta (im) of each iteration, [hist_len][w]
+    // complex, entry (k, j) for iteration k of column j (the CG coefficients
+    // that give the Lanczos tridiagonal of the preconditioned operator)
+    double *hist;
+    int32_t hist_len;
 };
 
 static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
@@ -515,6 +520,7 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
                 atomicOr(&f->broke, 1);
             }
             s.alpha = s.active ? s.rz / tre : 0.0;
+            if (a.hist && s.active && s.iters < a.hist_len) a.hist[(s.iters * a.w + j) * 2] = s.alpha;
             break;
         }
         case PC_RN: {
@@ -545,6 +551,8 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         }
         case PC_BETA: {
             s.beta = s.active ? tre / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
+            if (a.hist && s.active && s.iters >= 1 && s.iters <= a.hist_len)
+                a.hist[((s.iters - 1) * a.w + j) * 2 + 1] = s.beta;
             s.rz = tre;
             if (s.active) atomicAdd(&f->any_active[a.slot ^ 1], 1);
             break;
@@ -559,6 +567,8 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
             if (s.active) atomicAdd(&f->updated[a.slot], 1);
             s.active = s.active && s.rn > s.thr;
             s.beta = s.active ? tim / (s.rz != 0.0 ? s.rz : 1.0) : 0.0;
+            if (a.hist && s.active && s.iters >= 1 && s.iters <= a.hist_len)
+                a.hist[((s.iters - 1) * a.w + j) * 2 + 1] = s.beta;
             s.rz = tim;
             if (s.active) atomicAdd(&f->any_active[a.slot ^ 1], 1);
             break;
@@ -569,6 +579,27 @@ static __global__ __launch_bounds__(256) void pcg_col_kernel(const ColArgs a) {
         }
         case PC_CLEAR:
             break;
+    }
+}
+
+// Z = M^-1 R for uniform 4x4 block-Jacobi factors in FP32 (complex64
+// blocks): one thread per (block row, column).
+static __global__ __launch_bounds__(256) void block_apply32_kernel(const void *Rv, void *Zv, const float *fac32,
+                                                                   int64_t nq, int32_t w) {
+    const cx<float> *R = static_cast<const cx<float> *>(Rv);
+    cx<float> *Z = static_cast<cx<float> *>(Zv);
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nq * w; e += int64_t(gridDim.x) * 256) {
+        const int64_t q = e / w, j = e - q * w;
+        float re[4], im[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const cx<float> v = R[(q * 4 + r) * w + j];
+            re[r] = v.re;
+            im[r] = v.im;
+        }
+        blk4_solve_f32(fac32 + q * 16, re, im);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Z[(q * 4 + r) * w + j] = cx<float>{re[r], im[r]};
     }
 }
 
@@ -698,46 +729,6 @@ __device__ __forceinline__ void blk4_solve(const cx<double> *L, cx<double> (&v)[
         const double den = d.re * d.re + d.im * d.im;
         v[i] = {(s.re * d.re - s.im * d.im) / den, (s.im * d.re + s.re * d.im) / den};
     }
-}
-
-// The same solve with a real factor in FP32: L below the diagonal, 1/L_ii on
-// it (products instead of divisions), FP32 arithmetic (complex64 blocks).
-__device__ __forceinline__ void blk4_solve_r32(const float *L, cx<double> (&v)[4]) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4 l0 = *reinterpret_cast<const f4 *>(L), l1 = *reinterpret_cast<const f4 *>(L + 4),
-             l2 = *reinterpret_cast<const f4 *>(L + 8), l3 = *reinterpret_cast<const f4 *>(L + 12);
-    const float m[16] = {l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3],
-                         l2[0], l2[1], l2[2], l2[3], l3[0], l3[1], l3[2], l3[3]};
-    float re[4], im[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        re[i] = float(v[i].re);
-        im[i] = float(v[i].im);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float sr = re[i], si = im[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) {
-            sr = __builtin_fmaf(-m[i * 4 + k], re[k], sr);
-            si = __builtin_fmaf(-m[i * 4 + k], im[k], si);
-        }
-        re[i] = sr * m[i * 5];
-        im[i] = si * m[i * 5];
-    }
-#pragma unroll
-    for (int i = 3; i >= 0; --i) {
-        float sr = re[i], si = im[i];
-#pragma unroll
-        for (int k = i + 1; k < 4; ++k) {
-            sr = __builtin_fmaf(-m[k * 4 + i], re[k], sr);
-            si = __builtin_fmaf(-m[k * 4 + i], im[k], si);
-        }
-        re[i] = sr * m[i * 5];
-        im[i] = si * m[i * 5];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = {double(re[i]), double(im[i])};
 }
 
 // Fused iteration stages PS_UPD_PRE / PS_PUPD_PRE for a uniform 4x4

@@ -90,6 +90,7 @@ def test_mfma_metric_solve_reaches_fp64_residual(hip_ctx):
     blocks = [(a, b, la.cholesky(m.S[a:b, a:b].toarray(), lower=True)) for a, b in m.atom_slices]
     B = random_block(S.shape[0], 16, 5)
     out = {}
+    hip_ctx.set_tuning("cheb_inner", 0)  # PCG inner solves throughout (their S products are compared)
     try:
         for flag in (1, 0):
             hip_ctx.set_tuning("bsr_mfma", flag)
@@ -101,6 +102,7 @@ def test_mfma_metric_solve_reaches_fp64_residual(hip_ctx):
             out[flag] = (Yh, met.diagnostics()["max_iterations_sum"])
     finally:
         hip_ctx.set_tuning("bsr_mfma", 1)
+        hip_ctx.set_tuning("cheb_inner", 1)
     assert rel_err(out[1][0], out[0][0]) < 1e-6
     assert not np.array_equal(out[1][0], out[0][0])
     assert abs(out[1][1] - out[0][1]) <= 2

@@ -134,3 +134,42 @@ def test_mixed_complex128_solves_take_the_refinement(hip_ctx, eligible):
     assert rel_err(out[True][0], out[False][0]) < 1e-9
     if eligible:
         assert len(set(out[True][1]["iterations"])) == 1
+
+
+def test_block_jacobi_series_after_ritz_bounds(hip_ctx):
+    """4x4-block S with atom-block Jacobi (the simple-cubic orbital model):
+    the first complex64 inner solve runs PCG and records its CG coefficients;
+    the Ritz values of that Lanczos tridiagonal, widened, give the interval of
+    the block-Jacobi Chebyshev series that later inner solves use
+    (one matrix-core product with the block solve, recurrence and
+    accumulation per term).  Every solve reaches the FP64 residual and
+    SciPy's solution; the series solves run the same terms on every column;
+    with cheb_inner = 0 all solves stay PCG and agree to the rounding."""
+    import scipy.linalg as la
+    from kubomsd_amd.backends.hip import HipMetric
+    from kubomsd_amd.synthetic import cubic_orbital_model
+    model = cubic_orbital_model(6, seed=4411)
+    S = sp.csr_matrix(model.S)
+    blocks = [(a, b, la.cholesky(model.S[a:b, a:b].toarray(), lower=True)) for a, b in model.atom_slices]
+    Bs = [random_block(model.n, 16, 40 + i) for i in range(3)]
+    outs = {}
+    for flag in (1, 0):
+        hip_ctx.set_tuning("cheb_inner", flag)
+        try:
+            met = HipMetric(hip_ctx, S, S.diagonal().real, blocks)
+            outs[flag] = [met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 1000) for B in Bs]
+        finally:
+            hip_ctx.set_tuning("cheb_inner", 1)
+    for flag in (1, 0):
+        for B, (Y, rep) in zip(Bs, outs[flag]):
+            Y = host(Y)
+            Bq = B.astype(np.complex64).astype(np.complex128)
+            assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+            assert rel_err(Y, spla.spsolve(S.tocsc(), Bq)) < 2e-7
+    later = outs[1][2][1]
+    assert len(set(later["iterations"])) == 1 and later["iterations"][0] > 0
+    # the first solve's first refinement round is PCG in both settings (its
+    # coefficients give the bounds); later rounds differ by the inner solver
+    assert not np.array_equal(host(outs[1][2][0]), host(outs[0][2][0]))
+    for (Y1, _), (Y0, _) in zip(outs[1], outs[0]):
+        assert rel_err(host(Y1), host(Y0)) < 2e-7
