@@ -125,8 +125,10 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * with the diagonal or no preconditioner and has a Gershgorin interval
  * [lo, hi] with lo > 0 and hi <= 4 lo take Chebyshev-series inner solves
  * inside the FP64 refinement, one fused product per term and no
- * reductions; the acceptance stays the FP64 true residual; 0: PCG inner
- * solves). */
+ * reductions; uniform 4x4 block-Jacobi on a 4x4-block S: the first inner
+ * solve is PCG, whose CG coefficients give Ritz bounds for block-Jacobi
+ * series in the later ones; the acceptance stays the FP64 true residual;
+ * 0: PCG inner solves). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).
  * Times the fused Chebyshev step of a moment chain with product A on an
