@@ -197,6 +197,9 @@ extern "C" int kmsd_ctx_set_tuning(kmsd_ctx *c, const char *key, int64_t value) 
             c->pcg_poll = int(value);
         } else if (k == "cheb_inner") {
             c->cheb_inner = value != 0;
+        } else if (k == "bcheb_lo_pct") {
+            if (value < 1 || value > 10000) throw Error(KMSD_EINVAL, "bcheb_lo_pct must be in [1, 10000]");
+            c->bcheb_lo_pct = int(value);
         } else if (k == "min_blocks" || k == "pcg_min_blocks") {
             if (value < 1 || value > 64) throw Error(KMSD_EINVAL, "min_blocks must be in [1, 64]");
             (k == "min_blocks" ? c->min_blocks : c->pcg_min_blocks) = int(value);

@@ -714,11 +714,14 @@ void tri_extremes(const std::vector<double> &a, const std::vector<double> &b2, d
 // solve: the CG coefficients (alpha_k, beta_k) of each column give the
 // Lanczos tridiagonal of the preconditioned operator, T_kk = 1/alpha_k +
 // beta_(k-1)/alpha_(k-1), T_k,k+1 = sqrt(beta_k)/alpha_k; the extreme Ritz
-// values over the columns with at least 6 iterations, widened by 15 % below
-// and 3 % above (Ritz values lie inside the spectrum; a series whose interval
-// misses part of it still converges for every positive eigenvalue below
-// twice its center, only more slowly, and the refinement's FP64 residual
-// decides acceptance).
+// values over the columns with at least 6 iterations; the lower end is
+// bcheb_lo_pct percent of the smallest (92 by default), the upper end 2 %
+// above the largest.  Ritz values lie inside the spectrum; a series whose
+// interval misses part of it still reduces every eigencomponent in
+// (0, lo + hi), only more slowly below lo, the refinement's FP64 residual
+// decides acceptance, and a refinement round that reduces the residual by
+// less than the square root of what the interval promised makes the metric
+// return to PCG inner solves.
 void bcheb_measure(kmsd_metric *m, const double *hist, int32_t nb, int32_t hist_len,
                    const std::vector<ColState> &cs) {
     kmsd_ctx *c = m->ctx;
@@ -752,9 +755,9 @@ void bcheb_measure(kmsd_metric *m, const double *hist, int32_t nb, int32_t hist_
         ++used;
     }
     if (!used || !std::isfinite(lo) || !std::isfinite(hi) || !(lo > 0)) return;
-    lo *= 0.85;
-    hi *= 1.03;
-    if (hi > 20.0 * lo) return;
+    lo *= 0.01 * c->bcheb_lo_pct;
+    hi *= 1.02;
+    if (!(hi > lo) || hi > 20.0 * lo) return;
     m->bcheb_lo = lo;
     m->bcheb_hi = hi;
     m->bcheb_state = 1;
@@ -909,7 +912,9 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
                            rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
         HIPCHECK(hipGetLastError());
         Outcome o;
+        int series = 0;  // 1: point series, 2: block-Jacobi series
         if (cheb_usable(m)) {
+            series = 1;
             const int K = cheb_run(m, r64, d64, nb, inner_rtol);
             o.cs.assign(size_t(nb), ColState{});
             for (ColState &x : o.cs) {
@@ -918,6 +923,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
             }
             o.fl = PcgFlags{};
         } else if (bcheb_candidate(m, nb) && m->bcheb_state == 1) {
+            series = 2;
             const int K = cheb_block_run(m, r64, d64, nb, inner_rtol);
             o.cs.assign(size_t(nb), ColState{});
             for (ColState &x : o.cs) {
@@ -961,8 +967,19 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.n = n;
         s.w = nb;
         launch_step(c, KMSD_C128, s);
+        const std::vector<double> before = rn;
         rn = colnorm(rr);
         ++checks;
+        // A series round whose interval misses the spectrum: some column that
+        // still needed work reduced its residual by less than the square
+        // root of the promised reduction (or not at all).  This metric's later
+        // inner solves are PCG.
+        if (series) {
+            bool stalled = false;
+            for (int32_t j = 0; j < nb; ++j)
+                if (before[j] > thr[j] && !(rn[j] <= std::sqrt(inner_rtol) * before[j])) stalled = true;
+            if (stalled) (series == 1 ? m->cheb_state : m->bcheb_state) = 2;
+        }
         bool brk = false;
         for (int32_t j = 0; j < nb; ++j) brk = brk || agg[j].breakdown;
         if (brk) break;

@@ -173,3 +173,47 @@ def test_block_jacobi_series_after_ritz_bounds(hip_ctx):
     assert not np.array_equal(host(outs[1][2][0]), host(outs[0][2][0]))
     for (Y1, _), (Y0, _) in zip(outs[1], outs[0]):
         assert rel_err(host(Y1), host(Y0)) < 2e-7
+
+
+def test_block_series_that_misses_the_spectrum_returns_to_pcg(hip_ctx):
+    """bcheb_lo_pct puts the block-Jacobi series' lower end at 75 % of the
+    largest eigenvalue of M^-1 S (dense generalized eigenvalues on the host),
+    so a series round leaves the low eigencomponents nearly untouched: the
+    refinement still reaches the FP64 residual and SciPy's solution, that
+    round (the first solve's second) makes the metric return to PCG, and the
+    later solves are the PCG solves of cheb_inner = 0 bit for bit."""
+    import scipy.linalg as la
+    from kubomsd_amd.backends.hip import HipMetric
+    from kubomsd_amd.synthetic import cubic_orbital_model
+    model = cubic_orbital_model(6, seed=4411)
+    S = sp.csr_matrix(model.S)
+    Sd = model.S.toarray()
+    Md = np.zeros_like(Sd)
+    for a, b in model.atom_slices:
+        Md[a:b, a:b] = Sd[a:b, a:b]
+    w = la.eigh(Sd, Md, eigvals_only=True)
+    assert w.max() / w.min() > 3
+    pct = int(100 * 0.75 * w.max() / w.min())
+    blocks = [(a, b, la.cholesky(model.S[a:b, a:b].toarray(), lower=True)) for a, b in model.atom_slices]
+    Bs = [random_block(model.n, 16, 60 + i) for i in range(3)]
+    outs = {}
+    for flag in (1, 0):
+        hip_ctx.set_tuning("cheb_inner", flag)
+        hip_ctx.set_tuning("bcheb_lo_pct", pct)
+        try:
+            met = HipMetric(hip_ctx, S, S.diagonal().real, blocks)
+            outs[flag] = [met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 1000) for B in Bs]
+        finally:
+            hip_ctx.set_tuning("cheb_inner", 1)
+            hip_ctx.set_tuning("bcheb_lo_pct", 92)
+    for flag in (1, 0):
+        for B, (Y, rep) in zip(Bs, outs[flag]):
+            Bq = B.astype(np.complex64).astype(np.complex128)
+            assert all(rep["converged"]) and max(rep["relative_residual"]) <= 1e-10
+            assert rel_err(host(Y), spla.spsolve(S.tocsc(), Bq)) < 2e-7
+    # first solve: a PCG round (its coefficients give the interval), one
+    # series round, then PCG; the later solves are PCG only
+    assert not np.array_equal(host(outs[1][0][0]), host(outs[0][0][0]))
+    for i in (1, 2):
+        np.testing.assert_array_equal(host(outs[1][i][0]), host(outs[0][i][0]))
+        assert list(outs[1][i][1]["iterations"]) == list(outs[0][i][1]["iterations"])
