@@ -60,6 +60,7 @@ extern "C" int kmsd_ctx_create(int device, uint64_t budget, void *stream, kmsd_c
         c->budget = budget ? budget : uint64_t(0.85 * double(fr));
         c->groups = int64_t(std::max(8, c->cus * 8));
         c->flag = static_cast<int32_t *>(dalloc(c, 256));
+        HIPCHECK(hipMemset(c->flag, 0, 256));  // flag[1], flag[2] stay zero between uses (launch_convert_check)
         HIPCHECK(hipHostMalloc(&c->pin, kPinBytes, hipHostMallocDefault));
         noalloc_setup(device);
     });

@@ -40,6 +40,7 @@ struct kmsd_ctx {
     bool fuse_velocity = true; // kmsd_chain_v forms V T_k inside the chain's steps where V fits H's ELL slots
     hipStream_t capture = nullptr;  // private non-blocking stream for graph capture (lazy)
     int32_t *flag = nullptr;  // device scratch flags (nonfinite detection)
+    int cvt_slot = 0;         // flag[1 + cvt_slot]: the next launch_convert_check's flag
     void *pin = nullptr;      // pinned host staging for status reads (kmsd::kPinBytes, allocated at creation)
     void *mu_part = nullptr;  // partial tables of kmsd_mu_table (grow-only)
     size_t mu_part_bytes = 0;
@@ -280,6 +281,13 @@ void launch_scale(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, void *O, i
 // (R, C) grid of pixels of `words` 8-byte words -> (C, R) grid (out != in).
 void launch_pixel_transpose(kmsd_ctx *c, const void *in, void *out, int64_t R, int64_t C, int64_t words);
 bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w);
+// Conversion between vector precisions with the finiteness check of the
+// converted block in the same pass, and no fill: launches alternate between
+// flag[1] and flag[2], each clearing the other one for the next launch.
+// Returns the flag (nonzero if some converted element is not finite) to read
+// before the next call.
+const int32_t *launch_convert_check(kmsd_ctx *c, int dtype_in, const void *X, int64_t ldx, int dtype_out, void *O,
+                                    int64_t ldo, int64_t n, int32_t w);
 void launch_convert(kmsd_ctx *c, int dtype_in, const void *X, int64_t ldx, int dtype_out, void *O, int64_t ldo,
                     int64_t n, int32_t w);
 

@@ -1175,6 +1175,25 @@ __global__ __launch_bounds__(256) void convert_kernel(const void *Xv, int64_t ld
     }
 }
 
+// convert_kernel with flag |= 1 if any converted entry is nonfinite;
+// thread 0 of workgroup 0 zeroes clear (another flag, for the next launch).
+template <class VA, class VB>
+__global__ __launch_bounds__(256) void convert_check_kernel(const void *Xv, int64_t ldx, void *Ov, int64_t ldo,
+                                                            int64_t n, int32_t w, int32_t *flag, int32_t *clear) {
+    const cx<VA> *X = static_cast<const cx<VA> *>(Xv);
+    cx<VB> *O = static_cast<cx<VB> *>(Ov);
+    const int64_t total = n * w;
+    bool bad = false;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t r = e / w, c = e - r * w;
+        const cx<VB> v = to<VB>(X[r * ldx + c]);
+        O[r * ldo + c] = v;
+        bad |= !(__builtin_isfinite(v.re) && __builtin_isfinite(v.im));
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
+}
+
 // flag |= 1 if any entry of the block is nonfinite.
 template <class VT>
 __global__ __launch_bounds__(256) void nonfinite_kernel(const void *Xv, int64_t ldx, int64_t n, int32_t w,

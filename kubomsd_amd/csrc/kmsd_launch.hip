@@ -714,6 +714,27 @@ void launch_convert(kmsd_ctx *c, int din, const void *X, int64_t ldx, int dout, 
     HIPCHECK(hipGetLastError());
 }
 
+const int32_t *launch_convert_check(kmsd_ctx *c, int din, const void *X, int64_t ldx, int dout, void *O, int64_t ldo,
+                                    int64_t n, int32_t w) {
+    const int64_t g = blocks_for(n * w);
+    int32_t *flag = c->flag + 1 + c->cvt_slot, *clear = c->flag + 2 - c->cvt_slot;
+    c->cvt_slot ^= 1;
+    if (din == KMSD_C128 && dout == KMSD_C64)
+        hipLaunchKernelGGL((convert_check_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx,
+                           O, ldo, n, w, flag, clear);
+    else if (din == KMSD_C64 && dout == KMSD_C128)
+        hipLaunchKernelGGL((convert_check_kernel<float, double>), dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx,
+                           O, ldo, n, w, flag, clear);
+    else if (din == KMSD_C64)
+        hipLaunchKernelGGL((convert_check_kernel<float, float>), dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx,
+                           O, ldo, n, w, flag, clear);
+    else
+        hipLaunchKernelGGL((convert_check_kernel<double, double>), dim3(unsigned(g)), dim3(256), 0, c->stream, X, ldx,
+                           O, ldo, n, w, flag, clear);
+    HIPCHECK(hipGetLastError());
+    return flag;
+}
+
 bool launch_nonfinite(kmsd_ctx *c, int dtype, const void *X, int64_t ldx, int64_t n, int32_t w) {
     int32_t got = 0;
     HIPCHECK(hipMemsetAsync(c->flag, 0, sizeof(int32_t), c->stream));

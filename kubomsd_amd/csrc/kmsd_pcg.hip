@@ -20,7 +20,7 @@ using namespace kmsd;
 
 struct kmsd_metric::Work {
     DBuf q, y, r, z, p, sp, t, cs, fl, part, part2;
-    DBuf b128, y128, r128, r64, d64, cs2, sums, hist;
+    DBuf b128, y128, r128, r64, d64, cs2, sums, rsums, hist;
 };
 
 namespace kmsd {
@@ -434,7 +434,8 @@ void account(kmsd_metric *m, const Outcome &o, int64_t extra_checks) {
 std::vector<double> column_norms(kmsd_metric *m, const double *part, int64_t groups, int32_t w) {
     kmsd_ctx *c = m->ctx;
     double *sums = static_cast<double *>(m->work->sums.ensure(c, sizeof(double) * size_t(w)));
-    hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, part, groups, w, sums);
+    hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(w)), dim3(256), 0, c->stream, part, groups, w, sums,
+                       static_cast<ColState *>(nullptr), 1.0);
     HIPCHECK(hipGetLastError());
     std::vector<double> out(static_cast<size_t>(w));
     read_status(c, {{out.data(), sums, sizeof(double) * size_t(w)}});
@@ -861,8 +862,13 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     launch_convert(c, dtype, B, ldb, KMSD_C128, bb, nb, n, nb);
     // y = 0 and r = b are implicit in the first round: it reads b as its
     // residual and assigns y (zeroed here only if no round runs).
-    // ||b|| per column (FP64) -> threshold
-    auto colnorm = [&](const void *X) {
+    // Column norms (FP64) are summed on the device into rsums[0, nb) for b
+    // and rsums[nb, 2 nb) for the latest residual; each colsum also stores
+    // the column scales of the next round's FP32 residual in cs2[j].safe
+    // (rn / sqrt(n), 1 for a zero column), so rounds need no host copy.
+    double *rsums = static_cast<double *>(wk.rsums.ensure(c, sizeof(double) * 2 * size_t(nb)));
+    const double sqrt_n = std::sqrt(double(n));
+    auto colnorm_launch = [&](const void *X, double *dst) {
         Step s;
         s.Z = X;
         s.ldz = nb;
@@ -872,47 +878,26 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.n = n;
         s.w = nb;
         const int64_t g = launch_step(c, KMSD_C128, s);
-        return column_norms(m, nrm, g, nb);
+        hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(nb)), dim3(256), 0, c->stream, nrm, g, nb, dst, cs2, sqrt_n);
+        HIPCHECK(hipGetLastError());
     };
-    const std::vector<double> bn = colnorm(bb);
-    std::vector<double> thr(nb);
-    for (int32_t j = 0; j < nb; ++j) thr[j] = std::max(rtol * bn[j], atol);
-    std::vector<double> rn = bn;
     std::vector<ColState> agg(nb);
     for (auto &s : agg) {
         s = ColState{};
         s.fin = 1;
-        s.safe = 1.0;
     }
-    int64_t checks = 1;
     int rounds = 0;
-    for (int outer = 0; outer < 12; ++outer) {
-        bool done = true;
-        // The inner solve reduces every column's residual by at least
-        // inner_rtol: 1e-6 (about what FP32 iterations reach) when some
-        // column is far from its threshold, else a quarter of the reduction
-        // the strictest column still needs (the last round of a refinement
-        // usually needs 1e-3 or 1e-4, not 1e-6).
-        double need = 1.0;
-        for (int32_t j = 0; j < nb; ++j) {
-            done = done && rn[j] <= thr[j];
-            if (rn[j] > thr[j]) need = std::min(need, thr[j] / rn[j]);
-        }
-        if (done) break;
-        const double inner_rtol = std::min(0.1, std::max(std::max(rtol, 1e-6), 0.25 * need));
-        // column scale of r: max component (kept in cs2[j].safe)
-        std::vector<ColState> sc(nb);
-        for (int32_t j = 0; j < nb; ++j) {
-            sc[j] = ColState{};
-            sc[j].safe = rn[j] > 0 ? rn[j] / std::sqrt(double(n)) : 1.0;
-        }
-        copy_h2d(c, cs2, sc.data(), sizeof(ColState) * nb);
+    // One refinement round: the FP32 residual r64 = r / scale, the inner
+    // solve to inner_rtol, y += scale d, r = b - S y in FP64 and its column
+    // norms on the device.  Returns 1 or 2 for a point or block series inner
+    // solve, 0 for PCG.
+    auto round = [&](double inner_rtol) {
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
         hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream,
                            rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
         HIPCHECK(hipGetLastError());
         Outcome o;
-        int series = 0;  // 1: point series, 2: block-Jacobi series
+        int series = 0;
         if (cheb_usable(m)) {
             series = 1;
             const int K = cheb_run(m, r64, d64, nb, inner_rtol);
@@ -967,9 +952,76 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.n = n;
         s.w = nb;
         launch_step(c, KMSD_C128, s);
+        colnorm_launch(rr, rsums + nb);
+        return series;
+    };
+    auto sqrt_all = [](std::vector<double> v) {
+        for (double &x : v) x = std::sqrt(x);
+        return v;
+    };
+    // Where the inner solves are series, rtol < 1e-7 (two rounds of FP32
+    // inner solves are always needed) and the block is small (at most 2^22
+    // elements: a series term takes about as long as a host round trip),
+    // the first two rounds run back to back with the reductions fixed
+    // beforehand: 1e-6 (what the first round would take, since every column
+    // then needs rtol) and a quarter of rtol / 1e-6, which may cost a term
+    // more than sizing the second round from the first one's residual;
+    // ||b||, the second residual's norms and the finiteness of the converted
+    // solution are read once after both (the conversion and its check are
+    // repeated only if a further round runs).
+    colnorm_launch(bb, rsums);
+    std::vector<double> bn, rn, thr(nb);
+    bool out_ready = false;
+    int32_t out_nonfinite = 0;
+    const double ir1 = std::max(rtol, 1e-6), ir2 = std::min(0.1, std::max(ir1, 0.25 * rtol / ir1));
+    const bool ahead = atol == 0 && rtol < 1e-7 && n * nb <= (int64_t(1) << 22) &&
+                       (cheb_usable(m) || (bcheb_candidate(m, nb) && m->bcheb_state == 1));
+    if (ahead) {
+        const int s1 = round(ir1), s2 = round(ir2);
+        const int32_t *fl = launch_convert_check(c, KMSD_C128, yy, nb, dtype, Y, ldy, n, nb);
+        std::vector<double> both(2 * size_t(nb));
+        read_status(c, {{both.data(), rsums, sizeof(double) * 2 * size_t(nb)},
+                        {&out_nonfinite, fl, sizeof(out_nonfinite)}});
+        out_ready = true;
+        bn = sqrt_all(std::vector<double>(both.begin(), both.begin() + nb));
+        rn = sqrt_all(std::vector<double>(both.begin() + nb, both.end()));
+        for (int32_t j = 0; j < nb; ++j) thr[j] = std::max(rtol * bn[j], atol);
+        // the two series rounds together: the stall rule below with their
+        // combined promise
+        bool stalled = false;
+        for (int32_t j = 0; j < nb; ++j)
+            if (bn[j] > thr[j] && !(rn[j] <= std::sqrt(ir1 * ir2) * bn[j])) stalled = true;
+        if (stalled) {
+            if (s1 == 1 || s2 == 1) m->cheb_state = 2;
+            if (s1 == 2 || s2 == 2) m->bcheb_state = 2;
+        }
+    } else {
+        std::vector<double> v(static_cast<size_t>(nb));
+        read_status(c, {{v.data(), rsums, sizeof(double) * size_t(nb)}});
+        bn = sqrt_all(v);
+        for (int32_t j = 0; j < nb; ++j) thr[j] = std::max(rtol * bn[j], atol);
+        rn = bn;
+    }
+    for (int outer = rounds; outer < 12; ++outer) {
+        bool done = true;
+        // The inner solve reduces every column's residual by at least
+        // inner_rtol: 1e-6 (about what FP32 iterations reach) when some
+        // column is far from its threshold, else a quarter of the reduction
+        // the strictest column still needs (the last round of a refinement
+        // usually needs 1e-3 or 1e-4, not 1e-6).
+        double need = 1.0;
+        for (int32_t j = 0; j < nb; ++j) {
+            done = done && rn[j] <= thr[j];
+            if (rn[j] > thr[j]) need = std::min(need, thr[j] / rn[j]);
+        }
+        if (done) break;
+        const double inner_rtol = std::min(0.1, std::max(std::max(rtol, 1e-6), 0.25 * need));
         const std::vector<double> before = rn;
-        rn = colnorm(rr);
-        ++checks;
+        const int series = round(inner_rtol);
+        out_ready = false;
+        std::vector<double> v(static_cast<size_t>(nb));
+        read_status(c, {{v.data(), rsums + nb, sizeof(double) * size_t(nb)}});
+        rn = sqrt_all(v);
         // A series round whose interval misses the spectrum: some column that
         // still needed work reduced its residual by less than the square
         // root of the promised reduction (or not at all).  This metric's later
@@ -986,9 +1038,14 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     }
     ++m->calls;
     if (!rounds) launch_scale(c, KMSD_C128, bb, nb, yy, nb, n, nb, Cplx{0, 0});
-    launch_convert(c, KMSD_C128, yy, nb, dtype, Y, ldy, n, nb);
     // finiteness of the stored solution per column
-    bool fin_all = !launch_nonfinite(c, dtype, Y, ldy, n, nb);
+    bool fin_all;
+    if (out_ready) {
+        fin_all = out_nonfinite == 0;
+    } else {
+        launch_convert(c, KMSD_C128, yy, nb, dtype, Y, ldy, n, nb);
+        fin_all = !launch_nonfinite(c, dtype, Y, ldy, n, nb);
+    }
     for (auto &s : agg) s.fin = s.fin && fin_all;
     std::vector<double> qn = bn;
     store_report(m, agg, &rn, &qn, &thr);
@@ -998,7 +1055,6 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         ok = ok && m->rep_conv[j];
         brk = brk || m->rep_brk[j];
     }
-    (void)checks;
     if (!ok)
         throw Error(KMSD_ENUMERIC, !fin_all ? "S solve solution is nonfinite or not representable after rescaling"
                                    : brk    ? "PCG non-positive/complex curvature breakdown"

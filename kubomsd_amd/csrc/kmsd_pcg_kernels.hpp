@@ -653,8 +653,11 @@ __global__ __launch_bounds__(256) void gershgorin_kernel(const void *offv, const
 // j: one workgroup per column, each lane adding groups t, t + 256, ... in
 // ascending order, then a fixed LDS tree (the reduction of pcg_col_kernel),
 // so the sums do not depend on timing.
+// out[j] = sum over the partial rows of column j; with cs, also the column's
+// refinement scale cs[j].safe = sqrt(out[j]) / sqrt_n (1 where out[j] is zero
+// or not finite).
 static __global__ __launch_bounds__(256) void colsum_kernel(const double *part, int64_t groups, int32_t w,
-                                                            double *out) {
+                                                            double *out, ColState *cs, double sqrt_n) {
     __shared__ double red[256];
     const int64_t j = blockIdx.x;
     double acc = 0.0;
@@ -673,7 +676,10 @@ static __global__ __launch_bounds__(256) void colsum_kernel(const double *part, 
         if (int(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+    if (threadIdx.x == 0) {
+        out[j] = red[0];
+        if (cs) cs[j].safe = red[0] > 0 ? sqrt(red[0]) / sqrt_n : 1.0;
+    }
 }
 
 // y128 += scale_j * d64 (refinement update), and r64 = r128 / scale_j.

@@ -217,3 +217,27 @@ def test_block_series_that_misses_the_spectrum_returns_to_pcg(hip_ctx):
     for i in (1, 2):
         np.testing.assert_array_equal(host(outs[1][i][0]), host(outs[0][i][0]))
         assert list(outs[1][i][1]["iterations"]) == list(outs[0][i][1]["iterations"])
+
+
+def test_unrepresentable_complex64_solution_then_ordinary_solves(hip_ctx):
+    """S = I / 4 (a one-point Gershgorin interval: the inner solves are
+    exact divisions).  A complex64 right-hand side of 3e38 has a solution of
+    1.2e39, which complex64 cannot hold: the solve raises.  The output's
+    finiteness flag is checked in the conversion pass, alternating between
+    two flags that each launch clears for the next: the two following
+    solves on the same context, the second reusing the flag the failing
+    one set, return 4 B."""
+    from kubomsd_amd.backends.hip import HipMetric
+    from kubomsd_amd.errors import NumericalError
+    n = 64
+    S = (0.25 * sp.identity(n, format="csr")).astype(np.complex128).tocsr()
+    met = HipMetric(hip_ctx, S, S.diagonal().real)
+    bad = np.ones((n, 2), np.complex64)
+    bad[0, 0] = 3e38
+    with pytest.raises(NumericalError, match="representable"):
+        met.solve(to_dev(hip_ctx, bad, "c64"), 1e-10, 0.0, 100)
+    for seed in (1, 2):
+        B = random_block(n, 2, seed)
+        Y, rep = met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 100)
+        assert all(rep["converged"])
+        np.testing.assert_allclose(host(Y), 4 * B.astype(np.complex64), rtol=1e-6)
