@@ -223,6 +223,7 @@ struct Step {
     int cpl_cap = 4;            // columns-per-lane ceiling (keeps the shape fixed across a chain)
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
+    bool norm = false;              // partials (|t|^2, 0) of the stored result (CSR/ELL matrices, no W)
     int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
     int64_t max_groups = 0;     // if nonzero (a multiple of 8) and `groups` is not set: at most this many
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)

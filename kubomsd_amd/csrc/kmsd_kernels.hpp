@@ -65,6 +65,7 @@ enum : uint32_t {
     F_ELL = 1u << 7,    // off-diagonal part is ELL (ell entries per row, row-major)
     F_MOM2 = 1u << 8,   // partial.re += |X_i|^2, partial.im += Re conj(X_i) t (moment doubling)
     F_VOUT = 1u << 9,   // VOUT_i = valpha * sum_k V_ik X_k from the same gathers (ell_step_kernel<..., true>)
+    F_NORM = 1u << 10,  // partial.re += |t|^2 of the stored t (CSR/ELL step kernels only)
 };
 
 struct StepArgs {
@@ -187,6 +188,13 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
             const cx<double> xv = xi.get(c), tv = ts.get(c);
             mom[c].re = __builtin_fma(xv.re, xv.re, __builtin_fma(xv.im, xv.im, mom[c].re));
             mom[c].im = __builtin_fma(xv.re, tv.re, __builtin_fma(xv.im, tv.im, mom[c].im));
+        }
+    }
+    if (fl & F_NORM) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const cx<double> tv = ts.get(c);
+            mom[c].re = __builtin_fma(tv.re, tv.re, __builtin_fma(tv.im, tv.im, mom[c].re));
         }
     }
     if (fl & F_MOM) {
@@ -354,7 +362,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, uint32_t bx, uint32
         step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
     }
 
-    if (!(fl & (F_MOM | F_MOM2))) return;
+    if (!(fl & (F_MOM | F_MOM2 | F_NORM))) return;
     step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, bx);
 }
 
@@ -485,7 +493,7 @@ __device__ __forceinline__ void ell_step_body(const StepArgs &a, uint32_t bx, ui
         dd = dn;
     }
 
-    if (!(fl & (F_MOM | F_MOM2))) return;
+    if (!(fl & (F_MOM | F_MOM2 | F_NORM))) return;
     step_partials<CPL>(a, mom, lane, wave, lpr, col, colok, bx);
 }
 

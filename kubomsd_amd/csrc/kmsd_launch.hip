@@ -565,6 +565,11 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.OUT) fl |= F_STORE;
     if (s.W) fl |= F_MOM;
     if (s.doubling) fl |= F_MOM2;
+    if (s.norm) {
+        if (bsr || s.W || s.doubling || !s.OUT)
+            throw Error(KMSD_EDEVICE, "internal error: norm partials need a CSR/ELL step with a stored result");
+        fl |= F_NORM;
+    }
     if (s.ACC) fl |= F_ACC;
     if (bsr && f32 && A->vkind != VK_COMPLEX && (c->bsr_mfma || s.bfac32)) mfma_shape(c, s, A, fl, a, sh, fn, key);
     if (s.bfac32 && !key.blk)

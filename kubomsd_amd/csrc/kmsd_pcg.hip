@@ -600,19 +600,27 @@ bool cheb_usable(kmsd_metric *m) {
 // rho^(K+1)/(1 - rho) |r| sets the term count for the requested residual
 // reduction.  r is overwritten (it holds T_(k-1) in turn).  Returns the
 // number of products.
-int cheb_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction) {
+// The first term's coefficient g_0 (1 / c where the interval is one point).
+double cheb_first(const kmsd_metric *m) {
+    const double lo = m->cheb_lo, hi = m->cheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
+    if (!(h > 1e-14 * cc)) return 1.0 / cc;
+    const double z = cc / h, sq = std::sqrt(z * z - 1.0);
+    return 0.5 * (2.0 / (h * sq));
+}
+
+int cheb_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction, bool d_ready) {
     kmsd_ctx *c = m->ctx;
     const int64_t n = m->n;
     const double lo = m->cheb_lo, hi = m->cheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
     if (!(h > 1e-14 * cc)) {
-        launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{1.0 / cc, 0.0});  // S = c I
+        if (!d_ready) launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{cheb_first(m), 0.0});  // S = c I
         return 0;
     }
     const double z = cc / h, sq = std::sqrt(z * z - 1.0), rho = z - sq, amp = 2.0 / (h * sq);
     int K = 1;
     while (K < 400 && hi * amp * std::pow(rho, double(K + 1)) / (1.0 - rho) > reduction) ++K;
     void *t1 = m->work->t.ensure(c, size_t(n) * size_t(nb) * 8);
-    launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{0.5 * amp, 0.0});
+    if (!d_ready) launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{cheb_first(m), 0.0});
     double g = -amp * rho;
     Step s;
     s.A = m->S;
@@ -881,6 +889,8 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(nb)), dim3(256), 0, c->stream, nrm, g, nb, dst, cs2, sqrt_n);
         HIPCHECK(hipGetLastError());
     };
+    // CSR/ELL S: the residual product writes the norm partials itself
+    const bool fuse_norm = m->S && m->S->fmt != KMSD_BSR4;
     std::vector<ColState> agg(nb);
     for (auto &s : agg) {
         s = ColState{};
@@ -893,14 +903,17 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     // solve, 0 for PCG.
     auto round = [&](double inner_rtol) {
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
+        // point series: the conversion also writes the series' first term
+        const bool point = cheb_usable(m);
         hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream,
-                           rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0);
+                           rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0, point ? d64 : nullptr,
+                           point ? cheb_first(m) : 0.0);
         HIPCHECK(hipGetLastError());
         Outcome o;
         int series = 0;
-        if (cheb_usable(m)) {
+        if (point) {
             series = 1;
-            const int K = cheb_run(m, r64, d64, nb, inner_rtol);
+            const int K = cheb_run(m, r64, d64, nb, inner_rtol, true);
             o.cs.assign(size_t(nb), ColState{});
             for (ColState &x : o.cs) {
                 x.iters = K;
@@ -951,8 +964,17 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
         s.ldo = nb;
         s.n = n;
         s.w = nb;
-        launch_step(c, KMSD_C128, s);
-        colnorm_launch(rr, rsums + nb);
+        if (fuse_norm) {
+            s.norm = true;
+            s.part = nrm;
+            const int64_t gr = launch_step(c, KMSD_C128, s);
+            hipLaunchKernelGGL(colsum_kernel, dim3(unsigned(nb)), dim3(256), 0, c->stream, nrm, gr, nb, rsums + nb,
+                               cs2, sqrt_n);
+            HIPCHECK(hipGetLastError());
+        } else {
+            launch_step(c, KMSD_C128, s);
+            colnorm_launch(rr, rsums + nb);
+        }
         return series;
     };
     auto sqrt_all = [](std::vector<double> v) {

@@ -683,18 +683,27 @@ static __global__ __launch_bounds__(256) void colsum_kernel(const double *part, 
 }
 
 // y128 += scale_j * d64 (refinement update), and r64 = r128 / scale_j.
+// With D (mode 0 only): also D = dscale * O from the stored O, as
+// scale_kernel computes it (the first term of a Chebyshev series).
 template <class VA, class VB>
 __global__ __launch_bounds__(256) void scaled_convert_kernel(const void *Xv, int64_t ldx, void *Ov, int64_t ldo,
-                                                             int64_t n, int32_t w, const ColState *cs, int32_t mode) {
+                                                             int64_t n, int32_t w, const ColState *cs, int32_t mode,
+                                                             void *Dv = nullptr, double dscale = 0.0) {
     const cx<VA> *X = static_cast<const cx<VA> *>(Xv);
     cx<VB> *O = static_cast<cx<VB> *>(Ov);
+    cx<VB> *D = static_cast<cx<VB> *>(Dv);
     const int64_t total = n * w;
     for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
         const int64_t r = e / w, c = e - r * w;
         const cx<double> x = to<double>(X[r * ldx + c]);
         if (mode == 0) {  // O = X / safe
             const double inv = 1.0 / cs[c].safe;
-            O[r * ldo + c] = to<VB>(cx<double>{x.re * inv, x.im * inv});
+            const cx<VB> o = to<VB>(cx<double>{x.re * inv, x.im * inv});
+            O[r * ldo + c] = o;
+            if (D) {
+                const cx<double> y = to<double>(o);
+                D[r * ldo + c] = to<VB>(cx<double>{dscale * y.re - 0.0 * y.im, dscale * y.im + 0.0 * y.re});
+            }
         } else if (mode == 2) {  // O = X * safe
             O[r * ldo + c] = to<VB>(cx<double>{x.re * cs[c].safe, x.im * cs[c].safe});
         } else {  // O += X * safe
