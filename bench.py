@@ -96,6 +96,8 @@ def parse():
     p.add_argument("--flake-n", type=float, default=17672, help="flake: orbitals (about)")
     p.add_argument("--pcg-min-blocks", type=int, default=0,
                    help="override the least row blocks per workgroup of PCG vector passes")
+    p.add_argument("--bcheb-lo-pct", type=int, default=0,
+                   help="nonorthogonal: block-Jacobi series lower end in percent of the smallest Ritz value")
     p.add_argument("--side", type=int, default=2048, help="mu: square lattice side")
     p.add_argument("--samples", type=int, default=32, help="mu: random vectors (one table each)")
     p.add_argument("--atoms", type=int, default=64, help="nonorthogonal: atoms per cube side")
@@ -277,6 +279,8 @@ def nonorthogonal(args):
         ctx.set_tuning("tile_bytes", args.tile_bytes)
     if args.no_bsr_mfma:
         ctx.set_tuning("bsr_mfma", 0)
+    if args.bcheb_lo_pct:
+        ctx.set_tuning("bcheb_lo_pct", args.bcheb_lo_pct)
     t_build = time.perf_counter()
     model = cubic_orbital_model(args.atoms)
     n, nb, M = model.n, args.vectors, args.moments

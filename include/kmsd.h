@@ -130,7 +130,7 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * series in the later ones; the acceptance stays the FP64 true residual, and
  * a series round that reduces some column's residual by less than the square
  * root of the reduction its interval promised returns that metric to PCG
- * inner solves; 0: PCG inner solves), "bcheb_lo_pct" (1..10000, default 92:
+ * inner solves; 0: PCG inner solves), "bcheb_lo_pct" (1..10000, default 96:
  * the block-Jacobi series' lower end in percent of the smallest Ritz value;
  * its upper end is the largest Ritz value plus 2 %). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);

@@ -27,7 +27,7 @@ struct kmsd_ctx {
     int min_blocks = 1;      // row blocks per workgroup at least (small operators: fewer, fuller workgroups)
     int pcg_min_blocks = 4;  // the same for the vector passes and products of PCG solves
     bool cheb_inner = true;  // complex64 refinement: Chebyshev inner solves where S's Gershgorin interval allows
-    int bcheb_lo_pct = 92;   // block-Jacobi series: lower end in percent of the smallest Ritz value
+    int bcheb_lo_pct = 96;   // block-Jacobi series: lower end in percent of the smallest Ritz value
     int cpl[2] = {0, 0};    // columns per lane of complex64 [0] and complex128 [1] blocks; 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums

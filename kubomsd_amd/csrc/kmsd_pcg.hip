@@ -724,7 +724,7 @@ void tri_extremes(const std::vector<double> &a, const std::vector<double> &b2, d
 // Lanczos tridiagonal of the preconditioned operator, T_kk = 1/alpha_k +
 // beta_(k-1)/alpha_(k-1), T_k,k+1 = sqrt(beta_k)/alpha_k; the extreme Ritz
 // values over the columns with at least 6 iterations; the lower end is
-// bcheb_lo_pct percent of the smallest (92 by default), the upper end 2 %
+// bcheb_lo_pct percent of the smallest (96 by default), the upper end 2 %
 // above the largest.  Ritz values lie inside the spectrum; a series whose
 // interval misses part of it still reduces every eigencomponent in
 // (0, lo + hi), only more slowly below lo, the refinement's FP64 residual

@@ -205,7 +205,7 @@ def test_block_series_that_misses_the_spectrum_returns_to_pcg(hip_ctx):
             outs[flag] = [met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 1000) for B in Bs]
         finally:
             hip_ctx.set_tuning("cheb_inner", 1)
-            hip_ctx.set_tuning("bcheb_lo_pct", 92)
+            hip_ctx.set_tuning("bcheb_lo_pct", 96)
     for flag in (1, 0):
         for B, (Y, rep) in zip(Bs, outs[flag]):
             Bq = B.astype(np.complex64).astype(np.complex128)
