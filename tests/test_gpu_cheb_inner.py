@@ -241,3 +241,24 @@ def test_unrepresentable_complex64_solution_then_ordinary_solves(hip_ctx):
         Y, rep = met.solve(to_dev(hip_ctx, B, "c64"), 1e-10, 0.0, 100)
         assert all(rep["converged"])
         np.testing.assert_allclose(host(Y), 4 * B.astype(np.complex64), rtol=1e-6)
+
+
+@pytest.mark.parametrize("rtol,atol", [(1e-6, 0.0), (1e-10, 1e-9)])
+def test_point_series_rounds_sized_one_by_one(hip_ctx, rtol, atol):
+    """Solves outside the two-round lookahead (rtol >= 1e-7, or atol > 0):
+    each refinement round is sized from the previous residual, read on the
+    host.  The FP64 residual meets max(rtol ||b||, atol) per column and the
+    solution agrees with SciPy's to that accuracy."""
+    from kubomsd_amd.backends.hip import HipMetric
+    _, S = flake_S(n=600, seed=29)
+    B = random_block(S.shape[0], 8, 31)
+    met = HipMetric(hip_ctx, S, S.diagonal().real)
+    Y, rep = met.solve(to_dev(hip_ctx, B, "c64"), rtol, atol, 1000)
+    Y = host(Y)
+    Bq = B.astype(np.complex64).astype(np.complex128)
+    assert all(rep["converged"])
+    thr = np.maximum(rtol * np.linalg.norm(Bq, axis=0), atol)
+    res = np.linalg.norm(Bq - S @ Y.astype(np.complex128), axis=0)
+    assert (res <= np.maximum(thr, 2e-7 * np.linalg.norm(Bq, axis=0))).all()  # plus one complex64 rounding
+    assert len(set(rep["iterations"])) == 1 and rep["iterations"][0] > 0
+    assert rel_err(Y, spla.spsolve(S.tocsc(), Bq)) < max(10 * rtol, 2e-7)
