@@ -97,7 +97,7 @@ def parse():
     p.add_argument("--pcg-min-blocks", type=int, default=0,
                    help="override the least row blocks per workgroup of PCG vector passes")
     p.add_argument("--bcheb-lo-pct", type=int, default=0,
-                   help="nonorthogonal: block-Jacobi series lower end in percent of the smallest Ritz value")
+                   help="nonorthogonal: block-Jacobi Chebyshev iteration interval lower end in percent of the smallest Ritz value")
     p.add_argument("--side", type=int, default=2048, help="mu: square lattice side")
     p.add_argument("--samples", type=int, default=32, help="mu: random vectors (one table each)")
     p.add_argument("--atoms", type=int, default=64, help="nonorthogonal: atoms per cube side")
@@ -250,12 +250,12 @@ def atom_block_factors(S, norb):
 
 def inner_step_bytes(nnz_s, n, nb, s_val, s_vec):
     """Lower bound of HBM bytes per inner step of a metric solve on an
-    nb-column block: the S product (values, indices, row pointers) and five
+    nb-column block: the S product (values, indices, row pointers) and four
     block streams.  A PCG iteration moves at least six (read P; read and
-    write R; read and write Y; write P), a Chebyshev series term five (read
-    T_k; read T_(k-1); write T_(k+1); read and write the solution), so five
-    bounds both."""
-    return nnz_s * (s_val + 4) + 4 * (n + 1) + 5 * n * nb * s_vec
+    write R; read and write Y; write P), a step of the block-Jacobi Chebyshev
+    semi-iteration four (read d_k; read d_(k-1); read M^-1 r; write
+    d_(k+1)), so four bounds both."""
+    return nnz_s * (s_val + 4) + 4 * (n + 1) + 4 * n * nb * s_vec
 
 
 def nonorthogonal(args):
@@ -263,9 +263,10 @@ def nonorthogonal(args):
     second shell, nonorthogonal S; 32 vectors; M = 1024 moments of S^-1 H, each
     order one H product plus a block-Jacobi solve of S (FP32 inner solves to
     1e-6 inside FP64 refinement to 1e-10: PCG for the first inner solve, whose
-    CG coefficients give the Ritz bounds of the block-Jacobi Chebyshev series
-    that the later inner solves use).  Reports vector-steps/s and inner steps/s
-    (PCG iterations or series terms, max over columns per solve)."""
+    CG coefficients give the Ritz bounds of the block-Jacobi Chebyshev
+    semi-iteration that the later inner solves use).  Reports vector-steps/s
+    and inner steps/s (S products of PCG iterations or semi-iteration steps,
+    max over columns per solve)."""
     import torch
     from kubomsd_amd.backends.hip import HipContext, HipEngine, HipMetric
     from kubomsd_amd.synthetic import cubic_orbital_model
@@ -356,7 +357,7 @@ def nonorthogonal(args):
         "dtype": "c64+f64", "data": "synthetic (seeded simple-cubic orbital model, NumPy PCG64 random phases)",
         "config": {"workload": f"simple cubic {args.atoms}^3 atoms x 4 orbitals (N={n}), nonorthogonal S, "
                                f"{nb} vectors, M={M} moments of S^-1 H, {args.preconditioner} preconditioned FP32 inner "
-                               f"solves to 1e-6 (PCG, then Chebyshev series on its Ritz bounds) inside FP64 refinement "
+                               f"solves to 1e-6 (PCG, then the Chebyshev semi-iteration on its Ritz bounds) inside FP64 refinement "
                                f"to 1e-10, one moment chain per step",
                    "n": n, "nnz_H": int(model.H.nnz), "nnz_S": int(model.S.nnz), "vectors": nb, "moments": M,
                    "precision": "mixed", "parallelism": "single GPU", "storage": eng.H.info()["format"],

@@ -126,12 +126,13 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * [lo, hi] with lo > 0 and hi <= 4 lo take Chebyshev-series inner solves
  * inside the FP64 refinement, one fused product per term and no
  * reductions; uniform 4x4 block-Jacobi on a 4x4-block S: the first inner
- * solve is PCG, whose CG coefficients give Ritz bounds for block-Jacobi
- * series in the later ones; the acceptance stays the FP64 true residual, and
+ * solve is PCG, whose CG coefficients give Ritz bounds for the block-Jacobi
+ * Chebyshev semi-iteration in the later ones; the acceptance stays the FP64 true residual, and
  * a series round that reduces some column's residual by less than the square
  * root of the reduction its interval promised returns that metric to PCG
  * inner solves; 0: PCG inner solves), "bcheb_lo_pct" (1..10000, default 96:
- * the block-Jacobi series' lower end in percent of the smallest Ritz value;
+ * the block-Jacobi iteration interval's lower end in percent of the smallest
+ * Ritz value;
  * its upper end is the largest Ritz value plus 2 %). */
 int kmsd_ctx_set_tuning(kmsd_ctx *ctx, const char *key, int64_t value);
 /* No reference counterpart (the reference's CPU kernel has no launch shape).

@@ -27,7 +27,7 @@ struct kmsd_ctx {
     int min_blocks = 1;      // row blocks per workgroup at least (small operators: fewer, fuller workgroups)
     int pcg_min_blocks = 4;  // the same for the vector passes and products of PCG solves
     bool cheb_inner = true;  // complex64 refinement: Chebyshev inner solves where S's Gershgorin interval allows
-    int bcheb_lo_pct = 96;   // block-Jacobi series: lower end in percent of the smallest Ritz value
+    int bcheb_lo_pct = 96;   // block-Jacobi Chebyshev iteration: lower end in percent of the smallest Ritz value
     int cpl[2] = {0, 0};    // columns per lane of complex64 [0] and complex128 [1] blocks; 0 = automatic
     int moment_chunk = 16;  // Chebyshev orders per partial-reduction launch
     bool fp32_math = false; // complex64 blocks: FP32 matrix values and row sums
@@ -224,12 +224,14 @@ struct Step {
     const int32_t *gate = nullptr;  // device flag: the launch is skipped when *gate == 0
     bool doubling = false;          // partials (|X_i|^2, Re conj(X_i) t) instead of conj(W_i) t
     bool norm = false;              // partials (|t|^2, 0) of the stored result (CSR/ELL matrices, no W)
+    bool rhs = false;               // W is a right-hand side added as coef * W_i (needs bfac32; no ACC)
     int64_t groups = 0;         // if nonzero, the launch uses exactly this many workgroups (a multiple of 8)
     int64_t max_groups = 0;     // if nonzero (a multiple of 8) and `groups` is not set: at most this many
     bool f32_matrix = false;    // complex64 blocks: FP32 matrix values (row sums stay FP64)
     // 4x4-block products on the matrix cores: block-Jacobi solve of each block
-    // row's result with these FP32 factors before the epilogue (F_BETA and
-    // F_ACC allowed); the launch fails if the matrix-core kernel cannot run it
+    // row's result with these FP32 factors before the epilogue (F_BETA and a
+    // right-hand side, `rhs`, allowed; no accumulation or partial sums); the
+    // launch fails if the matrix-core kernel cannot run it
     const float *bfac32 = nullptr;
     // Fused second product (real width-3/4/6 ELL A with FP64 values): VOUT =
     // valpha * V X, V's real values in A's ELL slots.

@@ -66,6 +66,7 @@ enum : uint32_t {
     F_MOM2 = 1u << 8,   // partial.re += |X_i|^2, partial.im += Re conj(X_i) t (moment doubling)
     F_VOUT = 1u << 9,   // VOUT_i = valpha * sum_k V_ik X_k from the same gathers (ell_step_kernel<..., true>)
     F_NORM = 1u << 10,  // partial.re += |t|^2 of the stored t (CSR/ELL step kernels only)
+    F_RHS = 1u << 11,   // t += coef * W_i, W a right-hand side (block-Jacobi matrix-core kernel only)
 };
 
 struct StepArgs {
@@ -151,7 +152,7 @@ struct Pack {
 // optional store, series accumulation and moment partials.  Arithmetic in
 // FP64 whatever the storage type: one rounding per stored element, and
 // alpha/beta/gamma are never rounded to FP32.
-template <class VT, class AT, int CPL>
+template <class VT, class AT, int CPL, bool RHS = false>
 __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, int64_t row, int col,
                                               const cx<AT> (&y)[CPL], const Pack<VT, CPL> &xi,
                                               const Pack<VT, CPL> &pv, const Pack<VT, CPL> &wv,
@@ -172,7 +173,15 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
             t.re = __builtin_fma(a.gamma, p.re, t.re);
             t.im = __builtin_fma(a.gamma, p.im, t.im);
         }
+        if constexpr (RHS) {
+            if (fl & F_RHS) {
+                const cx<double> b = cmul<double>(a.coef_re, a.coef_im, wv.get(c));
+                t.re += b.re;
+                t.im += b.im;
+            }
+        }
         ts.set(c, t);
+        if (RHS) continue;  // the block-Jacobi semi-iteration: no accumulation, no partials
         if (fl & F_ACC) {
             cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
             const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t);
@@ -182,6 +191,7 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
         }
     }
     if (fl & F_STORE) ts.store_nt(static_cast<V *>(a.OUT) + row * a.ldo + col);
+    if constexpr (RHS) return;
     if (fl & F_MOM2) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
@@ -917,10 +927,11 @@ __device__ __forceinline__ void bsr4_mfma_body(const StepArgs &a) {
                 if (fl & F_BETA) xi.load(static_cast<const V *>(a.X) + (row0 + r) * a.ldx + col);
             }
             if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
-            if (fl & F_MOM) wv.load_nt(W + (row0 + r) * a.ldw + col);
-            step_epilogue<float, double, CPL>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
+            if (fl & (BLK ? F_RHS : F_MOM)) wv.load_nt(W + (row0 + r) * a.ldw + col);
+            step_epilogue<float, double, CPL, BLK>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
         }
     }
+    if constexpr (BLK) return;
     if (!(fl & F_MOM)) return;
     // Lanes of one column (lane bits >= lpr_log2) by shuffles, the four
     // wavefronts through LDS, one partial per (workgroup, column).
@@ -953,8 +964,9 @@ __device__ __forceinline__ void bsr4_mfma_body(const StepArgs &a) {
 static __global__ __launch_bounds__(256) void bsr4_mfma_kernel(const StepArgs a) { bsr4_mfma_body<false>(a); }
 
 // The same product with a 4x4 block-Jacobi solve of each block row's result
-// before the epilogue (a.bfac32): one term of a preconditioned Chebyshev
-// series in one launch.
+// before the epilogue (a.bfac32), which adds the own row (F_BETA), the
+// previous iterate (F_P) and the preconditioned right-hand side (F_RHS): one
+// step of the block-Jacobi Chebyshev semi-iteration in one launch.
 // (three wavefronts per SIMD, as the plain product, for its gather latency)
 static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bsr4_mfma_blk_kernel(
     const StepArgs a) {

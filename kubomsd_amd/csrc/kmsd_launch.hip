@@ -449,7 +449,7 @@ namespace {
 // offsets are 32-bit); otherwise the launch keeps the VALU kernel.
 void mfma_shape(kmsd_ctx *c, const Step &s, const kmsd_matrix *A, uint32_t fl, StepArgs &a, Shape &sh, KernelFn &fn,
                 KernelKey &key) {
-    const uint32_t allowed = F_SPMM | F_STORE | F_MOM | F_P | (s.bfac32 ? (F_BETA | F_ACC) : 0u);
+    const uint32_t allowed = F_SPMM | F_STORE | F_P | (s.bfac32 ? (F_BETA | F_RHS) : F_MOM);
     if (fl & ~allowed) return;
     if (s.w % 4 || cpl_limit(s, KMSD_C64) < 4) return;
     const int64_t lim = (int64_t(1) << 31) - (int64_t(1) << 16);
@@ -563,7 +563,13 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.use_beta) fl |= F_BETA;
     if (s.use_p) fl |= F_P;
     if (s.OUT) fl |= F_STORE;
-    if (s.W) fl |= F_MOM;
+    if (s.rhs) {
+        if (!s.bfac32 || !s.W || s.ACC)
+            throw Error(KMSD_EDEVICE, "internal error: a right-hand-side operand needs a block-Jacobi product");
+        fl |= F_RHS;
+    } else if (s.W) {
+        fl |= F_MOM;
+    }
     if (s.doubling) fl |= F_MOM2;
     if (s.norm) {
         if (bsr || s.W || s.doubling || !s.OUT)

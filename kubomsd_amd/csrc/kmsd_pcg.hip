@@ -719,13 +719,13 @@ void tri_extremes(const std::vector<double> &a, const std::vector<double> &b2, d
     hi = u;
 }
 
-// Interval for the block-Jacobi series from the Ritz values of a PCG inner
+// Interval for the block-Jacobi Chebyshev semi-iteration from the Ritz values of a PCG inner
 // solve: the CG coefficients (alpha_k, beta_k) of each column give the
 // Lanczos tridiagonal of the preconditioned operator, T_kk = 1/alpha_k +
 // beta_(k-1)/alpha_(k-1), T_k,k+1 = sqrt(beta_k)/alpha_k; the extreme Ritz
 // values over the columns with at least 6 iterations; the lower end is
 // bcheb_lo_pct percent of the smallest (96 by default), the upper end 2 %
-// above the largest.  Ritz values lie inside the spectrum; a series whose
+// above the largest.  Ritz values lie inside the spectrum; an iteration whose
 // interval misses part of it still reduces every eigencomponent in
 // (0, lo + hi), only more slowly below lo, the refinement's FP64 residual
 // decides acceptance, and a refinement round that reduces the residual by
@@ -772,59 +772,64 @@ void bcheb_measure(kmsd_metric *m, const double *hist, int32_t nb, int32_t hist_
     m->bcheb_state = 1;
 }
 
-// Block-Jacobi Chebyshev inner solve: d = sum_k g_k T_k((M^-1 S - c)/h) M^-1 r
-// on [bcheb_lo, bcheb_hi] (coefficients and term count as in cheb_run), each
-// term one matrix-core product with the 4x4 block solve of its result, the
-// recurrence and the accumulation in the epilogue (bsr4_mfma_blk_kernel).
-// r is overwritten.  Returns the number of products.
+// Block-Jacobi Chebyshev semi-iteration for S d = r with M^-1 S's spectrum
+// taken as [bcheb_lo, bcheb_hi]: with b = M^-1 r, theta and delta the
+// interval's center and half width and sigma = theta / delta,
+//   d_1 = b / theta,
+//   d_(k+1) = omega_(k+1) (d_k + (b - M^-1 S d_k) / theta) + (1 - omega_(k+1)) d_(k-1),
+//   omega_2 = 1 / (1 - 1 / (2 sigma^2)), omega_(k+1) = 1 / (1 - omega_k / (4 sigma^2)),
+// so that b - M^-1 S d_K = T_K((theta - M^-1 S) / delta) / T_K(sigma) b, the
+// residual polynomial of least maximum on the interval; K is the least step
+// count with 1 / T_K(sigma) <= reduction.  Each step after the first is one
+// matrix-core product with the 4x4 block solve of its result and the
+// combination with d_k, d_(k-1) and b in the epilogue (bsr4_mfma_blk_kernel),
+// d_(k+1) replacing d_(k-1) row by row.  r is overwritten.  Returns the
+// number of products (K - 1).
 int cheb_block_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction) {
     kmsd_ctx *c = m->ctx;
     const int64_t n = m->n;
-    const double lo = m->bcheb_lo, hi = m->bcheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
-    const double z = cc / h, sq = std::sqrt(z * z - 1.0), rho = z - sq, amp = 2.0 / (h * sq);
-    int K = 1;
-    while (K < 400 && hi * amp * std::pow(rho, double(K + 1)) / (1.0 - rho) > reduction) ++K;
-    void *t0 = m->work->z.ensure(c, size_t(n) * size_t(nb) * 8);
+    const double lo = m->bcheb_lo, hi = m->bcheb_hi, theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    const int K = std::max(1, std::min(400, int(std::ceil(std::acosh(1.0 / reduction) / std::acosh(sigma)))));
+    void *b = m->work->z.ensure(c, size_t(n) * size_t(nb) * 8);
     const int64_t nq = n / 4;
     hipLaunchKernelGGL(block_apply32_kernel, dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((nq * nb + 255) / 256, 16384)))),
-                       dim3(256), 0, c->stream, r, t0, m->factors32, nq, nb);
+                       dim3(256), 0, c->stream, r, b, m->factors32, nq, nb);
     HIPCHECK(hipGetLastError());
-    launch_scale(c, KMSD_C64, t0, nb, d, nb, n, nb, Cplx{0.5 * amp, 0.0});
-    double g = -amp * rho;
+    // d_K lands in d: d_1 goes to d when K is odd, to r when K is even
+    void *cur = (K % 2) ? d : r, *prev = (K % 2) ? r : d;
+    launch_scale(c, KMSD_C64, b, nb, cur, nb, n, nb, Cplx{1.0 / theta, 0.0});
     Step s;
     s.A = m->S;
     s.bfac32 = m->factors32;
-    s.X = t0;
     s.ldx = nb;
-    s.alpha = Cplx{1.0 / h, 0.0};
     s.use_beta = true;
-    s.beta = Cplx{-cc / h, 0.0};
-    s.OUT = r;
+    s.W = b;
+    s.ldw = nb;
+    s.rhs = true;
+    s.ldp = nb;
     s.ldo = nb;
-    s.ACC = d;
-    s.lda = nb;
-    s.coef = Cplx{g, 0.0};
     s.n = n;
     s.w = nb;
     s.f32_matrix = true;
-    launch_step(c, KMSD_C64, s);
-    void *prev = t0, *cur = r;
+    double omega = 1.0;
     for (int k = 1; k < K; ++k) {
-        g *= -rho;
+        omega = k == 1 ? 1.0 / (1.0 - 1.0 / (2.0 * sigma * sigma)) : 1.0 / (1.0 - omega / (4.0 * sigma * sigma));
         Step u = s;
         u.X = cur;
-        u.alpha = Cplx{2.0 / h, 0.0};
-        u.beta = Cplx{-2.0 * cc / h, 0.0};
-        u.P = prev;
-        u.ldp = nb;
-        u.use_p = true;
-        u.gamma = -1.0;
-        u.OUT = prev;  // T_(k+1) replaces T_(k-1) row by row
-        u.coef = Cplx{g, 0.0};
+        u.alpha = Cplx{-omega / theta, 0.0};
+        u.beta = Cplx{omega, 0.0};
+        u.coef = Cplx{omega / theta, 0.0};
+        if (k > 1) {  // d_0 = 0
+            u.P = prev;
+            u.use_p = true;
+            u.gamma = 1.0 - omega;
+        }
+        u.OUT = prev;
         launch_step(c, KMSD_C64, u);
         std::swap(prev, cur);
     }
-    return K;
+    return K - 1;
 }
 
 }  // namespace
@@ -899,7 +904,7 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     int rounds = 0;
     // One refinement round: the FP32 residual r64 = r / scale, the inner
     // solve to inner_rtol, y += scale d, r = b - S y in FP64 and its column
-    // norms on the device.  Returns 1 or 2 for a point or block series inner
+    // norms on the device.  Returns 1 or 2 for a point series or block semi-iteration inner
     // solve, 0 for PCG.
     auto round = [&](double inner_rtol) {
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));

@@ -140,10 +140,10 @@ def test_block_jacobi_series_after_ritz_bounds(hip_ctx):
     """4x4-block S with atom-block Jacobi (the simple-cubic orbital model):
     the first complex64 inner solve runs PCG and records its CG coefficients;
     the Ritz values of that Lanczos tridiagonal, widened, give the interval of
-    the block-Jacobi Chebyshev series that later inner solves use
-    (one matrix-core product with the block solve, recurrence and
-    accumulation per term).  Every solve reaches the FP64 residual and
-    SciPy's solution; the series solves run the same terms on every column;
+    the block-Jacobi Chebyshev semi-iteration that later inner solves use
+    (one matrix-core product with the block solve and the three-term update
+    per step).  Every solve reaches the FP64 residual and SciPy's solution;
+    the semi-iteration solves run the same steps on every column;
     with cheb_inner = 0 all solves stay PCG and agree to the rounding."""
     import scipy.linalg as la
     from kubomsd_amd.backends.hip import HipMetric
@@ -176,7 +176,7 @@ def test_block_jacobi_series_after_ritz_bounds(hip_ctx):
 
 
 def test_block_series_that_misses_the_spectrum_returns_to_pcg(hip_ctx):
-    """bcheb_lo_pct puts the block-Jacobi series' lower end at 75 % of the
+    """bcheb_lo_pct puts the block-Jacobi iteration interval's lower end at 75 % of the
     largest eigenvalue of M^-1 S (dense generalized eigenvalues on the host),
     so a series round leaves the low eigencomponents nearly untouched: the
     refinement still reaches the FP64 residual and SciPy's solution, that
