@@ -66,7 +66,7 @@ enum : uint32_t {
     F_MOM2 = 1u << 8,   // partial.re += |X_i|^2, partial.im += Re conj(X_i) t (moment doubling)
     F_VOUT = 1u << 9,   // VOUT_i = valpha * sum_k V_ik X_k from the same gathers (ell_step_kernel<..., true>)
     F_NORM = 1u << 10,  // partial.re += |t|^2 of the stored t (CSR/ELL step kernels only)
-    F_RHS = 1u << 11,   // t += coef * W_i, W a right-hand side (block-Jacobi matrix-core kernel only)
+    F_RHS = 1u << 11,   // t += coef * W_i, W a right-hand side (CSR/ELL step and block-Jacobi matrix-core kernels)
 };
 
 struct StepArgs {
@@ -152,7 +152,10 @@ struct Pack {
 // optional store, series accumulation and moment partials.  Arithmetic in
 // FP64 whatever the storage type: one rounding per stored element, and
 // alpha/beta/gamma are never rounded to FP32.
-template <class VT, class AT, int CPL, bool RHS = false>
+// RHS: F_RHS is compiled in (FP32-valued products of the inner solves only,
+// keeping the other instantiations' registers); LEAN: the block-Jacobi
+// semi-iteration's epilogue, without accumulation or partial sums.
+template <class VT, class AT, int CPL, bool RHS = false, bool LEAN = false>
 __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, int64_t row, int col,
                                               const cx<AT> (&y)[CPL], const Pack<VT, CPL> &xi,
                                               const Pack<VT, CPL> &pv, const Pack<VT, CPL> &wv,
@@ -181,7 +184,7 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
             }
         }
         ts.set(c, t);
-        if (RHS) continue;  // the block-Jacobi semi-iteration: no accumulation, no partials
+        if (LEAN) continue;  // the block-Jacobi semi-iteration: no accumulation, no partials
         if (fl & F_ACC) {
             cx<double> acc = to<double>(ACC[row * a.lda + col + c]);
             const cx<double> add = cmul<double>(a.coef_re, a.coef_im, t);
@@ -191,7 +194,7 @@ __device__ __forceinline__ void step_epilogue(const StepArgs &a, uint32_t fl, in
         }
     }
     if (fl & F_STORE) ts.store_nt(static_cast<V *>(a.OUT) + row * a.ldo + col);
-    if constexpr (RHS) return;
+    if constexpr (LEAN) return;
     if (fl & F_MOM2) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
@@ -308,7 +311,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, uint32_t bx, uint32
         PK xi, pv, wv;
         if (need_xi) xi.load(X + row * a.ldx + col);
         if (fl & F_P) pv.load_nt(P + row * a.ldp + col);
-        if (fl & F_MOM) wv.load_nt(W + row * a.ldw + col);
+        if (fl & (std::is_same<MR, float>::value ? (F_MOM | F_RHS) : F_MOM)) wv.load_nt(W + row * a.ldw + col);
         cx<AT> y[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) y[c] = {AT(0), AT(0)};
@@ -369,7 +372,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, uint32_t bx, uint32
 #pragma unroll
             for (int c = 0; c < CPL; ++c) y[c] = zv.template geta<AT>(c);
         }
-        step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
+        step_epilogue<VT, AT, CPL, std::is_same<MR, float>::value>(a, fl, row, col, y, xi, pv, wv, mom);
     }
 
     if (!(fl & (F_MOM | F_MOM2 | F_NORM))) return;
@@ -458,7 +461,7 @@ __device__ __forceinline__ void ell_step_body(const StepArgs &a, uint32_t bx, ui
         if (active) {
             if (need_xi) xi.load(X + row * a.ldx + col);
             if (fl & F_P) pv.load_nt(P + row * a.ldp + col);
-            if (fl & F_MOM) wv.load_nt(W + row * a.ldw + col);
+            if (fl & (std::is_same<MR, float>::value ? (F_MOM | F_RHS) : F_MOM)) wv.load_nt(W + row * a.ldw + col);
 #pragma unroll
             for (int e = 0; e < EW; ++e) xg[e].load(X + int64_t(jj[e]) * a.ldx + col);
         }
@@ -482,7 +485,7 @@ __device__ __forceinline__ void ell_step_body(const StepArgs &a, uint32_t bx, ui
                 for (int e = 0; e < EW; ++e) MV::fma(y[c], vv[e], xg[e].template geta<AT>(c));
                 if (diag) MV::fma(y[c], dd, xi.template geta<AT>(c));
             }
-            step_epilogue<VT, AT, CPL>(a, fl, row, col, y, xi, pv, wv, mom);
+            step_epilogue<VT, AT, CPL, std::is_same<MR, float>::value && !VO>(a, fl, row, col, y, xi, pv, wv, mom);
             if constexpr (VO) {
                 Pack<VT, CPL> vs;
 #pragma unroll
@@ -928,7 +931,7 @@ __device__ __forceinline__ void bsr4_mfma_body(const StepArgs &a) {
             }
             if (fl & F_P) pv.load_nt(P + (row0 + r) * a.ldp + col);
             if (fl & (BLK ? F_RHS : F_MOM)) wv.load_nt(W + (row0 + r) * a.ldw + col);
-            step_epilogue<float, double, CPL, BLK>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
+            step_epilogue<float, double, CPL, BLK, BLK>(a, fl, row0 + r, col, y, xi, pv, wv, mom);
         }
     }
     if constexpr (BLK) return;

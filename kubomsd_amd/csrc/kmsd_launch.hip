@@ -564,8 +564,8 @@ void prepare_step(kmsd_ctx *c, int dtype, const Step &s, StepArgs &a, Shape &sh,
     if (s.use_p) fl |= F_P;
     if (s.OUT) fl |= F_STORE;
     if (s.rhs) {
-        if (!s.bfac32 || !s.W || s.ACC)
-            throw Error(KMSD_EDEVICE, "internal error: a right-hand-side operand needs a block-Jacobi product");
+        if ((bsr && !s.bfac32) || (!bsr && !f32) || !s.W || s.ACC || s.doubling || s.norm || s.VOUT)
+            throw Error(KMSD_EDEVICE, "internal error: a right-hand-side operand needs a CSR/ELL or block-Jacobi step");
         fl |= F_RHS;
     } else if (s.W) {
         fl |= F_MOM;

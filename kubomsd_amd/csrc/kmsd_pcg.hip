@@ -600,70 +600,63 @@ bool cheb_usable(kmsd_metric *m) {
 // rho^(K+1)/(1 - rho) |r| sets the term count for the requested residual
 // reduction.  r is overwritten (it holds T_(k-1) in turn).  Returns the
 // number of products.
-// The first term's coefficient g_0 (1 / c where the interval is one point).
-double cheb_first(const kmsd_metric *m) {
-    const double lo = m->cheb_lo, hi = m->cheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
-    if (!(h > 1e-14 * cc)) return 1.0 / cc;
-    const double z = cc / h, sq = std::sqrt(z * z - 1.0);
-    return 0.5 * (2.0 / (h * sq));
+// Point Chebyshev semi-iteration for S d = r on S's Gershgorin interval
+// [cheb_lo, cheb_hi] (theta, delta its center and half width, sigma =
+// theta / delta): d_1 = r / theta, d_(k+1) = omega_(k+1) (d_k + (r - S d_k) /
+// theta) + (1 - omega_(k+1)) d_(k-1) with omega_2 = 1 / (1 - 1 / (2 sigma^2)),
+// omega_(k+1) = 1 / (1 - omega_k / (4 sigma^2)); the residual after K steps is
+// T_K((theta - S) / delta) / T_K(sigma) r, so K is the least count with
+// 1 / T_K(sigma) <= reduction (1 where the interval is one point: d_1 is
+// then the solution).
+int cheb_steps(const kmsd_metric *m, double reduction) {
+    const double theta = 0.5 * (m->cheb_hi + m->cheb_lo), delta = 0.5 * (m->cheb_hi - m->cheb_lo);
+    if (!(delta > 1e-14 * theta)) return 1;
+    return std::max(1, std::min(400, int(std::ceil(std::acosh(1.0 / reduction) / std::acosh(theta / delta)))));
 }
 
-int cheb_run(kmsd_metric *m, void *r, void *d, int32_t nb, double reduction, bool d_ready) {
+// The first step's coefficient: d_1 = r / theta.
+double cheb_first(const kmsd_metric *m) { return 1.0 / (0.5 * (m->cheb_hi + m->cheb_lo)); }
+
+// Steps 2..K of the point semi-iteration, d_1 already stored in d when K is
+// odd and in t when K is even, so that d_K lands in d; each step is one
+// fused product (S d_k, the own row d_k, d_(k-1) and r in the epilogue),
+// d_(k+1) replacing d_(k-1) row by row.  r is kept.  Returns the number of
+// products (K - 1).
+int cheb_run(kmsd_metric *m, const void *r, void *d, void *t, int32_t nb, int K) {
     kmsd_ctx *c = m->ctx;
-    const int64_t n = m->n;
-    const double lo = m->cheb_lo, hi = m->cheb_hi, cc = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
-    if (!(h > 1e-14 * cc)) {
-        if (!d_ready) launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{cheb_first(m), 0.0});  // S = c I
-        return 0;
-    }
-    const double z = cc / h, sq = std::sqrt(z * z - 1.0), rho = z - sq, amp = 2.0 / (h * sq);
-    int K = 1;
-    while (K < 400 && hi * amp * std::pow(rho, double(K + 1)) / (1.0 - rho) > reduction) ++K;
-    void *t1 = m->work->t.ensure(c, size_t(n) * size_t(nb) * 8);
-    if (!d_ready) launch_scale(c, KMSD_C64, r, nb, d, nb, n, nb, Cplx{cheb_first(m), 0.0});
-    double g = -amp * rho;
+    const double theta = 0.5 * (m->cheb_hi + m->cheb_lo), delta = 0.5 * (m->cheb_hi - m->cheb_lo);
+    const double sigma = theta / delta;
+    void *cur = (K % 2) ? d : t, *prev = (K % 2) ? t : d;
     Step s;
     s.A = m->S;
-    s.X = r;
     s.ldx = nb;
-    s.alpha = Cplx{1.0 / h, 0.0};
     s.use_beta = true;
-    s.beta = Cplx{-cc / h, 0.0};
-    s.OUT = t1;
+    s.W = r;
+    s.ldw = nb;
+    s.rhs = true;
+    s.ldp = nb;
     s.ldo = nb;
-    s.ACC = d;
-    s.lda = nb;
-    s.coef = Cplx{g, 0.0};
-    s.n = n;
+    s.n = m->n;
     s.w = nb;
     s.f32_matrix = true;
-    launch_step(c, KMSD_C64, s);
-    void *prev = r, *cur = t1;
+    double omega = 1.0;
     for (int k = 1; k < K; ++k) {
-        g *= -rho;
-        Step u;
-        u.A = m->S;
+        omega = k == 1 ? 1.0 / (1.0 - 1.0 / (2.0 * sigma * sigma)) : 1.0 / (1.0 - omega / (4.0 * sigma * sigma));
+        Step u = s;
         u.X = cur;
-        u.ldx = nb;
-        u.alpha = Cplx{2.0 / h, 0.0};
-        u.use_beta = true;
-        u.beta = Cplx{-2.0 * cc / h, 0.0};
-        u.P = prev;
-        u.ldp = nb;
-        u.use_p = true;
-        u.gamma = -1.0;
-        u.OUT = prev;  // T_(k+1) replaces T_(k-1) row by row (each row read before it is written)
-        u.ldo = nb;
-        u.ACC = d;
-        u.lda = nb;
-        u.coef = Cplx{g, 0.0};
-        u.n = n;
-        u.w = nb;
-        u.f32_matrix = true;
+        u.alpha = Cplx{-omega / theta, 0.0};
+        u.beta = Cplx{omega, 0.0};
+        u.coef = Cplx{omega / theta, 0.0};
+        if (k > 1) {  // d_0 = 0
+            u.P = prev;
+            u.use_p = true;
+            u.gamma = 1.0 - omega;
+        }
+        u.OUT = prev;
         launch_step(c, KMSD_C64, u);
         std::swap(prev, cur);
     }
-    return K;
+    return K - 1;
 }
 
 // Whether the complex64 refinement may take block-Jacobi Chebyshev inner
@@ -908,17 +901,19 @@ void pcg_solve(kmsd_metric *m, int dtype, const void *B, int64_t ldb, void *Y, i
     // solve, 0 for PCG.
     auto round = [&](double inner_rtol) {
         const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n * nb + 255) / 256, 8192));
-        // point series: the conversion also writes the series' first term
+        // point semi-iteration: the conversion also writes its first step
         const bool point = cheb_usable(m);
+        const int Kp = point ? cheb_steps(m, inner_rtol) : 0;
+        void *t64 = point ? wk.t.ensure(c, b64) : nullptr;
         hipLaunchKernelGGL((scaled_convert_kernel<double, float>), dim3(unsigned(g)), dim3(256), 0, c->stream,
-                           rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0, point ? d64 : nullptr,
-                           point ? cheb_first(m) : 0.0);
+                           rounds ? rr : bb, int64_t(nb), r64, int64_t(nb), n, nb, cs2, 0,
+                           point ? ((Kp % 2) ? d64 : t64) : nullptr, point ? cheb_first(m) : 0.0);
         HIPCHECK(hipGetLastError());
         Outcome o;
         int series = 0;
         if (point) {
             series = 1;
-            const int K = cheb_run(m, r64, d64, nb, inner_rtol, true);
+            const int K = cheb_run(m, r64, d64, t64, nb, Kp);
             o.cs.assign(size_t(nb), ColState{});
             for (ColState &x : o.cs) {
                 x.iters = K;
