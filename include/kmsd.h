@@ -123,9 +123,9 @@ int kmsd_ctx_transfers(const kmsd_ctx *ctx, uint64_t *out3);
  * both change the partial-sum partition, so sums round differently),
  * "cheb_inner" (1, the default: complex64 metric solves whose S is CSR/ELL
  * with the diagonal or no preconditioner and has a Gershgorin interval
- * [lo, hi] with lo > 0 and hi <= 4 lo take Chebyshev-series inner solves
- * inside the FP64 refinement, one fused product per term and no
- * reductions; uniform 4x4 block-Jacobi on a 4x4-block S: the first inner
+ * [lo, hi] with lo > 0 and hi <= 4 lo take Chebyshev semi-iteration inner
+ * solves on that interval inside the FP64 refinement, one fused product per
+ * step and no reductions; uniform 4x4 block-Jacobi on a 4x4-block S: the first inner
  * solve is PCG, whose CG coefficients give Ritz bounds for the block-Jacobi
  * Chebyshev semi-iteration in the later ones; the acceptance stays the FP64 true residual, and
  * a series round that reduces some column's residual by less than the square

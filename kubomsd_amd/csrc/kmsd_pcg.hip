@@ -590,16 +590,6 @@ bool cheb_usable(kmsd_metric *m) {
     return m->cheb_state == 1;
 }
 
-// Inner solve of the FP64 refinement by a Chebyshev series of S^-1 on S's
-// Gershgorin interval [lo, hi] (center c, half width h): d = sum_k g_k
-// T_k((S - c)/h) r with g_0 = 1/(h s), g_k = 2 (-rho)^k/(h s), z = c/h,
-// s = sqrt(z^2 - 1), rho = z - s (the expansion of 1/(z + t)).  Each term
-// is one fused step: T_(k+1) = (2/h) S T_k - (2c/h) T_k - T_(k-1) with
-// d += g_(k+1) T_(k+1) accumulated in the same launch; no reductions and no
-// host round trips.  The tail bound |r - S d_K| <= hi (2/(h s))
-// rho^(K+1)/(1 - rho) |r| sets the term count for the requested residual
-// reduction.  r is overwritten (it holds T_(k-1) in turn).  Returns the
-// number of products.
 // Point Chebyshev semi-iteration for S d = r on S's Gershgorin interval
 // [cheb_lo, cheb_hi] (theta, delta its center and half width, sigma =
 // theta / delta): d_1 = r / theta, d_(k+1) = omega_(k+1) (d_k + (r - S d_k) /

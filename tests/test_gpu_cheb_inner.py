@@ -1,8 +1,8 @@
 """Chebyshev inner solves of the complex64 metric refinement (tuning key
 "cheb_inner", on by default).  Where S is stored as CSR/ELL with the
 diagonal or no preconditioner and its Gershgorin interval [lo, hi] has
-lo > 0 and hi <= 4 lo, each FP32 inner solve is a Chebyshev series of S^-1
-on that interval, one fused step per term with no reductions; the FP64
+lo > 0 and hi <= 4 lo, each FP32 inner solve is the Chebyshev semi-iteration
+on that interval, one fused step per product with no reductions; the FP64
 refinement accepts on the FP64 true residual exactly as with PCG inner
 solves.  Checked against SciPy's solve of the complex64-rounded right-hand
 side (one complex64 rounding of the output), against the PCG inner solves
